@@ -1,0 +1,268 @@
+/*
+ * pyrit.h - C ABI of the MI355X-native Pyrit hot path (libpyrit.so).
+ *
+ * This is the subset of the reference's public header (include/pyrit.h of
+ * SimonVitzethum/Pyrit) that the GPU sparse-voxel-DAG build and the software
+ * DAG traversal sit behind. Struct layouts, constants, names and argument
+ * meanings are kept identical so a caller of the reference links against this
+ * library unchanged; each block cites the reference lines it mirrors. The
+ * implementation is HIP on gfx950; "cuda" in two legacy names below means the
+ * HIP stream of the context.
+ *
+ * Context functions are not thread-safe (reference pyrit.h:27-28). All GPU
+ * work is asynchronous on the context stream (pyr_cuda_stream).
+ */
+#ifndef PYRIT_H
+#define PYRIT_H
+
+#include <stddef.h>
+#if defined(__cplusplus)
+#include <cstdint>
+#else
+#include <stdint.h>
+#endif
+
+#define PYR_FLT_MAX 3.402823466e+38f
+
+/* ---- rays and hits (reference pyrit.h:46-91) ----------------------------- */
+
+typedef struct PyrRay {
+    float origin[3];
+    float tmin;
+    float direction[3]; /* need not be normalised; t is measured in its units */
+    float tmax;
+} PyrRay;
+
+#define PYR_NO_HIT 0xFFFFFFFFu
+
+/* face of the hit voxel in object space */
+#define PYR_FACE_POS_X 0u
+#define PYR_FACE_NEG_X 1u
+#define PYR_FACE_POS_Y 2u
+#define PYR_FACE_NEG_Y 3u
+#define PYR_FACE_POS_Z 4u
+#define PYR_FACE_NEG_Z 5u
+
+#define PYR_HIT_FACE_MASK 0x7u
+#define PYR_HIT_NEW 0x8u
+#define PYR_HIT_NO_HISTORY 0x10u
+#define PYR_HIT_INSIDE 0x20u /* ray starts inside a filled voxel: t = tmin */
+
+typedef struct PyrHit {
+    float t;            /* PYR_FLT_MAX on a miss */
+    uint32_t instance;  /* instance index or PYR_NO_HIT */
+    uint32_t attribute; /* voxel attribute */
+    uint32_t meta;      /* face (PYR_HIT_FACE_MASK) | PYR_HIT_* */
+} PyrHit;
+
+#define PYR_TRACE_ANY_HIT 0x1u
+#define PYR_TRACE_NO_ATTRIBUTE 0x2u
+#define PYR_TRACE_EXTENDED 0x4u
+#define PYR_TRACE_SKIP_TRANSPARENT 0x8u
+
+/* ---- camera (reference pyrit.h:96-113) ----------------------------------- */
+
+#define PYR_PROJECTION_PERSPECTIVE 0u
+#define PYR_PROJECTION_ORTHOGRAPHIC 1u
+
+typedef struct PyrCamera {
+    float view_to_world[12]; /* 3x4 row-major, column 3 = position; looks along -Z, +Y up */
+    uint32_t projection;
+    uint32_t width, height;
+    float scale[2];  /* perspective: tan(fov_x/2), tan(fov_y/2) */
+    float shift[2];  /* lens shift in NDC units */
+    float jitter[2]; /* sub-pixel offset in pixels */
+    float near_plane;
+    float far_plane; /* 0 = infinite */
+} PyrCamera;
+
+/* ---- scene on the device (reference pyrit.h:119-160) --------------------- */
+
+#define PYR_GEOMETRY_HAS_ATTRIBUTES 0x1u
+#define PYR_GEOMETRY_PALETTE 0x2u
+
+typedef struct PyrGeometryData {
+    uint32_t node_offset, leaf_offset, attribute_offset, root;
+    uint32_t log2_size, flags, default_attribute;
+    uint32_t palette_offset;
+} PyrGeometryData;
+
+#define PYR_INSTANCE_ACTIVE 0x1u
+
+typedef struct PyrInstanceData {
+    float object_to_world[12];
+    float world_to_object[12];
+    float bounds_min[3];
+    uint32_t geometry;
+    float bounds_max[3];
+    uint32_t mask;
+    uint32_t user;
+    uint32_t history;
+    uint32_t flags;
+    uint32_t reserved;
+} PyrInstanceData;
+
+typedef struct PyrScene {
+    uint64_t nodes, leaves, attributes, geometries; /* device pointers */
+    uint64_t instances, instances_prev;
+    uint32_t instance_count, geometry_count;
+    uint64_t frame;
+    double time, time_prev;
+    double origin[3], origin_prev[3];
+    uint64_t materials;
+    uint64_t lighting;
+    uint64_t transparent_materials[4];
+    uint64_t cutout_materials[4];
+    uint64_t textures;
+    uint32_t texture_count;
+    uint32_t reserved_tex;
+} PyrScene;
+
+/* ---- results (reference pyrit.h:320-353) --------------------------------- */
+
+#define PYR_VERSION_MAJOR 0
+#define PYR_VERSION_MINOR 2
+#define PYR_VERSION ((uint32_t)((PYR_VERSION_MAJOR << 16) | PYR_VERSION_MINOR))
+
+#define PYR_API __attribute__((visibility("default")))
+
+#define PYR_DEFINE_HANDLE(name) typedef struct name##_T* name
+
+typedef struct PyrContext PyrContext;
+PYR_DEFINE_HANDLE(PyrGeometry);
+PYR_DEFINE_HANDLE(PyrInstance);
+
+typedef int32_t PyrResult;
+#define PYR_OK 0
+#define PYR_ERROR_INVALID_ARGUMENT -1
+#define PYR_ERROR_INVALID_HANDLE -2
+#define PYR_ERROR_OUT_OF_MEMORY -3
+#define PYR_ERROR_CAPACITY -4
+#define PYR_ERROR_IN_USE -5
+#define PYR_ERROR_CUDA -6 /* a HIP runtime call failed */
+#define PYR_ERROR_COMPILE -7
+#define PYR_ERROR_NOT_FOUND -8
+#define PYR_ERROR_VERSION -9
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+PYR_API const char* pyr_result_string(PyrResult result);
+/* message of the last error on this thread (cleared at each API entry) */
+PYR_API const char* pyr_error_message(void);
+
+/* ---- context (reference pyrit.h:355-407) --------------------------------- */
+
+#define PYR_CREATE_DEBUG 0x1u
+#define PYR_CREATE_NO_RT 0x2u
+#define PYR_CREATE_FORCE_RT 0x4u
+
+typedef void (*PyrLogFn)(void* user, int32_t level, const char* message);
+
+typedef struct PyrCreateInfo {
+    uint32_t struct_size; /* sizeof(PyrCreateInfo) */
+    uint32_t version;     /* PYR_VERSION */
+    int32_t device;       /* HIP device index */
+    uint32_t flags;       /* PYR_CREATE_* */
+    void* cuda_context;   /* ignored (HIP has no explicit contexts) */
+    void* cuda_stream;    /* existing hipStream_t or NULL (own stream) */
+    uint32_t max_instances;  /* 0 = 65536 */
+    uint32_t max_geometries; /* 0 = 4096 */
+    uint32_t max_views;      /* 0 = 16 */
+    uint32_t max_textures;   /* 0 = 256 */
+    uint32_t rt_leaf_log2;   /* unused on this backend (no RT cores) */
+    uint64_t node_pool_bytes;      /* 0 = 256 MiB */
+    uint64_t leaf_pool_bytes;      /* 0 = 256 MiB */
+    uint64_t attribute_pool_bytes; /* 0 = 256 MiB */
+    uint64_t staging_bytes;        /* unused */
+    PyrLogFn log;
+    void* log_user;
+} PyrCreateInfo;
+
+PYR_API PyrResult pyr_create(const PyrCreateInfo* info, PyrContext** out);
+PYR_API void pyr_destroy(PyrContext* ctx); /* waits for the GPU */
+PYR_API void* pyr_cuda_stream(PyrContext* ctx); /* the context's hipStream_t */
+PYR_API uint32_t pyr_features(PyrContext* ctx); /* 0: no RT cores */
+PYR_API PyrResult pyr_synchronize(PyrContext* ctx);
+
+/* ---- voxels (reference pyrit.h:413-416) ---------------------------------- */
+
+typedef struct PyrVoxel {
+    int32_t x, y, z;
+    uint32_t attribute; /* != 0; 0 deletes in edits */
+} PyrVoxel;
+
+/* ---- geometry on the GPU (reference pyrit.h:454-480) --------------------- */
+
+#define PYR_BUILD_HOST_INPUT 0x1u /* voxels is a host pointer */
+#define PYR_BUILD_EDITABLE 0x2u   /* keep the sorted voxels: pyr_geometry_edit allowed */
+
+/* DAG from PyrVoxel[count] in any order (duplicates: last wins, attribute 0 =
+ * empty) built on the GPU; voxels is a device pointer unless HOST_INPUT. */
+PYR_API PyrResult pyr_geometry_build(PyrContext* ctx, uint32_t log2_size, const void* voxels, uint32_t count,
+                                     uint32_t flags, PyrGeometry* out);
+/* set or remove voxels of an editable geometry (rebuild on the GPU) */
+PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const void* voxels, uint32_t count,
+                                    uint32_t flags);
+PYR_API PyrResult pyr_geometry_destroy(PyrContext* ctx, PyrGeometry geometry); /* IN_USE while instanced */
+PYR_API uint32_t pyr_geometry_index(PyrGeometry geometry);
+
+/* ---- instances (reference pyrit.h:482-502) ------------------------------- */
+
+PYR_API PyrResult pyr_instance_create(PyrContext* ctx, PyrGeometry geometry, PyrInstance* out);
+PYR_API PyrResult pyr_instance_destroy(PyrContext* ctx, PyrInstance instance);
+PYR_API PyrResult pyr_instance_set_transform(PyrContext* ctx, PyrInstance instance, const float object_to_world[12]);
+PYR_API PyrResult pyr_instance_set_mask(PyrContext* ctx, PyrInstance instance, uint32_t mask); /* default 0xFF */
+PYR_API uint32_t pyr_instance_index(PyrInstance instance); /* = PyrHit.instance */
+
+/* ---- frames (reference pyrit.h:504-515) ---------------------------------- */
+
+typedef struct PyrFrameInfo {
+    double time;
+    double origin[3];
+} PyrFrameInfo;
+
+PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info);
+PYR_API uint64_t pyr_scene_device(PyrContext* ctx); /* device pointer to PyrScene */
+
+/* ---- rendering and tracing (reference pyrit.h:525-550) ------------------- */
+
+PYR_DEFINE_HANDLE(PyrView);
+
+typedef struct PyrTargets {
+    uint64_t hits;     /* PyrHit[width * height] or 0 */
+    uint64_t depth;    /* float[width * height]: linear depth along the view axis, PYR_FLT_MAX on a miss */
+    uint64_t motion;   /* not produced by this backend (must be 0) */
+    uint64_t color;    /* not produced (must be 0) */
+    uint64_t normal;   /* not produced (must be 0) */
+    uint64_t albedo;   /* not produced (must be 0) */
+    uint64_t material; /* not produced (must be 0) */
+    uint32_t ray_mask; /* 0 = all instances */
+    uint32_t flags;    /* PYR_TRACE_NO_ATTRIBUTE */
+    uint32_t transparent_mask;
+    uint32_t secondary_mask;
+    uint32_t coverage;
+    float detail_scale;
+} PyrTargets;
+
+PYR_API PyrResult pyr_view_create(PyrContext* ctx, PyrView* out);
+PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view);
+/* primary hits and depth of every pixel */
+PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* targets);
+
+/* arbitrary rays: rays and hits are device pointers (PyrRay[count], PyrHit[count]) */
+PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint32_t count, uint32_t ray_mask,
+                            uint32_t flags);
+
+/* ---- camera helpers (reference pyrit.h:917-921) -------------------------- */
+
+PYR_API void pyr_camera_look_at(PyrCamera* camera, const float eye[3], const float target[3], const float up[3]);
+PYR_API void pyr_camera_perspective(PyrCamera* camera, float fov_y_radians, uint32_t width, uint32_t height,
+                                    float near_plane);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PYRIT_H */

@@ -1,0 +1,131 @@
+/*
+ * pyrit_hip_build.h - the internal operator boundary of the GPU DAG build and
+ * the software traversal, as C-ABI entry points over device pointers.
+ *
+ * These replace the reference's build-side `gpu_build.Exec` vtable and its
+ * entry points (src/gpu_build.zig:14-23, 94-131) and the traversal launches
+ * pyr_k_trace / pyr_k_render (src/gpu_kernels.zig:63-85). Every kernel behind
+ * them is hand-written HIP for gfx950 (libpyrit.so).
+ *
+ * Conventions: device pointers are passed as uint64_t, streams as void*
+ * (hipStream_t). All work is enqueued on `stream`; calls that return sizes
+ * synchronise that stream once at the end (the reference reads its counters
+ * back once, src/gpu_build.zig:338-344). Results are PYR_* codes (pyrit.h).
+ */
+#ifndef PYRIT_HIP_BUILD_H
+#define PYRIT_HIP_BUILD_H
+
+#include <stdint.h>
+
+#include "pyrit.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Built DAG in the GPU-native layout (src/gpu_build.zig:25-47): nodes level by
+ * level bottom-up, word refs absolute, leaves in first-appearance Morton order,
+ * attributes in DFS rank order. All buffers are device memory owned by the
+ * caller (free with pyrhip_built_free). `keys`/`attrs` hold the sorted unique
+ * voxels (brick Morton << 6 | bit) for later edits. */
+typedef struct PyrhipBuilt {
+    uint32_t log2_size;
+    uint32_t root;
+    uint64_t nodes;
+    uint32_t node_words;
+    uint32_t leaf_count;
+    uint64_t leaves;
+    uint64_t keys;
+    uint64_t attrs;
+    uint32_t voxel_count;
+    uint32_t reserved;
+} PyrhipBuilt;
+
+/* One chunk batch (src/gpu_build.zig:98-117). Chunk c's voxels (PyrVoxel,
+ * chunk-local coordinates) are voxels[c*capacity + j] for
+ * j < offsets[c+1]-offsets[c] when capacity > 0 (the generator layout of
+ * pyrit.h:819-834), or voxels[offsets[c] .. offsets[c+1]) when capacity == 0
+ * (packed). offsets is a device array of count+1 u32, total = offsets[count].
+ * chunk_log2 (device u8[count], may be 0 = all log2_size) gives each chunk its
+ * own edge 2^chunk_log2[c] <= 2^log2_size: chunks of mixed LOD sizes build in
+ * one batch and each gets its root at its own top level. */
+typedef struct PyrhipChunkBatch {
+    uint32_t count;
+    uint32_t capacity;
+    uint64_t voxels;
+    uint64_t offsets;
+    uint32_t total;
+    uint32_t reserved;
+    uint64_t chunk_log2;
+} PyrhipChunkBatch;
+
+/* Tuning of one build; zeros select the defaults (8-bit digits). */
+typedef struct PyrhipBuildTuning {
+    uint32_t radix_bits;   /* 8 (default), 9, 10 or 11 */
+    uint32_t reserved[3];
+} PyrhipBuildTuning;
+
+/* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
+ * voxels (old_keys/old_vals of an earlier build, may be 0) plus n_new PyrVoxel
+ * records at `voxels`; later records win, attribute 0 deletes, out-of-range
+ * records are dropped. log2_size in [3, 20]. */
+PYR_API PyrResult pyrhip_build(uint32_t log2_size, uint64_t old_keys, uint64_t old_vals, uint32_t n_old,
+                               uint64_t voxels, uint32_t n_new, const PyrhipBuildTuning* tuning, void* stream,
+                               PyrhipBuilt* out);
+
+/* gpu_build.buildChunks (src/gpu_build.zig:119-122). All chunks share the
+ * node, leaf and attribute arrays; roots[c] / first_voxel[c] (host arrays of
+ * count) get each chunk's root word and first attribute rank, 0xFFFFFFFF for
+ * an empty chunk. */
+PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch* batch,
+                                      const PyrhipBuildTuning* tuning, void* stream, PyrhipBuilt* out,
+                                      uint32_t* roots, uint32_t* first_voxel);
+
+PYR_API void pyrhip_built_free(PyrhipBuilt* built, void* stream);
+
+/* Last build's kernel timings (device ms, from HIP events on the build
+ * stream) when timing was enabled with pyrhip_set_timing(1). */
+typedef struct PyrhipBuildTimes {
+    float encode_ms, sort_ms, unique_ms, leaves_ms, levels_ms, total_ms;
+    uint32_t sort_passes, radix_bits;
+} PyrhipBuildTimes;
+PYR_API void pyrhip_set_timing(int enable);
+PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out);
+
+/* One DAG in device memory for pyrhip_trace_dag (a single instance with the
+ * identity transform; src/device/dag.zig:24-36 Dag). */
+typedef struct PyrhipDag {
+    uint64_t nodes;
+    uint64_t leaves;
+    uint64_t attrs;        /* 0: every hit reports default_attribute */
+    uint32_t root;
+    uint32_t log2_size;
+    uint32_t default_attribute;
+    uint32_t reserved;
+} PyrhipDag;
+
+/* PyrRay[count] -> PyrHit[count] (device pointers) against one DAG, the
+ * traversal of src/device/dag.zig:160-351 plus toHit (src/device/trace.zig:
+ * 115-118) with instance 0. flags: PYR_TRACE_ANY_HIT / PYR_TRACE_NO_ATTRIBUTE. */
+PYR_API PyrResult pyrhip_trace_dag(const PyrhipDag* dag, uint64_t rays, uint64_t hits, uint32_t count,
+                                   uint32_t flags, void* stream);
+
+/* Primary rays of a camera (src/device/render.zig:16-32, 133-141) traced
+ * against one DAG: hits[width*height], rays generated on the device. */
+PYR_API PyrResult pyrhip_trace_camera(const PyrhipDag* dag, const PyrCamera* camera, uint64_t hits,
+                                      uint32_t flags, void* stream);
+
+/* The primary rays of a camera, PyrRay[width*height] row-major (device). */
+PYR_API PyrResult pyrhip_camera_rays(const PyrCamera* camera, uint64_t rays, void* stream);
+
+/* Shadow rays: for every hit of a primary pass (rays[i], hits[i]), a ray from
+ * the hit point offset 1e-3 along the face normal toward `sun` (normalised),
+ * any-hit (src/device/shade.zig:626-629, 940). out[i] = 1 occluded, 0 lit,
+ * 0xFFFFFFFF for a primary miss. */
+PYR_API PyrResult pyrhip_trace_shadow(const PyrhipDag* dag, uint64_t rays, uint64_t hits, const float sun[3],
+                                      uint64_t out, uint32_t count, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,171 @@
+"""ctypes mirrors of the C ABI (include/pyrit.h, include/pyrit_hip_build.h,
+pyrit_amd/csrc/terrain.h) and numpy dtypes of the record arrays."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+u8, u32, i32, u64, f32, f64 = C.c_uint8, C.c_uint32, C.c_int32, C.c_uint64, C.c_float, C.c_double
+vp = C.c_void_p
+
+# ---- record dtypes (byte-identical to the C structs) ----------------------
+VOXEL = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("attribute", "<u4")])
+RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
+HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
+
+PYR_OK = 0
+PYR_FLT_MAX = np.float32(3.402823466e38)
+PYR_NO_HIT = 0xFFFFFFFF
+PYR_HIT_INSIDE = 0x20
+PYR_TRACE_ANY_HIT = 0x1
+PYR_TRACE_NO_ATTRIBUTE = 0x2
+PYR_BUILD_HOST_INPUT = 0x1
+PYR_BUILD_EDITABLE = 0x2
+PYR_VERSION = (0 << 16) | 2
+
+
+class PyrCamera(C.Structure):
+    _fields_ = [("view_to_world", f32 * 12), ("projection", u32), ("width", u32), ("height", u32),
+                ("scale", f32 * 2), ("shift", f32 * 2), ("jitter", f32 * 2), ("near_plane", f32),
+                ("far_plane", f32)]
+
+
+class PyrGeometryData(C.Structure):
+    _fields_ = [("node_offset", u32), ("leaf_offset", u32), ("attribute_offset", u32), ("root", u32),
+                ("log2_size", u32), ("flags", u32), ("default_attribute", u32), ("palette_offset", u32)]
+
+
+class PyrInstanceData(C.Structure):
+    _fields_ = [("object_to_world", f32 * 12), ("world_to_object", f32 * 12), ("bounds_min", f32 * 3),
+                ("geometry", u32), ("bounds_max", f32 * 3), ("mask", u32), ("user", u32), ("history", u32),
+                ("flags", u32), ("reserved", u32)]
+
+
+class PyrScene(C.Structure):
+    _fields_ = [("nodes", u64), ("leaves", u64), ("attributes", u64), ("geometries", u64), ("instances", u64),
+                ("instances_prev", u64), ("instance_count", u32), ("geometry_count", u32), ("frame", u64),
+                ("time", f64), ("time_prev", f64), ("origin", f64 * 3), ("origin_prev", f64 * 3),
+                ("materials", u64), ("lighting", u64), ("transparent_materials", u64 * 4),
+                ("cutout_materials", u64 * 4), ("textures", u64), ("texture_count", u32), ("reserved_tex", u32)]
+
+
+class PyrCreateInfo(C.Structure):
+    _fields_ = [("struct_size", u32), ("version", u32), ("device", i32), ("flags", u32), ("cuda_context", vp),
+                ("cuda_stream", vp), ("max_instances", u32), ("max_geometries", u32), ("max_views", u32),
+                ("max_textures", u32), ("rt_leaf_log2", u32), ("node_pool_bytes", u64), ("leaf_pool_bytes", u64),
+                ("attribute_pool_bytes", u64), ("staging_bytes", u64), ("log", vp), ("log_user", vp)]
+
+
+class PyrFrameInfo(C.Structure):
+    _fields_ = [("time", f64), ("origin", f64 * 3)]
+
+
+class PyrTargets(C.Structure):
+    _fields_ = [("hits", u64), ("depth", u64), ("motion", u64), ("color", u64), ("normal", u64), ("albedo", u64),
+                ("material", u64), ("ray_mask", u32), ("flags", u32), ("transparent_mask", u32),
+                ("secondary_mask", u32), ("coverage", u32), ("detail_scale", f32)]
+
+
+class PyrhipBuilt(C.Structure):
+    _fields_ = [("log2_size", u32), ("root", u32), ("nodes", u64), ("node_words", u32), ("leaf_count", u32),
+                ("leaves", u64), ("keys", u64), ("attrs", u64), ("voxel_count", u32), ("reserved", u32)]
+
+
+class PyrhipChunkBatch(C.Structure):
+    _fields_ = [("count", u32), ("capacity", u32), ("voxels", u64), ("offsets", u64), ("total", u32),
+                ("reserved", u32), ("chunk_log2", u64)]
+
+
+class PyrhipBuildTuning(C.Structure):
+    _fields_ = [("radix_bits", u32), ("reserved", u32 * 3)]
+
+
+class PyrhipBuildTimes(C.Structure):
+    _fields_ = [("encode_ms", f32), ("sort_ms", f32), ("unique_ms", f32), ("leaves_ms", f32), ("levels_ms", f32),
+                ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32)]
+
+
+class PyrhipDag(C.Structure):
+    _fields_ = [("nodes", u64), ("leaves", u64), ("attrs", u64), ("root", u32), ("log2_size", u32),
+                ("default_attribute", u32), ("reserved", u32)]
+
+
+class PyrTerrainCave(C.Structure):
+    _fields_ = [("x", i32), ("y", i32), ("z", i32), ("r", i32)]
+
+
+class PyrTerrainChunk(C.Structure):
+    _fields_ = [("origin", i32 * 3), ("log2_size", u32), ("seed", u32), ("mean", i32), ("amplitude", i32),
+                ("octaves", u32), ("wavelength_log2", u32), ("material_mode", u32), ("skin", u32),
+                ("water_level", i32), ("cave_first", u32), ("cave_count", u32), ("grid_log2", u32),
+                ("grid_dim", u32), ("grid_first", u32)]
+
+
+P = C.POINTER
+
+# name -> (restype, argtypes) of every C entry point of libpyrit.so
+FUNCTIONS = {
+    "pyr_result_string": (C.c_char_p, [i32]),
+    "pyr_error_message": (C.c_char_p, []),
+    "pyr_create": (i32, [P(PyrCreateInfo), P(vp)]),
+    "pyr_destroy": (None, [vp]),
+    "pyr_cuda_stream": (vp, [vp]),
+    "pyr_features": (u32, [vp]),
+    "pyr_synchronize": (i32, [vp]),
+    "pyr_geometry_build": (i32, [vp, u32, vp, u32, u32, P(vp)]),
+    "pyr_geometry_edit": (i32, [vp, vp, vp, u32, u32]),
+    "pyr_geometry_destroy": (i32, [vp, vp]),
+    "pyr_geometry_index": (u32, [vp]),
+    "pyr_instance_create": (i32, [vp, vp, P(vp)]),
+    "pyr_instance_destroy": (i32, [vp, vp]),
+    "pyr_instance_set_transform": (i32, [vp, vp, P(f32)]),
+    "pyr_instance_set_mask": (i32, [vp, vp, u32]),
+    "pyr_instance_index": (u32, [vp]),
+    "pyr_commit": (i32, [vp, P(PyrFrameInfo)]),
+    "pyr_scene_device": (u64, [vp]),
+    "pyr_view_create": (i32, [vp, P(vp)]),
+    "pyr_view_destroy": (i32, [vp, vp]),
+    "pyr_render": (i32, [vp, vp, P(PyrCamera), P(PyrTargets)]),
+    "pyr_trace": (i32, [vp, u64, u64, u32, u32, u32]),
+    "pyr_camera_look_at": (None, [P(PyrCamera), P(f32), P(f32), P(f32)]),
+    "pyr_camera_perspective": (None, [P(PyrCamera), f32, u32, u32, f32]),
+    "pyrhip_build": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
+    "pyrhip_build_chunks": (i32, [u32, P(PyrhipChunkBatch), P(PyrhipBuildTuning), vp, P(PyrhipBuilt), P(u32),
+                                  P(u32)]),
+    "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),
+    "pyrhip_set_timing": (None, [i32]),
+    "pyrhip_last_build_times": (None, [P(PyrhipBuildTimes)]),
+    "pyrhip_trace_dag": (i32, [P(PyrhipDag), u64, u64, u32, u32, vp]),
+    "pyrhip_trace_camera": (i32, [P(PyrhipDag), P(PyrCamera), u64, u32, vp]),
+    "pyrhip_camera_rays": (i32, [P(PyrCamera), u64, vp]),
+    "pyrhip_trace_shadow": (i32, [P(PyrhipDag), u64, u64, P(f32), u64, u32, vp]),
+}
+
+TERRAIN_FUNCTIONS = {
+    "pyrgen_generate_host": (i32, [P(PyrTerrainChunk), u32, P(PyrTerrainCave), P(u32), P(u32), P(u64), vp]),
+    "pyrgen_scratch_bytes": (u64, [u64]),
+    "pyrgen_height_host": (i32, [P(PyrTerrainChunk), i32, i32]),
+    "pyrgen_generate_device": (i32, [u64, u32, u64, u64, u64, u64, u64, u64, u64, u64, vp]),
+}
+
+
+def declare(lib: C.CDLL) -> None:
+    for name, (res, args) in FUNCTIONS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def declare_terrain(lib: C.CDLL) -> None:
+    for name, (res, args) in TERRAIN_FUNCTIONS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+
+
+def check(lib: C.CDLL, code: int, what: str = "") -> None:
+    if code != PYR_OK:
+        msg = lib.pyr_error_message().decode(errors="replace")
+        name = lib.pyr_result_string(code).decode()
+        raise RuntimeError(f"{what}: {name} ({code}): {msg}")

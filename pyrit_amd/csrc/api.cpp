@@ -1,0 +1,624 @@
+// api.cpp - the pyrit.h context layer on the HIP runtime: geometry pools,
+// GPU builds installed into them, instances, frame commit and the trace /
+// render entry points (the hot-path subset of the reference's
+// src/context.zig and src/root.zig exports).
+#include <hip/hip_runtime.h>
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <vector>
+
+#include "internal.h"
+#include "range_alloc.h"
+
+namespace pyr {
+
+thread_local char g_diag[512];
+
+PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t hits, uint32_t count,
+                             uint32_t ray_mask, uint32_t flags, hipStream_t s);
+PyrResult launch_render_scene(const PyrScene* scene_dev, const PyrCamera& cam, const float* w2v_dev, uint64_t hits,
+                              uint64_t depth, uint32_t ray_mask, uint32_t flags, hipStream_t s);
+
+namespace {
+
+struct Geometry {
+    bool live = false;
+    uint32_t gen = 0;
+    PyrGeometryData data{};
+    uint64_t node_words = 0, leaf_count = 0, attr_count = 0;
+    uint64_t* keys = nullptr;  // editable: sorted unique voxels (device)
+    uint32_t* vals = nullptr;
+    uint32_t voxel_count = 0;
+    uint32_t users = 0;  // instances referring to it
+};
+
+struct Instance {
+    bool live = false;
+    uint32_t gen = 0;
+    PyrInstanceData data{};
+};
+
+struct Pool {
+    void* base = nullptr;
+    uint64_t elem = 0;  // bytes per element
+    RangeAlloc ra{0};
+};
+
+}  // namespace
+
+// 3x4 affine inverse in f64 (the reference's xform.zig:8-34)
+bool inverse34(const float m[12], float out[12]) {
+    const double a[9] = {m[0], m[1], m[2], m[4], m[5], m[6], m[8], m[9], m[10]};
+    const double t[3] = {m[3], m[7], m[11]};
+    const double c00 = a[4] * a[8] - a[5] * a[7];
+    const double c01 = a[5] * a[6] - a[3] * a[8];
+    const double c02 = a[3] * a[7] - a[4] * a[6];
+    const double det = a[0] * c00 + a[1] * c01 + a[2] * c02;
+    if (!(fabs(det) > 1e-30) || !isfinite(det)) return false;
+    const double id = 1.0 / det;
+    const double r[9] = {
+        c00 * id, (a[2] * a[7] - a[1] * a[8]) * id, (a[1] * a[5] - a[2] * a[4]) * id,
+        c01 * id, (a[0] * a[8] - a[2] * a[6]) * id, (a[2] * a[3] - a[0] * a[5]) * id,
+        c02 * id, (a[1] * a[6] - a[0] * a[7]) * id, (a[0] * a[4] - a[1] * a[3]) * id,
+    };
+    for (int row = 0; row < 3; row++) {
+        const double r0 = r[row * 3], r1 = r[row * 3 + 1], r2 = r[row * 3 + 2];
+        out[row * 4 + 0] = (float)r0;
+        out[row * 4 + 1] = (float)r1;
+        out[row * 4 + 2] = (float)r2;
+        out[row * 4 + 3] = (float)(-(r0 * t[0] + r1 * t[1] + r2 * t[2]));
+    }
+    return true;
+}
+
+// world AABB of [0, size]^3 under m with a 1e-5 relative margin (xform.zig:37-58)
+void transformed_box(const float m[12], float size, float lo[3], float hi[3]) {
+    for (int row = 0; row < 3; row++) {
+        lo[row] = m[row * 4 + 3];
+        hi[row] = m[row * 4 + 3];
+        for (int col = 0; col < 3; col++) {
+            const float e = m[row * 4 + col] * size;
+            lo[row] += fminf(e, 0.0f);
+            hi[row] += fmaxf(e, 0.0f);
+        }
+    }
+    for (int i = 0; i < 3; i++) {
+        const float eps = 1e-5f * fmaxf(1.0f, fmaxf(fabsf(lo[i]), fabsf(hi[i])));
+        lo[i] -= eps;
+        hi[i] += eps;
+    }
+}
+
+}  // namespace pyr
+
+using namespace pyr;
+
+struct PyrContext {
+    int device = 0;
+    uint32_t flags = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    Pool nodes, leaves, attrs;
+    std::vector<Geometry> geos;
+    std::vector<Instance> insts;
+    uint32_t max_instances = 0, max_geometries = 0;
+    uint64_t frame = 0;
+    double time = 0, time_prev = 0;
+    double origin[3] = {0, 0, 0}, origin_prev[3] = {0, 0, 0};
+    // committed device state
+    PyrGeometryData* geo_dev = nullptr;
+    PyrInstanceData* inst_dev[2] = {nullptr, nullptr};  // current, previous
+    PyrScene* scene_dev = nullptr;
+    float* w2v_dev = nullptr;
+    uint32_t history = 0;
+    uint32_t views = 0;
+};
+
+namespace {
+
+// handles: bits 0..31 index + 1, bits 32..63 generation (context.zig:36-43)
+inline uint64_t make_handle(uint32_t index, uint32_t gen) { return ((uint64_t)gen << 32) | (uint64_t)(index + 1); }
+inline uint32_t handle_index(uint64_t h) { return (uint32_t)(h & 0xFFFFFFFFu) - 1; }
+inline uint32_t handle_gen(uint64_t h) { return (uint32_t)(h >> 32); }
+
+Geometry* geo_of(PyrContext* ctx, PyrGeometry g) {
+    uint64_t h = (uint64_t)(uintptr_t)g;
+    uint32_t i = handle_index(h);
+    if (!h || i >= ctx->geos.size()) return nullptr;
+    Geometry& ge = ctx->geos[i];
+    return ge.live && ge.gen == handle_gen(h) ? &ge : nullptr;
+}
+
+Instance* inst_of(PyrContext* ctx, PyrInstance in) {
+    uint64_t h = (uint64_t)(uintptr_t)in;
+    uint32_t i = handle_index(h);
+    if (!h || i >= ctx->insts.size()) return nullptr;
+    Instance& it = ctx->insts[i];
+    return it.live && it.gen == handle_gen(h) ? &it : nullptr;
+}
+
+PyrResult hip_check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return PYR_OK;
+    hipGetLastError();
+    return fail(PYR_ERROR_CUDA, "%s: %s", what, hipGetErrorString(e));
+}
+
+PyrResult pool_init(Pool& p, uint64_t bytes, uint64_t elem) {
+    p.elem = elem;
+    p.ra = RangeAlloc(bytes / elem);
+    return hip_check(hipMalloc(&p.base, bytes), "pool allocation");
+}
+
+void release_geometry_ranges(PyrContext* ctx, Geometry& g) {
+    ctx->nodes.ra.release(g.data.node_offset, g.node_words);
+    ctx->leaves.ra.release(g.data.leaf_offset, g.leaf_count);
+    ctx->attrs.ra.release(g.data.attribute_offset, g.attr_count);
+    g.node_words = g.leaf_count = g.attr_count = 0;
+}
+
+// copy a finished build into the pools (context.zig:930-952, installBuilt)
+PyrResult install(PyrContext* ctx, Geometry& g, const PyrhipBuilt& b, bool editable) {
+    int64_t no = ctx->nodes.ra.alloc(b.node_words);
+    int64_t lo = ctx->leaves.ra.alloc(b.leaf_count);
+    int64_t ao = ctx->attrs.ra.alloc(b.voxel_count);
+    if (no < 0 || lo < 0 || ao < 0) {
+        if (no >= 0) ctx->nodes.ra.release((uint64_t)no, b.node_words);
+        if (lo >= 0) ctx->leaves.ra.release((uint64_t)lo, b.leaf_count);
+        if (ao >= 0) ctx->attrs.ra.release((uint64_t)ao, b.voxel_count);
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "geometry pool full (%u words, %u leaves, %u attributes)", b.node_words,
+                    b.leaf_count, b.voxel_count);
+    }
+    hipStream_t s = ctx->stream;
+    PyrResult r = hip_check(hipMemcpyAsync((char*)ctx->nodes.base + (uint64_t)no * 4, (void*)(uintptr_t)b.nodes,
+                                           (uint64_t)b.node_words * 4, hipMemcpyDeviceToDevice, s),
+                            "node copy");
+    if (r == PYR_OK)
+        r = hip_check(hipMemcpyAsync((char*)ctx->leaves.base + (uint64_t)lo * 8, (void*)(uintptr_t)b.leaves,
+                                     (uint64_t)b.leaf_count * 8, hipMemcpyDeviceToDevice, s),
+                      "leaf copy");
+    if (r == PYR_OK)
+        r = hip_check(hipMemcpyAsync((char*)ctx->attrs.base + (uint64_t)ao * 4, (void*)(uintptr_t)b.attrs,
+                                     (uint64_t)b.voxel_count * 4, hipMemcpyDeviceToDevice, s),
+                      "attribute copy");
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "install");
+    if (r != PYR_OK) {
+        ctx->nodes.ra.release((uint64_t)no, b.node_words);
+        ctx->leaves.ra.release((uint64_t)lo, b.leaf_count);
+        ctx->attrs.ra.release((uint64_t)ao, b.voxel_count);
+        return r;
+    }
+    g.data.node_offset = (uint32_t)no;
+    g.data.leaf_offset = (uint32_t)lo;
+    g.data.attribute_offset = (uint32_t)ao;
+    g.data.root = b.root;
+    g.data.log2_size = b.log2_size;
+    g.data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;
+    g.data.default_attribute = 1;
+    g.data.palette_offset = 0;
+    g.node_words = b.node_words;
+    g.leaf_count = b.leaf_count;
+    g.attr_count = b.voxel_count;
+    g.voxel_count = b.voxel_count;
+    if (editable) {
+        g.keys = (uint64_t*)(uintptr_t)b.keys;
+        g.vals = (uint32_t*)(uintptr_t)b.attrs;
+    } else {
+        (void)hipFree((void*)(uintptr_t)b.keys);
+        (void)hipFree((void*)(uintptr_t)b.attrs);
+    }
+    (void)hipFree((void*)(uintptr_t)b.nodes);
+    (void)hipFree((void*)(uintptr_t)b.leaves);
+    return PYR_OK;
+}
+
+// device copy of host voxels when PYR_BUILD_HOST_INPUT is set
+PyrResult stage_voxels(PyrContext* ctx, const void* voxels, uint32_t count, uint32_t flags, void** dev, bool* owned) {
+    *owned = false;
+    *dev = (void*)voxels;
+    if (!(flags & PYR_BUILD_HOST_INPUT) || count == 0) return PYR_OK;
+    PyrResult r = hip_check(hipMalloc(dev, (uint64_t)count * sizeof(PyrVoxel)), "voxel upload buffer");
+    if (r != PYR_OK) return r;
+    *owned = true;
+    return hip_check(hipMemcpyAsync(*dev, voxels, (uint64_t)count * sizeof(PyrVoxel), hipMemcpyHostToDevice,
+                                    ctx->stream),
+                     "voxel upload");
+}
+
+}  // namespace
+
+extern "C" {
+
+PYR_API const char* pyr_result_string(PyrResult r) {
+    switch (r) {
+        case PYR_OK: return "PYR_OK";
+        case PYR_ERROR_INVALID_ARGUMENT: return "PYR_ERROR_INVALID_ARGUMENT";
+        case PYR_ERROR_INVALID_HANDLE: return "PYR_ERROR_INVALID_HANDLE";
+        case PYR_ERROR_OUT_OF_MEMORY: return "PYR_ERROR_OUT_OF_MEMORY";
+        case PYR_ERROR_CAPACITY: return "PYR_ERROR_CAPACITY";
+        case PYR_ERROR_IN_USE: return "PYR_ERROR_IN_USE";
+        case PYR_ERROR_CUDA: return "PYR_ERROR_CUDA";
+        case PYR_ERROR_COMPILE: return "PYR_ERROR_COMPILE";
+        case PYR_ERROR_NOT_FOUND: return "PYR_ERROR_NOT_FOUND";
+        case PYR_ERROR_VERSION: return "PYR_ERROR_VERSION";
+        default: return "PYR_ERROR_UNKNOWN";
+    }
+}
+
+PYR_API const char* pyr_error_message(void) { return g_diag; }
+
+PYR_API PyrResult pyr_create(const PyrCreateInfo* info, PyrContext** out) {
+    diag_clear();
+    if (!info || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (info->struct_size != sizeof(PyrCreateInfo)) return fail(PYR_ERROR_VERSION, "PyrCreateInfo size mismatch");
+    if ((info->version >> 16) != PYR_VERSION_MAJOR) return fail(PYR_ERROR_VERSION, "incompatible PYR_VERSION");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        hipGetLastError();
+        return fail(PYR_ERROR_NOT_FOUND, "no HIP device");
+    }
+    if (info->device < 0 || info->device >= n) return fail(PYR_ERROR_INVALID_ARGUMENT, "device %d", info->device);
+    PyrContext* ctx = new PyrContext();
+    ctx->device = info->device;
+    ctx->flags = info->flags;
+    PyrResult r = hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    if (r == PYR_OK) {
+        if (info->cuda_stream) {
+            ctx->stream = (hipStream_t)info->cuda_stream;
+        } else {
+            r = hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "stream");
+            ctx->own_stream = r == PYR_OK;
+        }
+    }
+    ctx->max_instances = info->max_instances ? info->max_instances : 65536;
+    ctx->max_geometries = info->max_geometries ? info->max_geometries : 4096;
+    const uint64_t mib = 1ull << 20;
+    if (r == PYR_OK) r = pool_init(ctx->nodes, info->node_pool_bytes ? info->node_pool_bytes : 256 * mib, 4);
+    if (r == PYR_OK) r = pool_init(ctx->leaves, info->leaf_pool_bytes ? info->leaf_pool_bytes : 256 * mib, 8);
+    if (r == PYR_OK) r = pool_init(ctx->attrs, info->attribute_pool_bytes ? info->attribute_pool_bytes : 256 * mib, 4);
+    if (r == PYR_OK)
+        r = hip_check(hipMalloc(&ctx->geo_dev, (uint64_t)ctx->max_geometries * sizeof(PyrGeometryData)), "tables");
+    for (int q = 0; q < 2 && r == PYR_OK; q++)
+        r = hip_check(hipMalloc(&ctx->inst_dev[q], (uint64_t)ctx->max_instances * sizeof(PyrInstanceData)), "tables");
+    if (r == PYR_OK) r = hip_check(hipMalloc(&ctx->scene_dev, sizeof(PyrScene)), "scene");
+    if (r == PYR_OK) r = hip_check(hipMalloc(&ctx->w2v_dev, 64), "camera");
+    if (r == PYR_OK) r = hip_check(hipMemsetAsync(ctx->scene_dev, 0, sizeof(PyrScene), ctx->stream), "scene");
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(ctx->stream), "create");
+    if (r != PYR_OK) {
+        char msg[512];
+        memcpy(msg, g_diag, sizeof(msg));
+        pyr_destroy(ctx);
+        memcpy(g_diag, msg, sizeof(msg));
+        return r;
+    }
+    *out = ctx;
+    return pyr_commit(ctx, nullptr);
+}
+
+PYR_API void pyr_destroy(PyrContext* ctx) {
+    if (!ctx) return;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (Geometry& g : ctx->geos) {
+        if (!g.live) continue;
+        (void)hipFree(g.keys);
+        (void)hipFree(g.vals);
+    }
+    (void)hipFree(ctx->nodes.base);
+    (void)hipFree(ctx->leaves.base);
+    (void)hipFree(ctx->attrs.base);
+    (void)hipFree(ctx->geo_dev);
+    (void)hipFree(ctx->inst_dev[0]);
+    (void)hipFree(ctx->inst_dev[1]);
+    (void)hipFree(ctx->scene_dev);
+    (void)hipFree(ctx->w2v_dev);
+    if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+PYR_API void* pyr_cuda_stream(PyrContext* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+PYR_API uint32_t pyr_features(PyrContext* ctx) {
+    (void)ctx;
+    return 0;
+}
+
+PYR_API PyrResult pyr_synchronize(PyrContext* ctx) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    return hip_check(hipStreamSynchronize(ctx->stream), "synchronize");
+}
+
+// geometryBuild (context.zig:954-969)
+PYR_API PyrResult pyr_geometry_build(PyrContext* ctx, uint32_t log2_size, const void* voxels, uint32_t count,
+                                     uint32_t flags, PyrGeometry* out) {
+    diag_clear();
+    if (!ctx || !out || (count && !voxels)) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    uint32_t slot = 0;
+    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
+    void* dev = nullptr;
+    bool owned = false;
+    PyrResult r = stage_voxels(ctx, voxels, count, flags, &dev, &owned);
+    PyrhipBuilt b{};
+    if (r == PYR_OK) r = pyrhip_build(log2_size, 0, 0, 0, (uint64_t)(uintptr_t)dev, count, nullptr, ctx->stream, &b);
+    if (owned) (void)hipFree(dev);
+    if (r != PYR_OK) return r;
+    if (slot == ctx->geos.size()) ctx->geos.emplace_back();
+    Geometry& g = ctx->geos[slot];
+    uint32_t gen = g.gen + 1;
+    g = Geometry();
+    g.gen = gen;
+    r = install(ctx, g, b, (flags & PYR_BUILD_EDITABLE) != 0);
+    if (r != PYR_OK) {
+        pyrhip_built_free(&b, ctx->stream);
+        return r;
+    }
+    g.live = true;
+    *out = (PyrGeometry)(uintptr_t)make_handle(slot, g.gen);
+    return PYR_OK;
+}
+
+// geometryEdit (context.zig:973-1006): rebuild from the kept sorted voxels
+// plus the edits; the new version replaces the old in place
+PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const void* voxels, uint32_t count,
+                                    uint32_t flags) {
+    diag_clear();
+    if (!ctx || (count && !voxels)) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    Geometry* g = geo_of(ctx, geometry);
+    if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    if (!g->keys) return fail(PYR_ERROR_INVALID_ARGUMENT, "geometry was not built with PYR_BUILD_EDITABLE");
+    void* dev = nullptr;
+    bool owned = false;
+    PyrResult r = stage_voxels(ctx, voxels, count, flags, &dev, &owned);
+    PyrhipBuilt b{};
+    if (r == PYR_OK)
+        r = pyrhip_build(g->data.log2_size, (uint64_t)(uintptr_t)g->keys, (uint64_t)(uintptr_t)g->vals,
+                         g->voxel_count, (uint64_t)(uintptr_t)dev, count, nullptr, ctx->stream, &b);
+    if (owned) (void)hipFree(dev);
+    if (r != PYR_OK) return r;
+    Geometry old = *g;
+    release_geometry_ranges(ctx, *g);
+    r = install(ctx, *g, b, true);
+    if (r != PYR_OK) {
+        // pools stay as they were: put the old ranges back
+        *g = old;
+        ctx->nodes.ra.take(old.data.node_offset, old.node_words);
+        ctx->leaves.ra.take(old.data.leaf_offset, old.leaf_count);
+        ctx->attrs.ra.take(old.data.attribute_offset, old.attr_count);
+        pyrhip_built_free(&b, ctx->stream);
+        return r;
+    }
+    (void)hipFree(old.keys);
+    (void)hipFree(old.vals);
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_geometry_destroy(PyrContext* ctx, PyrGeometry geometry) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Geometry* g = geo_of(ctx, geometry);
+    if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    if (g->users) return fail(PYR_ERROR_IN_USE, "geometry still has %u instances", g->users);
+    (void)hipStreamSynchronize(ctx->stream);
+    release_geometry_ranges(ctx, *g);
+    (void)hipFree(g->keys);
+    (void)hipFree(g->vals);
+    g->keys = nullptr;
+    g->vals = nullptr;
+    g->live = false;
+    return PYR_OK;
+}
+
+PYR_API uint32_t pyr_geometry_index(PyrGeometry geometry) {
+    return geometry ? handle_index((uint64_t)(uintptr_t)geometry) : 0xFFFFFFFFu;
+}
+
+PYR_API PyrResult pyr_instance_create(PyrContext* ctx, PyrGeometry geometry, PyrInstance* out) {
+    diag_clear();
+    if (!ctx || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    Geometry* g = geo_of(ctx, geometry);
+    if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    uint32_t slot = 0;
+    while (slot < ctx->insts.size() && ctx->insts[slot].live) slot++;
+    if (slot >= ctx->max_instances) return fail(PYR_ERROR_CAPACITY, "max_instances reached");
+    if (slot == ctx->insts.size()) ctx->insts.emplace_back();
+    Instance& in = ctx->insts[slot];
+    uint32_t gen = in.gen + 1;
+    in = Instance();
+    in.gen = gen;
+    in.live = true;
+    PyrInstanceData& d = in.data;
+    static const float ident[12] = {1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0};
+    memcpy(d.object_to_world, ident, sizeof(ident));
+    memcpy(d.world_to_object, ident, sizeof(ident));
+    d.geometry = handle_index((uint64_t)(uintptr_t)geometry);
+    transformed_box(d.object_to_world, (float)(1u << g->data.log2_size), d.bounds_min, d.bounds_max);
+    d.mask = 0xFF;
+    d.history = ++ctx->history;
+    d.flags = PYR_INSTANCE_ACTIVE;
+    g->users++;
+    *out = (PyrInstance)(uintptr_t)make_handle(slot, in.gen);
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_instance_destroy(PyrContext* ctx, PyrInstance instance) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    Geometry& g = ctx->geos[in->data.geometry];
+    if (g.users) g.users--;
+    in->live = false;
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_instance_set_transform(PyrContext* ctx, PyrInstance instance, const float m[12]) {
+    diag_clear();
+    if (!ctx || !m) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    float inv[12];
+    if (!inverse34(m, inv)) return fail(PYR_ERROR_INVALID_ARGUMENT, "singular transform");
+    memcpy(in->data.object_to_world, m, 48);
+    memcpy(in->data.world_to_object, inv, 48);
+    const Geometry& g = ctx->geos[in->data.geometry];
+    transformed_box(m, (float)(1u << g.data.log2_size), in->data.bounds_min, in->data.bounds_max);
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_instance_set_mask(PyrContext* ctx, PyrInstance instance, uint32_t mask) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    in->data.mask = mask;
+    return PYR_OK;
+}
+
+PYR_API uint32_t pyr_instance_index(PyrInstance instance) {
+    return instance ? handle_index((uint64_t)(uintptr_t)instance) : 0xFFFFFFFFu;
+}
+
+// commit: upload geometry and instance tables and the scene block; the
+// previous frame's instances become instances_prev
+PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    hipStream_t s = ctx->stream;
+    std::swap(ctx->inst_dev[0], ctx->inst_dev[1]);
+    std::vector<PyrGeometryData> geo(ctx->geos.size());
+    for (size_t i = 0; i < geo.size(); i++) geo[i] = ctx->geos[i].data;
+    std::vector<PyrInstanceData> ins(ctx->insts.size());
+    for (size_t i = 0; i < ins.size(); i++) {
+        ins[i] = ctx->insts[i].data;
+        if (!ctx->insts[i].live) ins[i].flags = 0;
+    }
+    ctx->time_prev = ctx->time;
+    memcpy(ctx->origin_prev, ctx->origin, sizeof(ctx->origin));
+    if (info) {
+        ctx->time = info->time;
+        memcpy(ctx->origin, info->origin, sizeof(ctx->origin));
+    }
+    ctx->frame++;
+    PyrScene sc;
+    memset(&sc, 0, sizeof(sc));
+    sc.nodes = (uint64_t)(uintptr_t)ctx->nodes.base;
+    sc.leaves = (uint64_t)(uintptr_t)ctx->leaves.base;
+    sc.attributes = (uint64_t)(uintptr_t)ctx->attrs.base;
+    sc.geometries = (uint64_t)(uintptr_t)ctx->geo_dev;
+    sc.instances = (uint64_t)(uintptr_t)ctx->inst_dev[0];
+    sc.instances_prev = (uint64_t)(uintptr_t)ctx->inst_dev[1];
+    sc.instance_count = (uint32_t)ins.size();
+    sc.geometry_count = (uint32_t)geo.size();
+    sc.frame = ctx->frame;
+    sc.time = ctx->time;
+    sc.time_prev = ctx->time_prev;
+    memcpy(sc.origin, ctx->origin, sizeof(sc.origin));
+    memcpy(sc.origin_prev, ctx->origin_prev, sizeof(sc.origin_prev));
+    PyrResult r = PYR_OK;
+    if (!geo.empty())
+        r = hip_check(hipMemcpyAsync(ctx->geo_dev, geo.data(), geo.size() * sizeof(PyrGeometryData),
+                                     hipMemcpyHostToDevice, s),
+                      "geometry table");
+    if (r == PYR_OK && !ins.empty())
+        r = hip_check(hipMemcpyAsync(ctx->inst_dev[0], ins.data(), ins.size() * sizeof(PyrInstanceData),
+                                     hipMemcpyHostToDevice, s),
+                      "instance table");
+    if (r == PYR_OK)
+        r = hip_check(hipMemcpyAsync(ctx->scene_dev, &sc, sizeof(sc), hipMemcpyHostToDevice, s), "scene");
+    // host staging above is stack memory: finish the copies before returning
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "commit");
+    return r;
+}
+
+PYR_API uint64_t pyr_scene_device(PyrContext* ctx) { return ctx ? (uint64_t)(uintptr_t)ctx->scene_dev : 0; }
+
+// trace (context.zig:1869-1885); ray_mask 0 = all instances
+PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint32_t count, uint32_t ray_mask,
+                            uint32_t flags) {
+    diag_clear();
+    if (!ctx || (count && (!rays || !hits))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (flags & (PYR_TRACE_EXTENDED | PYR_TRACE_SKIP_TRANSPARENT))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "PYR_TRACE_EXTENDED / SKIP_TRANSPARENT are not supported");
+    PyrResult r = launch_trace_scene(ctx->scene_dev, rays, hits, count, ray_mask ? ray_mask : 0xFFFFFFFFu, flags,
+                                     ctx->stream);
+    if (r == PYR_OK && (ctx->flags & PYR_CREATE_DEBUG)) r = hip_check(hipStreamSynchronize(ctx->stream), "trace");
+    return r;
+}
+
+PYR_API PyrResult pyr_view_create(PyrContext* ctx, PyrView* out) {
+    diag_clear();
+    if (!ctx || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = (PyrView)(uintptr_t)make_handle(ctx->views++, 1);
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view) {
+    diag_clear();
+    if (!ctx || !view) return fail(PYR_ERROR_INVALID_HANDLE, "view");
+    return PYR_OK;
+}
+
+// render (context.zig:1766-1867), primary hits and linear depth only
+PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* t) {
+    diag_clear();
+    if (!ctx || !camera || !t) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!view) return fail(PYR_ERROR_INVALID_HANDLE, "view");
+    if (t->motion || t->color || t->normal || t->albedo || t->material)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "only hits and depth targets are produced by this backend");
+    if (camera->width == 0 || camera->height == 0) return PYR_OK;
+    float w2v[12];
+    if (!inverse34(camera->view_to_world, w2v)) return fail(PYR_ERROR_INVALID_ARGUMENT, "singular camera pose");
+    PyrResult r = hip_check(hipMemcpyAsync(ctx->w2v_dev, w2v, sizeof(w2v), hipMemcpyHostToDevice, ctx->stream),
+                            "camera upload");
+    if (r == PYR_OK)
+        r = launch_render_scene(ctx->scene_dev, *camera, ctx->w2v_dev, t->hits, t->depth,
+                                t->ray_mask ? t->ray_mask : 0xFFFFFFFFu, t->flags & PYR_TRACE_NO_ATTRIBUTE,
+                                ctx->stream);
+    // w2v is host stack memory
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(ctx->stream), "render");
+    return r;
+}
+
+// camera helpers (root.zig:449-481)
+static void normalize3(const float v[3], float r[3]) {
+    const float l = sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    r[0] = v[0] / l;
+    r[1] = v[1] / l;
+    r[2] = v[2] / l;
+}
+
+static void cross3(const float a[3], const float b[3], float r[3]) {
+    r[0] = a[1] * b[2] - a[2] * b[1];
+    r[1] = a[2] * b[0] - a[0] * b[2];
+    r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+PYR_API void pyr_camera_look_at(PyrCamera* c, const float eye[3], const float target[3], const float up[3]) {
+    if (!c || !eye || !target || !up) return;
+    const float b0[3] = {eye[0] - target[0], eye[1] - target[1], eye[2] - target[2]};
+    float back[3], r0[3], right[3], tu[3];
+    normalize3(b0, back);
+    cross3(up, back, r0);
+    normalize3(r0, right);
+    cross3(back, right, tu);
+    const float m[12] = {right[0], tu[0], back[0], eye[0], right[1], tu[1], back[1], eye[1],
+                         right[2], tu[2], back[2], eye[2]};
+    memcpy(c->view_to_world, m, sizeof(m));
+}
+
+PYR_API void pyr_camera_perspective(PyrCamera* c, float fov_y, uint32_t width, uint32_t height, float near_plane) {
+    if (!c) return;
+    const float ty = tanf(fov_y * 0.5f);
+    c->projection = PYR_PROJECTION_PERSPECTIVE;
+    c->width = width;
+    c->height = height;
+    c->scale[0] = ty * (float)width / (float)height;
+    c->scale[1] = ty;
+    c->near_plane = near_plane;
+}
+
+}  // extern "C"

@@ -1,0 +1,1119 @@
+// build.hip - GPU sparse-voxel-DAG build for gfx950.
+//
+// Produces exactly the arrays the reference GPU builder produces
+// (src/device/gbuild.zig, op order of src/gpu_build.zig:135-360): nodes level by
+// level bottom-up with absolute word refs, each level's nodes in the order of
+// their first appearance in Morton order, leaves in first-appearance order,
+// attributes in DFS rank order. The pipeline is this file's own:
+//
+//   encode      PyrVoxel -> (key, attr) + LDS histograms of every radix digit
+//   sort        P onesweep LSD passes: wave-ballot ranking, per-wave digit
+//               counters in LDS, decoupled look-back per digit, LDS-staged
+//               coalesced scatter
+//   unique      last record of a key wins, attribute 0 deletes (tile scan)
+//   bricks      brick heads (tile scan), 64-bit masks by wave-segmented OR
+//   leaves      open-addressing HBM hash, 64-bit CAS, owner = atomicMin index,
+//               compaction by tile scan
+//   levels      per level: parent tuples (tile scan), tuple hash (CAS of
+//               index + 1, word compare), owner = atomicMin, compaction of the
+//               owners' words by tile scan, pointer remap via the owners
+//   finalize    roots, empty geometry
+//
+// Every count lives on the device; kernels are launched with host upper bounds
+// and read the true counts, so one build is a fixed launch sequence with a
+// single read-back at the end (graph-capturable).
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "common.h"
+#include "internal.h"
+
+namespace pyr {
+
+// ---------------------------------------------------------------------------
+// counters
+// ---------------------------------------------------------------------------
+enum : uint32_t {
+    C_ERR = 0,     // non-zero: a bounded wait gave up or a table filled up
+    C_U = 1,       // unique voxels
+    C_B = 2,       // bricks
+    C_LEAVES = 3,  // distinct leaves
+    C_ROOT = 4,
+    C_WORDS = 5,
+    C_NCH = 8,     // + lv: elements of level lv (children of the level-lv parents)
+    C_CUR = 40,    // + lv: node-word cursor when level lv's parents are written
+    C_TILE = 72,   // tile counters, one per launch
+    C_COUNT = 256,
+};
+constexpr uint32_t kMaxLaunches = C_COUNT - C_TILE;
+
+constexpr int kSortItems = 16;                    // keys per thread in a sort tile
+constexpr int kSortTile = kBlock * kSortItems;    // 4096
+constexpr int kScanItems = 8;                     // elements per thread in a scan tile
+constexpr int kScanTile = kBlock * kScanItems;    // 2048
+constexpr int kNodeStride = 10;                   // words per parent in node_tmp (gbuild.zig:66)
+
+__host__ __device__ inline uint64_t hash_cap_for(uint64_t n) {
+    uint64_t want = n + n / 4, c = 1024;
+    while (c < want) c <<= 1;
+    return c;
+}
+
+// ---------------------------------------------------------------------------
+// input description shared by the encode kernel
+// ---------------------------------------------------------------------------
+struct EncodeArgs {
+    const uint64_t* old_keys;
+    const uint32_t* old_vals;
+    uint32_t n_old;
+    const int4* voxels;       // PyrVoxel records
+    uint32_t n_new;
+    uint32_t log2_size;
+    // chunk batches
+    uint32_t chunk_count;     // 0 = single build
+    uint32_t capacity;        // 0 = packed segments
+    const uint32_t* offsets;  // [chunk_count + 1]
+    const uint8_t* chunk_log2;
+    uint32_t chunk_bits;
+    // outputs
+    uint64_t* keys;
+    uint32_t* vals;
+    uint32_t* hist;           // [passes][radix]
+    uint32_t passes, radix_bits;
+};
+
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t i, uint64_t& key, uint32_t& val) {
+    const uint32_t lbits = local_bits(a.log2_size);
+    const uint64_t invalid = 1ull << (lbits + a.chunk_bits);
+    if (i < a.n_old) {
+        key = a.old_keys[i];
+        val = a.old_vals[i];
+        return;
+    }
+    uint32_t j = i - a.n_old;
+    key = invalid;
+    val = 0;
+    if (a.chunk_count == 0) {
+        // encode (gbuild.zig:184-199)
+        int4 v = a.voxels[j];
+        const int32_t n = 1 << a.log2_size;
+        if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
+            key = local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
+            val = (uint32_t)v.w;
+        }
+        return;
+    }
+    // pack_chunks (gbuild.zig:203-230): chunk by binary search in the prefixes
+    uint32_t lo = 0, hi = a.chunk_count;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) / 2;
+        if (a.offsets[mid] <= j) lo = mid; else hi = mid;
+    }
+    const uint32_t c = lo, local = j - a.offsets[c];
+    if (a.capacity && local >= a.capacity) return;
+    int4 v = a.voxels[a.capacity ? (uint64_t)c * a.capacity + local : (uint64_t)j];
+    const int32_t n = 1 << (a.chunk_log2 ? a.chunk_log2[c] : a.log2_size);
+    if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
+        key = ((uint64_t)c << lbits) | local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
+        val = (uint32_t)v.w;
+    }
+}
+
+// keys/vals of every record plus, per block, LDS histograms of all radix
+// digits flushed to the global histogram with one atomic per non-zero bin
+__global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, uint32_t n_total) {
+    extern __shared__ uint32_t s_hist[];
+    const uint32_t radix = 1u << a.radix_bits, bins = a.passes * radix;
+    for (uint32_t i = threadIdx.x; i < bins; i += kBlock) s_hist[i] = 0;
+    __syncthreads();
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_total; i += stride) {
+        uint64_t key;
+        uint32_t val;
+        encode_one(a, i, key, val);
+        a.keys[i] = key;
+        a.vals[i] = val;
+        for (uint32_t p = 0; p < a.passes; p++)
+            atomicAdd(&s_hist[p * radix + (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1))], 1u);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < bins; i += kBlock)
+        if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
+}
+
+// exclusive scan of each pass's digit histogram (one block)
+__global__ void __launch_bounds__(kBlock) k_digit_scan(const uint32_t* hist, uint32_t* base, uint32_t passes,
+                                                       uint32_t radix) {
+    __shared__ uint32_t s_part[kBlock];
+    for (uint32_t p = 0; p < passes; p++) {
+        const uint32_t per = radix / kBlock;
+        const uint32_t* h = hist + p * radix;
+        uint32_t sum = 0;
+        for (uint32_t k = 0; k < per; k++) sum += h[threadIdx.x * per + k];
+        s_part[threadIdx.x] = sum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t acc = 0;
+            for (int t = 0; t < kBlock; t++) {
+                uint32_t v = s_part[t];
+                s_part[t] = acc;
+                acc += v;
+            }
+        }
+        __syncthreads();
+        uint32_t acc = s_part[threadIdx.x];
+        for (uint32_t k = 0; k < per; k++) {
+            uint32_t d = threadIdx.x * per + k;
+            base[p * radix + d] = acc;
+            acc += h[d];
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// onesweep LSD pass
+// ---------------------------------------------------------------------------
+struct SortArgs {
+    const uint64_t* keys_in;
+    const uint32_t* vals_in;
+    uint64_t* keys_out;
+    uint32_t* vals_out;
+    uint32_t n;
+    uint32_t shift;
+    const uint32_t* digit_base;  // [radix] global exclusive offsets of this pass
+    uint64_t* status;            // [tiles][radix]
+    uint32_t epoch;
+    uint32_t* tile_ctr;
+    uint32_t* err;
+};
+
+template <int RB>
+__global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
+    constexpr uint32_t R = 1u << RB;
+    constexpr uint32_t DPT = R / kBlock;  // digits per thread
+    constexpr int W = kBlock / kWave;
+    __shared__ uint32_t s_tile;
+    __shared__ uint32_t s_wave_cnt[W][R];  // per-wave digit counters, then wave-exclusive prefixes
+    __shared__ uint32_t s_local[R];        // tile-local exclusive digit offsets
+    __shared__ uint32_t s_part[kBlock];
+    __shared__ uint64_t s_dst[R];          // global position of local slot 0 per digit
+    __shared__ uint64_t s_keys[kSortTile];
+    __shared__ uint32_t s_vals[kSortTile];
+
+    const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
+    for (uint32_t i = tid; i < W * R; i += kBlock) (&s_wave_cnt[0][0])[i] = 0;
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const uint64_t base = (uint64_t)tile * kSortTile;
+
+    uint64_t k[kSortItems];
+    uint32_t v[kSortItems], rank[kSortItems];
+    const uint64_t wbase = base + (uint64_t)wave * kWave * kSortItems;
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        bool ok = idx < a.n;
+        k[i] = ok ? a.keys_in[idx] : 0;
+        v[i] = ok ? a.vals_in[idx] : 0;
+    }
+    // stable ranking inside the wave: lanes with equal digits found by RB
+    // ballots; the lowest peer bumps the wave's counter for that digit
+    volatile uint32_t* cnt = s_wave_cnt[wave];
+    const uint64_t lt = (1ull << lane) - 1;
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        bool ok = idx < a.n;
+        uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
+        uint64_t peers = __ballot(ok);
+#pragma unroll
+        for (int b = 0; b < RB; b++) {
+            uint64_t m = __ballot((d >> b) & 1);
+            peers &= ((d >> b) & 1) ? m : ~m;
+        }
+        uint32_t c = cnt[d];
+        uint32_t below = (uint32_t)__popcll(peers & lt);
+        rank[i] = c + below;
+        if (ok && below == 0) cnt[d] = c + (uint32_t)__popcll(peers);
+        __builtin_amdgcn_wave_barrier();
+    }
+    __syncthreads();
+    // per digit: wave-exclusive prefixes and the tile total
+    uint32_t tot[DPT];
+    uint32_t tsum = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; j++) {
+        const uint32_t d = tid * DPT + j;
+        uint32_t acc = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            uint32_t c = s_wave_cnt[w][d];
+            s_wave_cnt[w][d] = acc;
+            acc += c;
+        }
+        tot[j] = acc;
+        tsum += acc;
+    }
+    // tile-local exclusive offsets of the digits (thread t owns digits t*DPT..)
+    s_part[tid] = tsum;
+    __syncthreads();
+    if (wave == 0) {
+        uint32_t x0 = s_part[lane * 4 + 0], x1 = s_part[lane * 4 + 1], x2 = s_part[lane * 4 + 2],
+                 x3 = s_part[lane * 4 + 3];
+        uint32_t s = x0 + x1 + x2 + x3;
+        uint32_t inc = wave_inclusive_scan(s);
+        uint32_t e = inc - s;
+        s_part[lane * 4 + 0] = e;
+        s_part[lane * 4 + 1] = e + x0;
+        s_part[lane * 4 + 2] = e + x0 + x1;
+        s_part[lane * 4 + 3] = e + x0 + x1 + x2;
+    }
+    __syncthreads();
+    {
+        uint32_t acc = s_part[tid];
+#pragma unroll
+        for (uint32_t j = 0; j < DPT; j++) {
+            s_local[tid * DPT + j] = acc;
+            acc += tot[j];
+        }
+    }
+    // decoupled look-back per digit: publish the tile count, then the prefix
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; j++) {
+        const uint32_t d = tid * DPT + j;
+        uint64_t* st = a.status + d;
+        uint64_t excl = 0;
+        if (tile == 0) {
+            __hip_atomic_store(&st[0], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        } else {
+            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            int64_t t = (int64_t)tile - 1;
+            uint32_t spins = 0;
+            while (t >= 0) {
+                uint64_t w = __hip_atomic_load(&st[(uint64_t)t * R], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+                uint32_t f = status_flag(w, a.epoch);
+                if (f == 0) {
+                    if (++spins > kSpinLimit) {
+                        atomicOr(a.err, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    continue;
+                }
+                excl += w & kValueMask;
+                if (f == 2) break;
+                t--;
+            }
+            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, excl + tot[j]),
+                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        s_dst[d] = (uint64_t)a.digit_base[d] + excl;
+    }
+    __syncthreads();
+    // stage the tile in LDS in digit order
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
+        if (idx < a.n) {
+            uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
+            uint32_t pos = s_local[d] + s_wave_cnt[wave][d] + rank[i];
+            s_keys[pos] = k[i];
+            s_vals[pos] = v[i];
+        }
+    }
+    __syncthreads();
+    // coalesced write-out: consecutive slots of one digit are consecutive in HBM
+    const uint32_t count = a.n - base < (uint64_t)kSortTile ? (uint32_t)(a.n - base) : (uint32_t)kSortTile;
+    for (uint32_t j = tid; j < count; j += kBlock) {
+        uint64_t key = s_keys[j];
+        uint32_t d = (uint32_t)(key >> a.shift) & (R - 1);
+        uint64_t dst = s_dst[d] + (j - s_local[d]);
+        if (dst >= a.n) {  // only after a failed look-back
+            atomicOr(a.err, 16u);
+            continue;
+        }
+        a.keys_out[dst] = key;
+        a.vals_out[dst] = s_vals[j];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// tile-scan kernels (persistent blocks, tiles handed out in order)
+// ---------------------------------------------------------------------------
+struct ScanCtx {
+    uint64_t* status;
+    uint32_t epoch;
+    uint32_t* tile_ctr;
+    uint32_t* counters;
+};
+
+// Returns the tile's global exclusive prefix; every thread gets it. `total`
+// is the tile aggregate from tile_scan. Uses s_bcast (LDS, 1 x u64).
+__device__ __forceinline__ uint64_t tile_prefix(const ScanCtx& sc, uint32_t tile, uint32_t total,
+                                                uint64_t* s_bcast) {
+    if (threadIdx.x < kWave) {
+        uint64_t p = lookback_wave(sc.status, tile, total, sc.epoch, &sc.counters[C_ERR]);
+        if (threadIdx.x == 0) *s_bcast = p;
+    }
+    __syncthreads();
+    return *s_bcast;
+}
+
+__device__ __forceinline__ uint32_t next_tile(const ScanCtx& sc, uint32_t* s_tile) {
+    __syncthreads();  // previous tile's LDS use is over
+    if (threadIdx.x == 0) *s_tile = atomicAdd(sc.tile_ctr, 1u);
+    __syncthreads();
+    return *s_tile;
+}
+
+__device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    return (uint64_t)tile * kScanTile + (uint64_t)wave * kWave * kScanItems + (uint64_t)i * kWave + lane;
+}
+
+// unique (gbuild.zig:366-388): keep the last record of each key unless it is
+// invalid or deletes (attribute 0)
+__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, const uint64_t* keys, const uint32_t* vals,
+                                                    uint32_t n, uint64_t invalid, uint64_t* ukey, uint32_t* uval) {
+    __shared__ uint32_t s_tile, s_wave[4];
+    __shared__ uint64_t s_b;
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (;;) {
+        const uint32_t tile = next_tile(sc, &s_tile);
+        if (tile >= ntiles) break;
+        uint32_t f[kScanItems], e[kScanItems];
+        uint64_t k[kScanItems];
+        uint32_t v[kScanItems];
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            f[i] = 0;
+            if (idx < n) {
+                k[i] = keys[idx];
+                v[i] = vals[idx];
+                bool last = idx + 1 == n || keys[idx + 1] != k[i];
+                f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+            }
+        }
+        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
+        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++)
+            if (f[i]) {
+                const uint64_t pos = pre + e[i];
+                if (pos >= n) {
+                    atomicOr(&sc.counters[C_ERR], 16u);
+                    continue;
+                }
+                ukey[pos] = k[i];
+                uval[pos] = v[i];
+            }
+        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_U] = (uint32_t)(pre + total);
+    }
+}
+
+// bricks (gbuild.zig:394-428): head of each run of equal key >> 6; the run's
+// mask is OR-reduced inside each 64-element round of a wave and merged into
+// masks[] with one 64-bit atomicOr per (brick, round)
+__global__ void __launch_bounds__(kBlock) k_bricks(ScanCtx sc, const uint64_t* ukey, uint32_t lbits,
+                                                    uint64_t* brick_key, uint64_t* masks, uint32_t* first_voxel,
+                                                    uint64_t brick_cap, uint32_t n_cap) {
+    __shared__ uint32_t s_tile, s_wave[4];
+    __shared__ uint64_t s_b;
+    const uint32_t n = min(sc.counters[C_U], n_cap);
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    const uint32_t lane = lane_id();
+    for (;;) {
+        const uint32_t tile = next_tile(sc, &s_tile);
+        if (tile >= ntiles) break;
+        uint32_t f[kScanItems], e[kScanItems];
+        uint64_t k[kScanItems];
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            f[i] = 0;
+            k[i] = ~0ull;
+            if (idx < n) {
+                k[i] = ukey[idx];
+                uint64_t prev = idx ? ukey[idx - 1] : ~0ull;
+                f[i] = idx == 0 || (prev >> 6) != (k[i] >> 6);
+                if (first_voxel && (idx == 0 || (prev >> lbits) != (k[i] >> lbits)))
+                    first_voxel[(uint32_t)(k[i] >> lbits)] = (uint32_t)idx;
+            }
+        }
+        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
+        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            uint64_t b = pre + e[i] + f[i] - 1;  // brick of this voxel
+            bool ok = idx < n;
+            if (ok && b >= brick_cap) {  // only after a failed look-back
+                atomicOr(&sc.counters[C_ERR], 16u);
+                ok = false;
+            }
+            if (ok && f[i]) brick_key[b] = k[i] >> 6;
+            // segmented OR over equal b (equal b are adjacent lanes)
+            uint64_t m = ok ? 1ull << (k[i] & 63) : 0;
+#pragma unroll
+            for (int o = 1; o < kWave; o <<= 1) {
+                uint64_t y = __shfl_up(m, o, kWave);
+                uint64_t yb = __shfl_up(b, o, kWave);
+                if (lane >= (uint32_t)o && yb == b) m |= y;
+            }
+            uint64_t nb = __shfl_down(b, 1, kWave);
+            bool nok = __shfl_down((uint32_t)ok, 1, kWave);
+            bool run_end = lane == kWave - 1 || nb != b || !nok;
+            if (ok && run_end) atomicOr((unsigned long long*)&masks[b], (unsigned long long)m);
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) {
+            sc.counters[C_B] = (uint32_t)(pre + total);
+            sc.counters[C_NCH + 2] = (uint32_t)(pre + total);
+        }
+    }
+}
+
+// clear the first hash_cap_for(count) slots of a table (keys 0, owner ~0)
+__global__ void __launch_bounds__(kBlock) k_hash_clear(const uint32_t* counters, uint32_t count_slot, void* table,
+                                                        uint32_t key_bytes, uint32_t* owner, uint64_t max_cap) {
+    const uint64_t want = hash_cap_for(counters[count_slot]);
+    const uint64_t cap = want < max_cap ? want : max_cap;
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cap; i += stride) {
+        if (key_bytes == 8) ((uint64_t*)table)[i] = 0;
+        else ((uint32_t*)table)[i] = 0;
+        owner[i] = 0xFFFFFFFFu;
+    }
+}
+
+// leaf_insert + owner (gbuild.zig:436-454): CAS of the mask (0 = empty slot),
+// then the smallest brick index with that mask becomes its owner
+__global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, const uint64_t* masks, uint64_t* table,
+                                                         uint32_t* owner, uint32_t* slot, uint32_t* ch_count,
+                                                         uint64_t max_cap, uint32_t brick_cap) {
+    const uint32_t n = min(counters[C_B], brick_cap);
+    const uint64_t want = hash_cap_for(n);
+    const uint64_t cap = want < max_cap ? want : max_cap, m = cap - 1;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n; b += stride) {
+        const uint64_t mask = masks[b];
+        uint64_t h = mix64(mask) & m;
+        uint64_t probes = 0;
+        for (;;) {
+            uint64_t prev = atomicCAS((unsigned long long*)&table[h], 0ull, (unsigned long long)mask);
+            if (prev == 0 || prev == mask) break;
+            h = (h + 1) & m;
+            if (++probes > cap) {
+                atomicOr(&counters[C_ERR], 4u);
+                break;
+            }
+        }
+        atomicMin(&owner[h], b);
+        slot[b] = (uint32_t)h;
+        ch_count[b] = (uint32_t)__popcll(mask);
+    }
+}
+
+// leaf_first/leaf_write (gbuild.zig:456-469): owners get leaf indices in brick
+// order; every brick records its owner
+__global__ void __launch_bounds__(kBlock) k_leaf_compact(ScanCtx sc, const uint64_t* masks, const uint32_t* owner,
+                                                          const uint32_t* slot, uint64_t* leaves_out,
+                                                          uint32_t* ch_rep, uint32_t* ch_pos, uint32_t brick_cap) {
+    __shared__ uint32_t s_tile, s_wave[4];
+    __shared__ uint64_t s_b;
+    const uint32_t n = min(sc.counters[C_B], brick_cap);
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (;;) {
+        const uint32_t tile = next_tile(sc, &s_tile);
+        if (tile >= ntiles) break;
+        uint32_t f[kScanItems], e[kScanItems], o[kScanItems];
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            f[i] = 0;
+            if (idx < n) {
+                o[i] = owner[slot[idx]];
+                f[i] = o[i] == (uint32_t)idx;
+            }
+        }
+        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
+        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            if (idx < n) {
+                ch_rep[idx] = o[i];
+                if (f[i]) {
+                    const uint64_t pos = pre + e[i];
+                    if (pos >= n) {  // only after a failed look-back
+                        atomicOr(&sc.counters[C_ERR], 16u);
+                        continue;
+                    }
+                    leaves_out[pos] = masks[idx];
+                    ch_pos[idx] = (uint32_t)pos;
+                }
+            }
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_LEAVES] = (uint32_t)(pre + total);
+    }
+}
+
+// One element set of a level: keys, the owner (representative) of each
+// element, the final ref of each owner, and voxel counts.
+struct LevelArrays {
+    uint64_t* key;
+    uint32_t* rep;
+    uint32_t* pos;
+    uint32_t* count;
+};
+
+struct LevelArgs {
+    uint32_t lv;              // children have edge 2^lv
+    uint32_t top_shift;       // 3 * (log2_size - lv): key >> top_shift = chunk (children level)
+    const uint8_t* chunk_log2;
+    uint32_t* roots;          // chunk roots (batches) or null
+    LevelArrays ch, par;
+    uint32_t* node_tmp;       // [parents][10]
+    uint32_t* node_len;
+    uint32_t* table;          // tuple hash: index + 1, 0 = empty
+    uint32_t* owner;
+    uint32_t* slot;
+    // capacities of the arrays above (guards against corrupt counts)
+    uint32_t ch_cap, par_cap;
+    uint64_t table_cap, word_cap;
+    uint32_t chunk_count;
+};
+
+// node_heads/node_build/node_insert/node_owner (gbuild.zig:475-550) in one
+// pass: parents are numbered by a tile scan of the head flags, each head
+// gathers its <= 8 children, and the tuple is inserted into the level's hash
+__global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a) {
+    __shared__ uint32_t s_tile, s_wave[4];
+    __shared__ uint64_t s_b;
+    const uint32_t n = min(sc.counters[C_NCH + a.lv], a.ch_cap);
+    const uint64_t want = hash_cap_for(n);
+    const uint64_t cap = want < a.table_cap ? want : a.table_cap, hm = cap - 1;
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    for (;;) {
+        const uint32_t tile = next_tile(sc, &s_tile);
+        if (tile >= ntiles) break;
+        uint32_t f[kScanItems], e[kScanItems];
+        uint64_t k[kScanItems];
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            f[i] = 0;
+            if (idx < n) {
+                k[i] = a.ch.key[idx];
+                bool finished = false;
+                if (a.chunk_log2) {
+                    // a chunk whose edge is 2^lv ends here: this element is its root
+                    uint32_t c = (uint32_t)(k[i] >> a.top_shift);
+                    if (c < a.chunk_count && a.chunk_log2[c] == a.lv) {
+                        finished = true;
+                        a.roots[c] = a.ch.pos[a.ch.rep[idx]];
+                    }
+                }
+                f[i] = !finished && (idx == 0 || (a.ch.key[idx - 1] >> 3) != (k[i] >> 3));
+            }
+        }
+        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
+        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            if (!f[i]) continue;
+            const uint64_t idx = scan_index(tile, i);
+            if (pre + e[i] >= a.par_cap) {  // only after a failed look-back
+                atomicOr(&sc.counters[C_ERR], 16u);
+                continue;
+            }
+            const uint32_t p = (uint32_t)(pre + e[i]);
+            const uint64_t pk = k[i] >> 3;
+            uint32_t w[kNodeStride];
+            uint32_t mask = 0, count = 0, len = 2;
+            for (uint64_t j = idx; j < n && len < kNodeStride && (a.ch.key[j] >> 3) == pk; j++) {
+                mask |= 1u << (uint32_t)(a.ch.key[j] & 7);
+                w[len++] = a.ch.pos[a.ch.rep[j]];
+                count = sat_add(count, a.ch.count[j]);
+            }
+            w[0] = mask;
+            w[1] = count;
+            uint32_t* dst = a.node_tmp + (uint64_t)p * kNodeStride;
+            for (uint32_t q = 0; q < len; q++) dst[q] = w[q];
+            a.node_len[p] = len;
+            a.par.key[p] = pk;
+            a.par.count[p] = count;
+            __threadfence();  // words visible before the index is published
+            uint64_t hv = len;
+            for (uint32_t q = 0; q < len; q++) hv = mix64(hv ^ w[q]);
+            uint64_t h = hv & hm;
+            uint64_t probes = 0;
+            for (;;) {
+                uint32_t prev = atomicCAS(&a.table[h], 0u, p + 1);
+                if (prev == 0) break;
+                const uint32_t other = prev - 1;
+                if (load_agent(&a.node_len[other]) == len) {
+                    const uint32_t* ow = a.node_tmp + (uint64_t)other * kNodeStride;
+                    bool same = true;
+                    for (uint32_t q = 0; q < len; q++) same &= load_agent(&ow[q]) == w[q];
+                    if (same) break;
+                }
+                h = (h + 1) & hm;
+                if (++probes > cap) {
+                    atomicOr(&sc.counters[C_ERR], 8u);
+                    break;
+                }
+            }
+            atomicMin(&a.owner[h], p);
+            a.slot[p] = (uint32_t)h;
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_NCH + a.lv + 1] = (uint32_t)(pre + total);
+    }
+}
+
+// node_first/node_write (gbuild.zig:552-571): owners' words are appended at
+// the level cursor in parent order; every parent records its owner
+__global__ void __launch_bounds__(kBlock) k_level_compact(ScanCtx sc, LevelArgs a, uint32_t* nodes_out) {
+    __shared__ uint32_t s_tile, s_wave[4];
+    __shared__ uint64_t s_b;
+    const uint32_t n = min(sc.counters[C_NCH + a.lv + 1], a.par_cap);
+    const uint32_t cur = sc.counters[C_CUR + a.lv];
+    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    if (ntiles == 0 && blockIdx.x == 0 && threadIdx.x == 0) sc.counters[C_CUR + a.lv + 1] = cur;
+    for (;;) {
+        const uint32_t tile = next_tile(sc, &s_tile);
+        if (tile >= ntiles) break;
+        uint32_t f[kScanItems], e[kScanItems], o[kScanItems];
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            f[i] = 0;
+            if (idx < n) {
+                o[i] = a.owner[a.slot[idx]];
+                f[i] = o[i] == (uint32_t)idx ? a.node_len[idx] : 0;
+            }
+        }
+        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
+        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            uint64_t idx = scan_index(tile, i);
+            if (idx >= n) continue;
+            a.par.rep[idx] = o[i];
+            if (f[i]) {
+                const uint32_t at = cur + (uint32_t)(pre + e[i]);
+                if ((uint64_t)at + f[i] > a.word_cap) {  // only after a failed look-back
+                    atomicOr(&sc.counters[C_ERR], 16u);
+                    continue;
+                }
+                const uint32_t* src = a.node_tmp + idx * kNodeStride;
+                for (uint32_t q = 0; q < f[i]; q++) nodes_out[at + q] = src[q];
+                a.par.pos[idx] = at;
+            }
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_CUR + a.lv + 1] = cur + (uint32_t)(pre + total);
+    }
+}
+
+// roots of the top level and the empty geometry (gbuild.zig:607-624)
+__global__ void __launch_bounds__(kBlock) k_finalize(uint32_t* counters, uint32_t top, LevelArrays par,
+                                                      uint32_t* roots, uint32_t* nodes_out, uint32_t par_cap,
+                                                      uint32_t chunk_count, uint64_t word_cap) {
+    const uint32_t np = min(counters[C_NCH + top], par_cap);
+    const uint32_t cur = counters[C_CUR + top];
+    if (roots) {
+        for (uint32_t p = threadIdx.x; p < np; p += kBlock) {
+            const uint64_t c = par.key[p];
+            if (c < chunk_count) roots[c] = par.pos[par.rep[p]];
+            else atomicOr(&counters[C_ERR], 16u);
+        }
+    }
+    if (threadIdx.x == 0) {
+        uint32_t root = np > 0 ? par.pos[par.rep[0]] : 0;
+        uint32_t words = cur;
+        if (counters[C_U] == 0 && (uint64_t)cur + 2 <= word_cap) {
+            nodes_out[cur] = 0;
+            nodes_out[cur + 1] = 0;
+            root = cur;
+            words = cur + 2;
+        }
+        counters[C_ROOT] = root;
+        counters[C_WORDS] = words;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// host orchestration
+// ---------------------------------------------------------------------------
+namespace {
+
+inline uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
+    uint64_t g = (work + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    return (uint32_t)std::min<uint64_t>(g, cap);
+}
+
+void fill32(hipStream_t s, uint32_t* p, uint32_t v, uint64_t n) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3(grid_for(n, 256 * 4, 4096)), dim3(256), 0, s, p, v, n);
+}
+void fill64(hipStream_t s, uint64_t* p, uint64_t v, uint64_t n) {
+    if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256 * 4, 4096)), dim3(256), 0, s, p, v, n);
+}
+
+struct Alloc {
+    std::vector<void*> ptrs;
+    hipStream_t s;
+    bool failed = false;
+    template <class T>
+    T* get(uint64_t count) {
+        void* p = nullptr;
+        if (hipMallocAsync(&p, std::max<uint64_t>(count * sizeof(T), 256), s) != hipSuccess) {
+            failed = true;
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        return (T*)p;
+    }
+    void release() {
+        for (void* p : ptrs) hipFreeAsync(p, s);
+        ptrs.clear();
+    }
+};
+
+bool g_timing = false;
+PyrhipBuildTimes g_times{};
+
+struct Timer {
+    hipEvent_t ev[8];
+    int n = 0;
+    hipStream_t s;
+    void mark() {
+        if (!g_timing) return;
+        hipEventCreate(&ev[n]);
+        hipEventRecord(ev[n], s);
+        n++;
+    }
+    float between(int a, int b) {
+        float ms = 0;
+        if (g_timing && b < n) hipEventElapsedTime(&ms, ev[a], ev[b]);
+        return ms;
+    }
+    ~Timer() {
+        for (int i = 0; i < n; i++) hipEventDestroy(ev[i]);
+    }
+};
+
+template <int RB>
+void launch_sort(hipStream_t s, uint32_t tiles, const SortArgs& a) {
+    hipLaunchKernelGGL(k_sort_pass<RB>, dim3(tiles), dim3(kBlock), 0, s, a);
+}
+
+}  // namespace
+
+// One build: single geometry (enc.chunk_count == 0) or chunk batch.
+PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBuildTuning* tuning, hipStream_t s,
+                    PyrhipBuilt* out, uint32_t* roots_host, uint32_t* first_host) {
+    memset(out, 0, sizeof(*out));
+    EncodeArgs enc = enc_in;
+    const uint64_t n64 = (uint64_t)enc.n_old + enc.n_new;
+    if (n64 >= 0xFFFFFFFFull) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many voxels for one build (%llu)",
+                                          (unsigned long long)n64);
+    const uint32_t n = (uint32_t)n64, nmax = std::max<uint32_t>(n, 1);
+    const uint32_t rb = tuning && tuning->radix_bits ? tuning->radix_bits : 8;
+    if (rb < 8 || rb > 11) return fail(PYR_ERROR_INVALID_ARGUMENT, "radix_bits must be in [8, 11]");
+    const uint32_t lbits = local_bits(log2_size);
+    const uint32_t key_bits = lbits + enc.chunk_bits + 1;
+    if (key_bits > 64) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
+    const uint32_t passes = (key_bits + rb - 1) / rb;
+    const uint32_t radix = 1u << rb;
+    const uint64_t k = enc.chunk_count ? enc.chunk_count : 1;
+
+    // upper bounds (gpu_build.zig:142-154, 291-300)
+    auto cells = [&](uint32_t lv) -> uint64_t {  // cells of edge 2^lv in all chunks
+        uint64_t per = 1ull << (3 * (log2_size - std::min(lv, log2_size)));
+        return std::max<uint64_t>(std::min<uint64_t>(nmax, per * k), 1);
+    };
+    const uint64_t nb = cells(2);                 // bricks
+    const uint64_t np2 = std::min(nb, cells(3));  // parents of bricks
+    uint64_t word_bound = 2;
+    for (uint32_t lv = 2; lv < log2_size; lv++) word_bound += std::min(nb, cells(lv + 1)) * kNodeStride;
+    const uint64_t cap = hash_cap_for(nb);
+
+    Alloc A{{}, s};
+    uint32_t* counters = A.get<uint32_t>(C_COUNT);
+    uint32_t* hist = A.get<uint32_t>((uint64_t)passes * radix * 2);
+    uint32_t* digit_base = hist + (uint64_t)passes * radix;
+    const uint64_t sort_tiles = (nmax + kSortTile - 1) / kSortTile;
+    const uint64_t scan_tiles = (nmax + kScanTile - 1) / kScanTile;
+    const uint64_t status_words = std::max<uint64_t>(sort_tiles * radix, scan_tiles);
+    uint64_t* status = A.get<uint64_t>(status_words);
+    uint64_t* brick_key = A.get<uint64_t>(nb);
+    uint64_t* masks = A.get<uint64_t>(nb);
+    uint64_t* table = A.get<uint64_t>(cap);
+    uint32_t* owner = A.get<uint32_t>(cap);
+    uint32_t* slot = A.get<uint32_t>(nb);
+    LevelArrays set[2];
+    const uint64_t set_cap[2] = {nb, np2};
+    for (int q = 0; q < 2; q++) {
+        set[q].key = q == 0 ? brick_key : A.get<uint64_t>(np2);
+        set[q].rep = A.get<uint32_t>(q == 0 ? nb : np2);
+        set[q].pos = A.get<uint32_t>(q == 0 ? nb : np2);
+        set[q].count = A.get<uint32_t>(q == 0 ? nb : np2);
+    }
+    uint32_t* node_tmp = A.get<uint32_t>(np2 * kNodeStride);
+    uint32_t* node_len = A.get<uint32_t>(np2);
+    uint32_t* roots_dev = enc.chunk_count ? A.get<uint32_t>(enc.chunk_count * 2) : nullptr;
+    uint32_t* first_dev = enc.chunk_count ? roots_dev + enc.chunk_count : nullptr;
+    // caller-owned outputs; of the two key/value buffer pairs one ends up
+    // holding the unique voxels (Built.keys/attrs), the other is freed
+    uint64_t* kb[2] = {nullptr, nullptr};
+    uint32_t* vb[2] = {nullptr, nullptr};
+    uint64_t* leaves_out = nullptr;
+    uint32_t* nodes_out = nullptr;
+    bool out_ok = true;
+    for (int q = 0; q < 2; q++) {
+        out_ok &= hipMalloc(&kb[q], (uint64_t)nmax * 8) == hipSuccess;
+        out_ok &= hipMalloc(&vb[q], (uint64_t)nmax * 4) == hipSuccess;
+    }
+    out_ok &= hipMalloc(&leaves_out, nb * 8) == hipSuccess;
+    out_ok &= hipMalloc(&nodes_out, word_bound * 4) == hipSuccess;
+    auto free_outputs = [&]() {
+        for (int q = 0; q < 2; q++) {
+            hipFree(kb[q]);
+            hipFree(vb[q]);
+        }
+        hipFree(leaves_out);
+        hipFree(nodes_out);
+    };
+    if (A.failed || !out_ok) {
+        A.release();
+        free_outputs();
+        hipGetLastError();
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "device memory for a build of %u voxels", n);
+    }
+
+    Timer tm;
+    tm.s = s;
+    tm.mark();
+    // zero state
+    fill32(s, counters, 0, C_COUNT);
+    fill32(s, hist, 0, (uint64_t)passes * radix);
+    fill64(s, status, 0, status_words);
+    fill64(s, masks, 0, nb);
+    if (enc.chunk_count) fill32(s, roots_dev, 0xFFFFFFFFu, enc.chunk_count * 2);
+
+    uint32_t launch = 0, epoch = 0;
+    auto tile_ctr = [&]() { return counters + C_TILE + (launch++ % kMaxLaunches); };
+
+    // 1. encode + histograms
+    enc.keys = kb[0];
+    enc.vals = vb[0];
+    enc.hist = hist;
+    enc.passes = passes;
+    enc.radix_bits = rb;
+    if (n) {
+        hipLaunchKernelGGL(k_encode, dim3(grid_for(n, kBlock * 16, 2048)), dim3(kBlock),
+                           passes * radix * sizeof(uint32_t), s, enc, n);
+        hipLaunchKernelGGL(k_digit_scan, dim3(1), dim3(kBlock), 0, s, hist, digit_base, passes, radix);
+    }
+    tm.mark();
+    // 2. radix passes
+    uint32_t cur = 0;
+    for (uint32_t p = 0; p < passes && n; p++) {
+        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, p * rb, digit_base + p * radix, status,
+                    ++epoch, tile_ctr(), counters + C_ERR};
+        switch (rb) {
+            case 8: launch_sort<8>(s, (uint32_t)sort_tiles, sa); break;
+            case 9: launch_sort<9>(s, (uint32_t)sort_tiles, sa); break;
+            case 10: launch_sort<10>(s, (uint32_t)sort_tiles, sa); break;
+            default: launch_sort<11>(s, (uint32_t)sort_tiles, sa); break;
+        }
+        cur ^= 1;
+    }
+    tm.mark();
+    // 3. unique into the other buffer pair, which becomes Built.keys/attrs
+    uint64_t* ukey = kb[cur ^ 1];
+    uint32_t* uval = vb[cur ^ 1];
+    const uint64_t invalid = 1ull << (lbits + enc.chunk_bits);
+    const uint32_t scan_grid = grid_for(nmax, kScanTile, 2048);
+    if (n) {
+        ScanCtx sc{status, ++epoch, tile_ctr(), counters};
+        hipLaunchKernelGGL(k_unique, dim3(scan_grid), dim3(kBlock), 0, s, sc, kb[cur], vb[cur], n, invalid, ukey,
+                           uval);
+        ScanCtx sb{status, ++epoch, tile_ctr(), counters};
+        hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, ukey, lbits, brick_key, masks,
+                           first_dev, nb, nmax);
+    }
+    tm.mark();
+    // 4. leaves
+    const uint32_t brick_grid = grid_for(nb, kScanTile, 2048);
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(cap, 1024, 2048)), dim3(kBlock), 0, s, counters, (uint32_t)C_B,
+                       (void*)table, 8u, owner, cap);
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s, counters, masks,
+                       table, owner, slot, set[0].count, cap, (uint32_t)nb);
+    {
+        ScanCtx sc{status, ++epoch, tile_ctr(), counters};
+        hipLaunchKernelGGL(k_leaf_compact, dim3(brick_grid), dim3(kBlock), 0, s, sc, masks, owner, slot,
+                           leaves_out, set[0].rep, set[0].pos, (uint32_t)nb);
+    }
+    tm.mark();
+    // 5. node levels
+    int ch = 0;
+    for (uint32_t lv = 2; lv < log2_size; lv++) {
+        LevelArgs la{};
+        la.lv = lv;
+        la.top_shift = 3 * (log2_size - lv);
+        la.chunk_log2 = enc.chunk_log2;
+        la.roots = roots_dev;
+        la.ch = set[ch];
+        la.par = set[ch ^ 1];
+        la.node_tmp = node_tmp;
+        la.node_len = node_len;
+        la.table = (uint32_t*)table;
+        la.owner = owner;
+        la.slot = slot;
+        la.ch_cap = (uint32_t)set_cap[ch];
+        la.par_cap = (uint32_t)std::min(set_cap[ch ^ 1], np2);
+        la.table_cap = cap;
+        la.word_cap = word_bound;
+        la.chunk_count = enc.chunk_count;
+        const uint64_t nch = lv == 2 ? nb : std::min(nb, cells(lv));
+        const uint32_t g = grid_for(nch, kScanTile, 2048);
+        hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(hash_cap_for(nch), 1024, 2048)), dim3(kBlock), 0, s,
+                           counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, cap);
+        ScanCtx sc{status, ++epoch, tile_ctr(), counters};
+        hipLaunchKernelGGL(k_level_build, dim3(g), dim3(kBlock), 0, s, sc, la);
+        ScanCtx sc2{status, ++epoch, tile_ctr(), counters};
+        hipLaunchKernelGGL(k_level_compact, dim3(grid_for(std::min(nb, cells(lv + 1)), kScanTile, 2048)),
+                           dim3(kBlock), 0, s, sc2, la, nodes_out);
+        ch ^= 1;
+    }
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, counters, log2_size, set[ch], roots_dev,
+                       nodes_out, (uint32_t)std::min(set_cap[ch], np2), enc.chunk_count, word_bound);
+    tm.mark();
+
+    // single read-back (gpu_build.zig:338-344)
+    uint32_t c[C_TILE];
+    hipError_t e = hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, s);
+    if (enc.chunk_count && e == hipSuccess) {
+        e = hipMemcpyAsync(roots_host, roots_dev, enc.chunk_count * 4, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(first_host, first_dev, enc.chunk_count * 4, hipMemcpyDeviceToHost, s);
+    }
+    A.release();
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = hipGetLastError();
+    if (g_timing && e == hipSuccess) {
+        g_times.encode_ms = tm.between(0, 1);
+        g_times.sort_ms = tm.between(1, 2);
+        g_times.unique_ms = tm.between(2, 3);
+        g_times.leaves_ms = tm.between(3, 4);
+        g_times.levels_ms = tm.between(4, 5);
+        g_times.total_ms = tm.between(0, 5);
+        g_times.sort_passes = passes;
+        g_times.radix_bits = rb;
+    }
+    if (e != hipSuccess || c[C_ERR] != 0) {
+        free_outputs();
+        if (e != hipSuccess) return fail(PYR_ERROR_CUDA, "build: %s", hipGetErrorString(e));
+        return fail(PYR_ERROR_CUDA, "build: device error flags 0x%x", c[C_ERR]);
+    }
+    out->log2_size = log2_size;
+    out->root = c[C_ROOT];
+    out->nodes = (uint64_t)(uintptr_t)nodes_out;
+    out->node_words = c[C_WORDS];
+    out->leaves = (uint64_t)(uintptr_t)leaves_out;
+    out->leaf_count = c[C_LEAVES];
+    out->keys = (uint64_t)(uintptr_t)ukey;
+    out->attrs = (uint64_t)(uintptr_t)uval;
+    out->voxel_count = c[C_U];
+    // the buffer pair not handed out held the sorted records
+    hipFree(kb[cur]);
+    hipFree(vb[cur]);
+    return PYR_OK;
+}
+
+}  // namespace pyr
+
+using namespace pyr;
+
+extern "C" {
+
+PYR_API PyrResult pyrhip_build(uint32_t log2_size, uint64_t old_keys, uint64_t old_vals, uint32_t n_old,
+                               uint64_t voxels, uint32_t n_new, const PyrhipBuildTuning* tuning, void* stream,
+                               PyrhipBuilt* out) {
+    diag_clear();
+    if (!out) return fail(PYR_ERROR_INVALID_ARGUMENT, "out is null");
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if ((n_old && (!old_keys || !old_vals)) || (n_new && !voxels))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "null input with a non-zero count");
+    EncodeArgs enc{};
+    enc.old_keys = (const uint64_t*)(uintptr_t)old_keys;
+    enc.old_vals = (const uint32_t*)(uintptr_t)old_vals;
+    enc.n_old = n_old;
+    enc.voxels = (const int4*)(uintptr_t)voxels;
+    enc.n_new = n_new;
+    enc.log2_size = log2_size;
+    return run_build(enc, log2_size, tuning, (hipStream_t)stream, out, nullptr, nullptr);
+}
+
+PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch* batch,
+                                      const PyrhipBuildTuning* tuning, void* stream, PyrhipBuilt* out,
+                                      uint32_t* roots, uint32_t* first_voxel) {
+    diag_clear();
+    if (!out || !batch || !roots || !first_voxel) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if (batch->count == 0) return fail(PYR_ERROR_INVALID_ARGUMENT, "empty chunk batch");
+    if (!batch->offsets || (batch->total && !batch->voxels))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "null batch buffers");
+    EncodeArgs enc{};
+    enc.voxels = (const int4*)(uintptr_t)batch->voxels;
+    enc.n_new = batch->total;
+    enc.log2_size = log2_size;
+    enc.chunk_count = batch->count;
+    enc.capacity = batch->capacity;
+    enc.offsets = (const uint32_t*)(uintptr_t)batch->offsets;
+    enc.chunk_log2 = (const uint8_t*)(uintptr_t)batch->chunk_log2;
+    uint32_t bits = 0;
+    while ((1u << bits) < batch->count) bits++;
+    enc.chunk_bits = bits;
+    return run_build(enc, log2_size, tuning, (hipStream_t)stream, out, roots, first_voxel);
+}
+
+PYR_API void pyrhip_built_free(PyrhipBuilt* b, void* stream) {
+    if (!b) return;
+    (void)stream;
+    hipFree((void*)(uintptr_t)b->nodes);
+    hipFree((void*)(uintptr_t)b->leaves);
+    hipFree((void*)(uintptr_t)b->keys);
+    hipFree((void*)(uintptr_t)b->attrs);
+    memset(b, 0, sizeof(*b));
+}
+
+PYR_API void pyrhip_set_timing(int enable) { g_timing = enable != 0; }
+
+PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out) {
+    if (out) *out = g_times;
+}
+
+}  // extern "C"

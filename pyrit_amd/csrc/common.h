@@ -1,0 +1,174 @@
+// common.h - shared device helpers for the gfx950 build and trace kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pyr {
+
+constexpr int kWave = 64;    // CDNA wavefront width
+constexpr int kBlock = 256;  // 4 waves per workgroup for the tile kernels
+
+__device__ __forceinline__ uint64_t spread3(uint32_t v) {
+    uint64_t x = v & 0x1FFFFFu;
+    x = (x | (x << 32)) & 0x1F00000000FFFFull;
+    x = (x | (x << 16)) & 0x1F0000FF0000FFull;
+    x = (x | (x << 8)) & 0x100F00F00F00F00Full;
+    x = (x | (x << 4)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x << 2)) & 0x1249249249249249ull;
+    return x;
+}
+
+// Key of a voxel inside a chunk: brick Morton << 6 | bit in the 4^3 brick
+// (bit = x + 4y + 16z), the key of gbuild.zig:98-101.
+__device__ __forceinline__ uint64_t local_key(uint32_t x, uint32_t y, uint32_t z) {
+    uint64_t brick = spread3(x >> 2) | (spread3(y >> 2) << 1) | (spread3(z >> 2) << 2);
+    return (brick << 6) | ((x & 3) + 4 * (y & 3) + 16 * (z & 3));
+}
+
+__host__ __device__ __forceinline__ uint32_t local_bits(uint32_t log2_size) { return 3 * (log2_size - 2) + 6; }
+
+// 64-bit finalizer used to place masks and node tuples in the hash tables; the
+// output never depends on it (owner = smallest index, positions from scans).
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {
+    x ^= x >> 33;
+    x *= 0xff51afd7ed558ccdull;
+    x ^= x >> 33;
+    x *= 0xc4ceb9fe1a85ec53ull;
+    x ^= x >> 33;
+    return x;
+}
+
+__device__ __forceinline__ uint32_t sat_add(uint32_t a, uint32_t b) {
+    uint32_t r = a + b;
+    return r < a ? 0xFFFFFFFFu : r;
+}
+
+// Loads that bypass the non-coherent per-CU vector L1, for data another
+// workgroup of the same launch published.
+template <class T>
+__device__ __forceinline__ T load_agent(const T* p) {
+    return __hip_atomic_load(const_cast<T*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void store_agent(T* p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+    const uint32_t lane = lane_id();
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+        uint32_t y = __shfl_up(x, o, kWave);
+        if (lane >= (uint32_t)o) x += y;
+    }
+    return x;
+}
+
+__device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) x += __shfl_xor(x, o, kWave);
+    return x;
+}
+
+// ---------------------------------------------------------------------------
+// Decoupled look-back scan across the tiles of one launch.
+//
+// Status word: [63:62] flag (0 = not ready, 1 = aggregate, 2 = inclusive
+// prefix), [61:40] epoch of the launch, [39:0] value. A word from another
+// launch (other epoch) reads as "not ready", so one zero-initialised status
+// array serves every launch of a build without clearing. Waits are bounded:
+// after kSpinLimit polls the launch records an error and goes on, so every
+// wave terminates even if a predecessor never publishes.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPre = 2ull << 62;
+constexpr uint64_t kValueMask = (1ull << 40) - 1;
+constexpr uint32_t kSpinLimit = 1u << 22;
+
+__device__ __forceinline__ uint64_t status_word(uint64_t flag, uint32_t epoch, uint64_t v) {
+    return flag | ((uint64_t)(epoch & 0x3FFFFFu) << 40) | (v & kValueMask);
+}
+
+__device__ __forceinline__ uint32_t status_flag(uint64_t w, uint32_t epoch) {
+    if (((w >> 40) & 0x3FFFFFu) != (epoch & 0x3FFFFFu)) return 0;
+    return (uint32_t)(w >> 62);
+}
+
+// Exclusive prefix of this tile's `aggregate`. Called by all 64 lanes of one
+// wave; returns the same value in every lane. Predecessors are polled 64 at a
+// time.
+__device__ __forceinline__ uint64_t lookback_wave(uint64_t* status, uint32_t tile, uint64_t aggregate,
+                                                  uint32_t epoch, uint32_t* err) {
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0)
+            __hip_atomic_store(&status[0], status_word(kFlagPre, epoch, aggregate), __ATOMIC_RELEASE,
+                               __HIP_MEMORY_SCOPE_AGENT);
+        return 0;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], status_word(kFlagAgg, epoch, aggregate), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    uint64_t excl = 0;
+    int64_t pos = (int64_t)tile - 1;
+    uint32_t spins = 0;
+    while (true) {
+        int64_t idx = pos - (int64_t)lane;
+        uint64_t w = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+                              : status_word(kFlagPre, epoch, 0);
+        uint32_t f = status_flag(w, epoch);
+        if (__ballot(f == 0) != 0) {
+            if (++spins > kSpinLimit) {
+                if (lane == 0) atomicOr(err, 1u);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            continue;
+        }
+        uint64_t pre = __ballot(f == 2);
+        uint64_t v = w & kValueMask;
+        if (pre) {
+            uint32_t first = (uint32_t)__ffsll((unsigned long long)pre) - 1;
+            excl += wave_sum64(lane <= first ? v : 0);
+            break;
+        }
+        excl += wave_sum64(v);
+        pos -= kWave;
+    }
+    if (lane == 0)
+        __hip_atomic_store(&status[tile], status_word(kFlagPre, epoch, excl + aggregate), __ATOMIC_RELEASE,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    return excl;
+}
+
+// Tile-local exclusive scan of per-item values laid out wave-striped: item i
+// of lane l in wave w is element w*64*ITEMS + i*64 + l of the tile. Returns
+// the tile total; s_wave must hold 4 words of LDS.
+template <int ITEMS>
+__device__ __forceinline__ uint32_t tile_scan(const uint32_t (&v)[ITEMS], uint32_t (&excl)[ITEMS],
+                                              uint32_t* s_wave) {
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    uint32_t carry = 0;
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) {
+        uint32_t inc = wave_inclusive_scan(v[i]);
+        excl[i] = carry + inc - v[i];
+        carry += __shfl(inc, kWave - 1, kWave);
+    }
+    if (lane == 0) s_wave[wave] = carry;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (int w = 0; w < kBlock / kWave; w++) {
+        uint32_t t = s_wave[w];
+        before += (uint32_t)w < wave ? t : 0;
+        total += t;
+    }
+#pragma unroll
+    for (int i = 0; i < ITEMS; i++) excl[i] += before;
+    return total;
+}
+
+} // namespace pyr

@@ -1,0 +1,181 @@
+"""Python mirror of the reference's build / trace operator interface
+(src/gpu_build.zig:94-131 build / buildChunks, src/device/trace.zig trace)
+over the C ABI of libpyrit.so. Device memory comes from pyrit_amd.device."""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import abi, lib
+from .device import Buffer, download, to_device
+
+
+@dataclass
+class Built:
+    """gpu_build.Built (src/gpu_build.zig:25-47): device arrays owned here."""
+    raw: abi.PyrhipBuilt
+
+    @property
+    def log2_size(self): return self.raw.log2_size
+    @property
+    def root(self): return self.raw.root
+    @property
+    def node_words(self): return self.raw.node_words
+    @property
+    def leaf_count(self): return self.raw.leaf_count
+    @property
+    def voxel_count(self): return self.raw.voxel_count
+
+    def nodes(self) -> np.ndarray:
+        return download(self.raw.nodes, np.uint32, self.raw.node_words)
+
+    def leaves(self) -> np.ndarray:
+        return download(self.raw.leaves, np.uint64, self.raw.leaf_count)
+
+    def attrs(self) -> np.ndarray:
+        return download(self.raw.attrs, np.uint32, self.raw.voxel_count)
+
+    def keys(self) -> np.ndarray:
+        return download(self.raw.keys, np.uint64, self.raw.voxel_count)
+
+    def dag(self) -> abi.PyrhipDag:
+        d = abi.PyrhipDag()
+        d.nodes, d.leaves, d.attrs = self.raw.nodes, self.raw.leaves, self.raw.attrs
+        d.root, d.log2_size, d.default_attribute = self.raw.root, self.raw.log2_size, 1
+        return d
+
+    def free(self) -> None:
+        if self.raw.nodes or self.raw.keys:
+            lib().pyrhip_built_free(C.byref(self.raw), None)
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _tuning(radix_bits: int) -> abi.PyrhipBuildTuning:
+    t = abi.PyrhipBuildTuning()
+    t.radix_bits = radix_bits
+    return t
+
+
+def _ptr(x) -> int:
+    if x is None:
+        return 0
+    if isinstance(x, Buffer):
+        return x.ptr
+    return int(x)
+
+
+def build(log2_size: int, voxels, count: int | None = None, old: Built | None = None, radix_bits: int = 0,
+          stream: int | None = None) -> Built:
+    """Build a DAG from PyrVoxel records (numpy VOXEL array or device Buffer).
+    `old` supplies the sorted voxels of an earlier editable build (edits)."""
+    keep = None
+    if isinstance(voxels, np.ndarray):
+        assert voxels.dtype == abi.VOXEL
+        count = len(voxels)
+        keep = to_device(voxels) if count else None
+        vptr = keep.ptr if keep else 0
+    else:
+        vptr = _ptr(voxels)
+        assert count is not None
+    out = abi.PyrhipBuilt()
+    t = _tuning(radix_bits)
+    L = lib()
+    code = L.pyrhip_build(log2_size, old.raw.keys if old else 0, old.raw.attrs if old else 0,
+                          old.voxel_count if old else 0, vptr, count, C.byref(t), stream, C.byref(out))
+    abi.check(L, code, "pyrhip_build")
+    return Built(out)
+
+
+def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None,
+                 capacity: int = 0, radix_bits: int = 0, stream: int | None = None):
+    """Chunk batch (src/gpu_build.zig:98-122). voxels: VOXEL array or device
+    Buffer; offsets: u32[count+1] host array. Returns (Built, roots, first)."""
+    offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
+    count = len(offsets) - 1
+    keep = []
+    if isinstance(voxels, np.ndarray):
+        vb = to_device(voxels) if len(voxels) else Buffer(16)
+        keep.append(vb)
+        vptr = vb.ptr
+    else:
+        vptr = _ptr(voxels)
+    ob = to_device(offsets)
+    keep.append(ob)
+    lptr = 0
+    if chunk_log2 is not None:
+        lb = to_device(np.ascontiguousarray(chunk_log2, dtype=np.uint8))
+        keep.append(lb)
+        lptr = lb.ptr
+    batch = abi.PyrhipChunkBatch()
+    batch.count, batch.capacity, batch.voxels, batch.offsets = count, capacity, vptr, ob.ptr
+    batch.total, batch.chunk_log2 = int(offsets[-1]), lptr
+    roots = np.zeros(count, np.uint32)
+    first = np.zeros(count, np.uint32)
+    out = abi.PyrhipBuilt()
+    t = _tuning(radix_bits)
+    L = lib()
+    code = L.pyrhip_build_chunks(log2_size, C.byref(batch), C.byref(t), stream, C.byref(out),
+                                 roots.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                 first.ctypes.data_as(C.POINTER(C.c_uint32)))
+    abi.check(L, code, "pyrhip_build_chunks")
+    return Built(out), roots, first
+
+
+def trace_dag(dag: abi.PyrhipDag, rays, flags: int = 0, stream: int | None = None) -> np.ndarray:
+    """PyrRay records -> PyrHit records against one DAG (instance 0)."""
+    keep = None
+    if isinstance(rays, np.ndarray):
+        assert rays.dtype == abi.RAY
+        count = len(rays)
+        keep = to_device(rays)
+        rptr = keep.ptr
+    else:
+        rptr, count = rays
+    hits = Buffer(max(count, 1) * abi.HIT.itemsize)
+    L = lib()
+    abi.check(L, L.pyrhip_trace_dag(C.byref(dag), rptr, hits.ptr, count, flags, stream), "pyrhip_trace_dag")
+    return hits.download(abi.HIT, count)
+
+
+def trace_camera(dag: abi.PyrhipDag, cam: abi.PyrCamera, flags: int = 0, stream: int | None = None) -> np.ndarray:
+    n = cam.width * cam.height
+    hits = Buffer(max(n, 1) * abi.HIT.itemsize)
+    L = lib()
+    abi.check(L, L.pyrhip_trace_camera(C.byref(dag), C.byref(cam), hits.ptr, flags, stream), "pyrhip_trace_camera")
+    return hits.download(abi.HIT, n)
+
+
+def camera_rays(cam: abi.PyrCamera, stream: int | None = None) -> np.ndarray:
+    n = cam.width * cam.height
+    rays = Buffer(max(n, 1) * abi.RAY.itemsize)
+    L = lib()
+    abi.check(L, L.pyrhip_camera_rays(C.byref(cam), rays.ptr, stream), "pyrhip_camera_rays")
+    return rays.download(abi.RAY, n)
+
+
+def trace_shadow(dag: abi.PyrhipDag, rays: np.ndarray, hits: np.ndarray, sun, stream: int | None = None) -> np.ndarray:
+    count = len(rays)
+    rb, hb = to_device(rays), to_device(hits)
+    out = Buffer(max(count, 1) * 4)
+    s = (C.c_float * 3)(*[float(v) for v in sun])
+    L = lib()
+    abi.check(L, L.pyrhip_trace_shadow(C.byref(dag), rb.ptr, hb.ptr, s, out.ptr, count, stream),
+              "pyrhip_trace_shadow")
+    return out.download(np.uint32, count)
+
+
+def set_timing(on: bool) -> None:
+    lib().pyrhip_set_timing(1 if on else 0)
+
+
+def last_build_times() -> dict:
+    t = abi.PyrhipBuildTimes()
+    lib().pyrhip_last_build_times(C.byref(t))
+    return {k: getattr(t, k) for k, _ in abi.PyrhipBuildTimes._fields_}

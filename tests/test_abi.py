@@ -1,0 +1,122 @@
+"""ABI checks in the style of the reference's tests/abi_test.zig:11-55: the
+library loads (no GPU needed), exports every function include/*.h declares,
+and the struct layouts of the C header match the Python mirrors and the
+reference's sizes."""
+import ctypes as C
+import os
+import re
+import subprocess
+import tempfile
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADERS = [os.path.join(ROOT, "include", h) for h in ("pyrit.h", "pyrit_hip_build.h")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        text = "\n".join(l for l in text.splitlines() if not l.lstrip().startswith("#"))
+        for m in re.finditer(r"PYR_API\s+[^;(]*?\b(\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_loads_and_exports_declared_symbols():
+    from pyrit_amd import LIB_PATH, abi
+
+    assert os.path.exists(LIB_PATH), "libpyrit.so not built"
+    lib = C.CDLL(LIB_PATH)
+    names = declared_functions()
+    assert len(names) >= 30
+    for n in sorted(names):
+        assert hasattr(lib, n), f"{n} declared in include/ but not exported"
+    # every declared function has a ctypes prototype
+    assert names <= set(abi.FUNCTIONS), sorted(names - set(abi.FUNCTIONS))
+    nm = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = {l.split()[-1] for l in nm.splitlines() if " T " in l}
+    assert names <= exported
+
+
+def test_terrain_library_loads():
+    from pyrit_amd import TERRAIN_PATH
+
+    lib = C.CDLL(TERRAIN_PATH)
+    for n in ("pyrgen_generate_host", "pyrgen_generate_device", "pyrgen_scratch_bytes", "pyrgen_height_host"):
+        assert hasattr(lib, n)
+
+
+LAYOUT_C = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "pyrit.h"
+#include "pyrit_hip_build.h"
+#define S(T) printf(#T " size %zu\n", sizeof(T));
+#define O(T, f) printf(#T "." #f " %zu\n", offsetof(T, f));
+int main(void) {
+  S(PyrRay) S(PyrHit) S(PyrCamera) S(PyrGeometryData) S(PyrInstanceData) S(PyrScene) S(PyrCreateInfo)
+  S(PyrVoxel) S(PyrFrameInfo) S(PyrTargets) S(PyrhipBuilt) S(PyrhipChunkBatch) S(PyrhipDag)
+  O(PyrInstanceData, bounds_min) O(PyrInstanceData, geometry) O(PyrInstanceData, bounds_max)
+  O(PyrInstanceData, mask) O(PyrInstanceData, flags)
+  O(PyrScene, instance_count) O(PyrScene, frame) O(PyrScene, origin) O(PyrScene, materials)
+  O(PyrScene, transparent_materials) O(PyrScene, textures) O(PyrScene, texture_count)
+  O(PyrCamera, projection) O(PyrCamera, scale) O(PyrCamera, near_plane)
+  O(PyrCreateInfo, cuda_context) O(PyrCreateInfo, max_instances) O(PyrCreateInfo, node_pool_bytes)
+  O(PyrCreateInfo, log)
+  O(PyrTargets, ray_mask) O(PyrTargets, detail_scale)
+  return 0;
+}
+"""
+
+# sizes the reference's ABI fixes (reference include/pyrit.h, checked there by
+# tests/abi_test.zig); computed from its field lists on LP64
+REFERENCE_SIZES = {"PyrRay": 32, "PyrHit": 16, "PyrCamera": 92, "PyrGeometryData": 32, "PyrInstanceData": 144,
+                   "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80}
+
+
+def test_struct_layouts_match_reference_and_mirrors():
+    from pyrit_amd import abi
+
+    with tempfile.TemporaryDirectory() as d:
+        src = os.path.join(d, "layout.c")
+        exe = os.path.join(d, "layout")
+        open(src, "w").write(LAYOUT_C)
+        subprocess.run(["gcc", "-std=c11", f"-I{ROOT}/include", src, "-o", exe], check=True)
+        out = subprocess.run([exe], capture_output=True, text=True, check=True).stdout
+    vals = {}
+    for line in out.splitlines():
+        k, kind, v = line.rsplit(" ", 2) if " size " in line else (line.rsplit(" ", 1)[0], "off", line.rsplit(" ", 1)[1])
+        vals[k] = int(v)
+    for name, size in REFERENCE_SIZES.items():
+        assert vals[name] == size, (name, vals[name], size)
+    mirrors = {"PyrCamera": abi.PyrCamera, "PyrGeometryData": abi.PyrGeometryData,
+               "PyrInstanceData": abi.PyrInstanceData, "PyrScene": abi.PyrScene, "PyrCreateInfo": abi.PyrCreateInfo,
+               "PyrFrameInfo": abi.PyrFrameInfo, "PyrTargets": abi.PyrTargets, "PyrhipBuilt": abi.PyrhipBuilt,
+               "PyrhipChunkBatch": abi.PyrhipChunkBatch, "PyrhipDag": abi.PyrhipDag}
+    for name, cls in mirrors.items():
+        assert C.sizeof(cls) == vals[name], name
+    for key, v in vals.items():
+        if "." in key:
+            sname, field = key.split(".")
+            if sname in mirrors:
+                assert getattr(mirrors[sname], field).offset == v, key
+    assert abi.VOXEL.itemsize == vals["PyrVoxel"] and abi.RAY.itemsize == vals["PyrRay"]
+    assert abi.HIT.itemsize == vals["PyrHit"]
+
+
+def test_error_strings_without_device():
+    from pyrit_amd import lib
+
+    L = lib()
+    assert L.pyr_result_string(0) == b"PYR_OK"
+    assert L.pyr_result_string(-3) == b"PYR_ERROR_OUT_OF_MEMORY"
+    # argument validation happens before any device work
+    from pyrit_amd import abi
+
+    out = abi.PyrhipBuilt()
+    assert L.pyrhip_build(2, 0, 0, 0, 0, 0, None, None, C.byref(out)) == -1
+    assert b"log2_size" in L.pyr_error_message()
+    assert L.pyrhip_build(5, 0, 0, 0, 0, 10, None, None, C.byref(out)) == -1

@@ -1,0 +1,168 @@
+"""Parity of the HIP build against the oracle: GPU-native arrays bit-exact
+against the serial restatement of the reference's GPU build, and canonical
+form equal to the reference's CPU builder (src/dag_builder.zig)."""
+import numpy as np
+import pytest
+
+import oracle_py as O
+from scenes import final_points, grid_to_points, random_points, scene_grid
+
+pytestmark = pytest.mark.gpu
+
+
+def _gpu():
+    from pyrit_amd import gpu
+
+    return gpu
+
+
+def assert_same_build(b, ref, log2):
+    assert b.voxel_count == ref["voxel_count"]
+    assert b.root == ref["root"]
+    assert b.node_words == len(ref["nodes"])
+    assert b.leaf_count == len(ref["leaves"])
+    assert np.array_equal(b.nodes(), ref["nodes"])
+    assert np.array_equal(b.leaves(), ref["leaves"])
+    assert np.array_equal(b.attrs(), ref["attrs"])
+    assert np.array_equal(b.keys(), ref["keys"])
+
+
+def test_config1_bit_exact(hip_device):
+    from pyrit_amd import workloads
+
+    print("case: config1 64^3 build", flush=True)
+    vox, _ = workloads.config1().generate_host()
+    ref = O.gb_build(6, vox)
+    b = _gpu().build(6, vox)
+    assert_same_build(b, ref, 6)
+    c = O.canon(b.nodes(), b.leaves(), b.root, 6)
+    cpu = O.build_points(6, vox)
+    assert c.same(cpu)
+    assert np.array_equal(b.attrs(), cpu.attrs)
+
+
+@pytest.mark.parametrize("log2,n,seed", [(3, 50, 1), (4, 300, 2), (5, 3000, 3), (6, 20000, 4), (7, 60000, 5),
+                                         (8, 200000, 6), (10, 100000, 7), (12, 50000, 8), (20, 30000, 9)])
+def test_random_points_bit_exact(hip_device, log2, n, seed):
+    print(f"case: random points log2={log2} n={n} seed={seed}", flush=True)
+    pts = random_points(n, log2, seed, dup_frac=0.2, delete_frac=0.05, out_frac=0.02)
+    ref = O.gb_build(log2, pts)
+    b = _gpu().build(log2, pts)
+    assert_same_build(b, ref, log2)
+    cpu = O.build_points(log2, final_points(pts, log2))
+    assert O.canon(b.nodes(), b.leaves(), b.root, log2).same(cpu)
+
+
+@pytest.mark.parametrize("radix_bits", [8, 9, 10, 11])
+def test_radix_widths_identical(hip_device, radix_bits):
+    print(f"case: radix bits {radix_bits}", flush=True)
+    pts = random_points(120000, 9, 77, dup_frac=0.3, delete_frac=0.1)
+    ref = O.gb_build(9, pts)
+    b = _gpu().build(9, pts, radix_bits=radix_bits)
+    assert_same_build(b, ref, 9)
+
+
+def test_scene_dense_and_empty_and_single(hip_device):
+    print("case: scene 64^3 / empty / single voxel", flush=True)
+    g = _gpu()
+    grid = scene_grid(64)
+    pts = grid_to_points(grid)
+    np.random.default_rng(3).shuffle(pts)
+    b = g.build(6, pts)
+    assert_same_build(b, O.gb_build(6, pts), 6)
+    e = g.build(4, np.zeros(0, O.VOXEL))
+    assert e.voxel_count == 0 and e.nodes().tolist() == [0, 0] and e.root == 0
+    s = g.build(4, np.array([(5, 6, 7, 42)], O.VOXEL))
+    assert_same_build(s, O.gb_build(4, np.array([(5, 6, 7, 42)], O.VOXEL)), 4)
+    # all records deleted or out of range
+    z = g.build(5, np.array([(1, 1, 1, 0), (99, 0, 0, 3), (-1, 2, 2, 4)], O.VOXEL))
+    assert z.voxel_count == 0 and z.nodes().tolist() == [0, 0]
+
+
+def test_edit_equals_rebuild(hip_device):
+    # cpu_test.zig:809-850 on the GPU path
+    print("case: edit = rebuild", flush=True)
+    g = _gpu()
+    base = np.array([(x, 0, z, 0x10) for z in range(32) for x in range(32)], O.VOXEL)
+    b1 = g.build(5, base)
+    edits = np.array([(10, y, 10, 0x20) for y in range(1, 20)] + [(x, 0, 3, 0) for x in range(8)]
+                     + [(20, 0, 20, 0x30)], O.VOXEL)
+    b2 = g.build(5, edits, old=b1)
+    r1 = O.gb_build(5, base)
+    r2 = O.gb_build(5, edits, r1["keys"], r1["attrs"])
+    assert_same_build(b2, r2, 5)
+    # large random edit on a large base
+    pts = random_points(50000, 8, 11)
+    b3 = g.build(8, pts)
+    ed = random_points(5000, 8, 12, dup_frac=0.1, delete_frac=0.3)
+    b4 = g.build(8, ed, old=b3)
+    r3 = O.gb_build(8, pts)
+    assert_same_build(b4, O.gb_build(8, ed, r3["keys"], r3["attrs"]), 8)
+
+
+def _shell_batch(k=5, cap=8000, empty=3):
+    from test_oracle import _shell_batch as sb
+
+    return sb(k, cap, empty)
+
+
+def test_batch_reference_layout_bit_exact(hip_device):
+    # cpu_test.zig:1001-1060 layout (segments of `capacity`)
+    print("case: chunk batch, capacity layout", flush=True)
+    vox, cnt, offs, cap = _shell_batch()
+    ref, rroots, rfirst = O.gb_build_chunks(5, vox, offs, capacity=cap)
+    b, roots, first = _gpu().build_chunks(5, vox, offs, capacity=cap)
+    assert_same_build(b, ref, 5)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+    assert roots[3] == 0xFFFFFFFF
+
+
+def test_batch_packed_many_chunks(hip_device):
+    print("case: packed batch of 300 chunks", flush=True)
+    rng = np.random.default_rng(4)
+    chunks = [random_points(int(rng.integers(0, 800)), 5, int(rng.integers(1 << 30)), dup_frac=0.1,
+                            delete_frac=0.05) for _ in range(300)]
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in chunks])]).astype(np.uint32)
+    vox = np.concatenate(chunks)
+    ref, rroots, rfirst = O.gb_build_chunks(5, vox, offs)
+    b, roots, first = _gpu().build_chunks(5, vox, offs)
+    assert_same_build(b, ref, 5)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+
+
+def test_mixed_size_batch(hip_device):
+    print("case: mixed-size batch", flush=True)
+    from pyrit_amd import workloads
+
+    w = workloads.config3(seed=5, count=24, edges=(8, 16, 32, 64))
+    vox, offs64 = w.generate_host()
+    offs = offs64.astype(np.uint32)
+    sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
+    ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs, chunk_log2=sizes)
+    b, roots, first = _gpu().build_chunks(6, vox, offs, chunk_log2=sizes)
+    assert_same_build(b, ref, 6)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+    nodes, leaves = b.nodes(), b.leaves()
+    attrs = b.attrs()
+    for c in range(len(sizes)):
+        seg = vox[offs[c]:offs[c + 1]]
+        if len(seg) == 0:
+            assert roots[c] == 0xFFFFFFFF
+            continue
+        cpu = O.build_points(int(sizes[c]), seg)
+        got = O.canon(nodes, leaves, int(roots[c]), int(sizes[c]))
+        assert got.same(cpu), c
+        assert np.array_equal(attrs[first[c]:first[c] + cpu.voxel_count], cpu.attrs)
+
+
+def test_device_generator_matches_host(hip_device):
+    print("case: terrain generator host == device", flush=True)
+    from pyrit_amd import workloads
+
+    for w in (workloads.config1(), workloads.config3(seed=8, count=16, edges=(8, 16, 32, 64)),
+              workloads.config2(log2_size=6, caves=40), workloads.config4(log2_size=7, caves=30)):
+        hv, ho = w.generate_host()
+        dv, do, total = w.generate_device()
+        assert total == len(hv) and np.array_equal(do, ho)
+        got = dv.download(O.VOXEL, total)
+        assert np.array_equal(got.view(np.uint32), hv.view(np.uint32)), w.name
