@@ -1,0 +1,348 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X sparse-voxel-DAG hot path (one GPU).
+
+Headline (`value`): full-depth DAG build of the 1024^3 grid (heightfield with
+4 octaves, amplitude 256, 2000 carved caves, depth-band materials) in
+Gvoxels/s - input PyrVoxel records resident in HBM, one step = one complete
+pyrhip_build (encode, radix sort, unique, bricks, leaf and node dedup,
+compaction, final read-back of the sizes).
+
+Beside it: traversal of the 4096^3 skin DAG (1920x1080 primary rays, 30
+seeded poses, plus one shadow ray per hit), the batched build of 1024 uneven
+chunks against one build call per chunk, the roofline of the dominant kernel
+and the CPU baseline (the oracle's restatement of the reference's CPU builder
+and traversal, timed on this host).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import math
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+METRIC = "DAG build Gvoxels/s + traversal Mrays/s, % HBM peak; p99 chunk-build latency"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--quick", action="store_true", help="smaller traversal/batch sections")
+    ap.add_argument("--only", default="", help="comma list of sections: build,trace,batch,cpu,probe")
+    ap.add_argument("--radix-bits", type=int, default=0)
+    ap.add_argument("--log2", type=int, default=10, help="edge of the headline grid (10 = 1024^3)")
+    return ap.parse_args()
+
+
+def build_section(args, out):
+    from pyrit_amd import abi, gpu, lib, workloads
+    from pyrit_amd.device import Events, synchronize
+
+    w = workloads.config2(log2_size=args.log2)
+    log(f"generating {w.name} on the device")
+    vox, offs, n = w.generate_device()
+    log(f"{n} voxels resident ({n * 16 / 1e9:.2f} GB)")
+    L = lib()
+    t = abi.PyrhipBuildTuning()
+    t.radix_bits = args.radix_bits
+    built = abi.PyrhipBuilt()
+
+    def one():
+        abi.check(L, L.pyrhip_build(args.log2, 0, 0, 0, vox.ptr, n, C.byref(t), None, C.byref(built)), "build")
+
+    for i in range(args.warmup):
+        one()
+        info = (built.voxel_count, built.node_words, built.leaf_count)
+        L.pyrhip_built_free(C.byref(built), None)
+    synchronize()
+    gpu.set_timing(True)
+    ev = Events()
+    ev.start()
+    sort_ms = []
+    phases = []
+    for i in range(args.steps):
+        one()
+        tm = gpu.last_build_times()
+        phases.append(tm)
+        sort_ms.append(tm["sort_ms"])
+        info = (built.voxel_count, built.node_words, built.leaf_count)
+        L.pyrhip_built_free(C.byref(built), None)
+    ms = ev.stop() / args.steps
+    synchronize()
+    gpu.set_timing(False)
+    passes = phases[-1]["sort_passes"]
+    rb = phases[-1]["radix_bits"]
+    pass_ms = float(np.mean(sort_ms)) / passes
+    pass_bytes = 24.0 * n  # read + write of one (u64 key, u32 value) pair per voxel
+    achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
+    U, W, Lc = info
+    B_est = None
+    out["build"] = {
+        "workload": w.name, "voxels": n, "unique": U, "node_words": W, "leaves": Lc, "ms": ms,
+        "gvoxels_per_s": n / (ms * 1e-3) / 1e9, "sort_passes": passes, "radix_bits": rb,
+        "phase_ms": {k: float(np.mean([p[k] for p in phases])) for k in
+                     ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
+        "sort_pass_ms": pass_ms,
+    }
+    # algorithmic bytes of the whole build (model: encode 16N read + 12N write,
+    # P passes x 24N, unique 12N read + 12U write, brick pass 8U, outputs)
+    alg = 28.0 * n + passes * 24.0 * n + 12.0 * n + 12.0 * U + 8.0 * U + 4.0 * W + 8.0 * Lc
+    out["build"]["algorithmic_GBps"] = alg / (ms * 1e-3) / 1e9
+    out["roofline"] = {"kernel": "k_sort_pass<%d>" % rb, "bound": "hbm", "achieved": achieved,
+                       "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+                       "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms}
+    vox.free()
+    return n, ms
+
+
+def camera_poses(w, count, width, height, seed=30):
+    from pyrit_amd import abi, lib, workloads
+
+    L = lib()
+    n = 1 << w.chunks[0].log2_size
+    s = n / 4096.0
+    rng = workloads.SplitMix64(seed)
+    cams = []
+    for _ in range(count):
+        x = int(n * 0.0625 + rng.uniform() * n * 0.875)
+        z = int(n * 0.0625 + rng.uniform() * n * 0.875)
+        h = workloads.height_host(w, 0, x, z)
+        y = h + (50 + 350 * rng.uniform()) * s
+        yaw = 2 * math.pi * rng.uniform()
+        pitch = math.radians(-40 + 35 * rng.uniform())
+        eye = (float(x), float(y), float(z))
+        tgt = (x + math.cos(pitch) * math.sin(yaw), y + math.sin(pitch), z + math.cos(pitch) * math.cos(yaw))
+        c = abi.PyrCamera()
+        L.pyr_camera_look_at(C.byref(c), (C.c_float * 3)(*eye), (C.c_float * 3)(*tgt), (C.c_float * 3)(0, 1, 0))
+        L.pyr_camera_perspective(C.byref(c), math.pi / 3, width, height, 0.0)
+        cams.append(c)
+    return cams
+
+
+def trace_section(args, out):
+    from pyrit_amd import abi, gpu, lib, workloads
+    from pyrit_amd.device import Buffer, Events, synchronize
+
+    log2 = 12 if not args.quick else 10
+    w = workloads.config4(log2_size=log2, caves=20000 if log2 == 12 else 2000)
+    log(f"generating {w.name}")
+    vox, offs, n = w.generate_device()
+    L = lib()
+    built = abi.PyrhipBuilt()
+    t0 = time.time()
+    abi.check(L, L.pyrhip_build(log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build config4")
+    log(f"built {n} skin voxels in {time.time() - t0:.2f}s: {built.node_words} words, {built.leaf_count} leaves")
+    vox.free()
+    dag = abi.PyrhipDag()
+    dag.nodes, dag.leaves, dag.attrs = built.nodes, built.leaves, built.attrs
+    dag.root, dag.log2_size, dag.default_attribute = built.root, log2, 1
+    W, H = 1920, 1080
+    poses = 30 if not args.quick else 6
+    cams = camera_poses(w, poses, W, H)
+    hits = Buffer(W * H * 16)
+    rays = Buffer(W * H * 32)
+    occ = Buffer(W * H * 4)
+    sun = workloads.normalize(workloads.SUN)
+    sunc = (C.c_float * 3)(*[float(v) for v in sun])
+    for c in cams[:2]:  # warm-up
+        abi.check(L, L.pyrhip_trace_camera(C.byref(dag), C.byref(c), hits.ptr, 0, None), "trace")
+    synchronize()
+    ev = Events()
+    ev.start()
+    for c in cams:
+        abi.check(L, L.pyrhip_trace_camera(C.byref(dag), C.byref(c), hits.ptr, 0, None), "trace")
+    prim_ms = ev.stop()
+    # primary + shadow: rays materialised, primary from them, shadow per hit
+    hit_total = 0
+    ev.start()
+    for c in cams:
+        abi.check(L, L.pyrhip_camera_rays(C.byref(c), rays.ptr, None), "rays")
+        abi.check(L, L.pyrhip_trace_dag(C.byref(dag), rays.ptr, hits.ptr, W * H, 0, None), "trace")
+        abi.check(L, L.pyrhip_trace_shadow(C.byref(dag), rays.ptr, hits.ptr, sunc, occ.ptr, W * H, None),
+                  "shadow")
+    both_ms = ev.stop()
+    h = hits.download(abi.HIT, W * H)
+    hit_total = int((h["instance"] != 0xFFFFFFFF).sum())
+    nrays = W * H * poses
+    out["traversal"] = {
+        "workload": w.name, "dag_voxels": built.voxel_count, "dag_node_words": built.node_words,
+        "dag_leaves": built.leaf_count, "resolution": [W, H], "poses": poses,
+        "primary_ms_per_frame": prim_ms / poses, "primary_grays_per_s": nrays / (prim_ms * 1e-3) / 1e9,
+        "primary_plus_shadow_ms_per_frame": both_ms / poses,
+        "primary_plus_shadow_mrays_per_s": (nrays + hit_total * poses) / (both_ms * 1e-3) / 1e6,
+        "last_frame_hit_fraction": hit_total / (W * H),
+    }
+    L.pyrhip_built_free(C.byref(built), None)
+    for b in (hits, rays, occ):
+        b.free()
+    return dag, w, cams
+
+
+def batch_section(args, out):
+    from pyrit_amd import abi, lib, workloads
+    from pyrit_amd.device import Buffer, Events, synchronize, to_device
+
+    count = 1024 if not args.quick else 128
+    w = workloads.config3(count=count)
+    log(f"generating {w.name}")
+    vox, offs, n = w.generate_device()
+    sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
+    L = lib()
+    d_offs = to_device(offs.astype(np.uint32))
+    d_sizes = to_device(sizes)
+    batch = abi.PyrhipChunkBatch()
+    batch.count, batch.capacity, batch.voxels, batch.offsets = count, 0, vox.ptr, d_offs.ptr
+    batch.total, batch.chunk_log2 = n, d_sizes.ptr
+    roots = np.zeros(count, np.uint32)
+    first = np.zeros(count, np.uint32)
+    built = abi.PyrhipBuilt()
+    top = int(sizes.max())
+
+    def batched():
+        abi.check(L, L.pyrhip_build_chunks(top, C.byref(batch), None, None, C.byref(built),
+                                           roots.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           first.ctypes.data_as(C.POINTER(C.c_uint32))), "batch")
+        L.pyrhip_built_free(C.byref(built), None)
+
+    batched()
+    synchronize()
+    ev = Events()
+    reps = 3
+    ev.start()
+    for _ in range(reps):
+        batched()
+    b_ms = ev.stop() / reps
+    # one build call per chunk on the same GPU (same inputs)
+    lat = []
+    ev.start()
+    for c in range(count):
+        t0 = time.perf_counter()
+        cnt = int(offs[c + 1] - offs[c])
+        abi.check(L, L.pyrhip_build(int(sizes[c]), 0, 0, 0, vox.ptr + int(offs[c]) * 16, cnt, None, None,
+                                    C.byref(built)), "chunk")
+        L.pyrhip_built_free(C.byref(built), None)
+        lat.append((time.perf_counter() - t0) * 1e3)
+    p_ms = ev.stop()
+    out["batched"] = {
+        "workload": w.name, "chunks": count, "voxels": n, "batched_ms": b_ms,
+        "batched_gvoxels_per_s": n / (b_ms * 1e-3) / 1e9, "per_chunk_ms": p_ms,
+        "per_chunk_gvoxels_per_s": n / (p_ms * 1e-3) / 1e9, "speedup_vs_per_chunk": p_ms / b_ms,
+        "per_chunk_latency_ms_p50": float(np.percentile(lat, 50)),
+        "per_chunk_latency_ms_p99": float(np.percentile(lat, 99)),
+    }
+    for b in (vox, d_offs, d_sizes):
+        b.free()
+
+
+def cpu_section(args, out):
+    """The oracle's restatement of dag_builder.buildPoints (single-threaded by
+    construction in the reference) on a bounded sample of the same scheme."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    from pyrit_amd import workloads
+
+    w = workloads.config2(log2_size=8, caves=2000 // 64)
+    vox, _ = w.generate_host()
+    reps, t_total, nvox = 0, 0.0, 0
+    t_end = time.time() + (12.0 if not args.quick else 3.0)
+    while time.time() < t_end:
+        t0 = time.perf_counter()
+        O.build_points(8, vox)
+        t_total += time.perf_counter() - t0
+        nvox += len(vox)
+        reps += 1
+    cores = 1
+    out["cpu_baseline"] = {"value": nvox / t_total / 1e9, "unit": "Gvoxels/s", "cores": cores, "kind": "port",
+                           "sample": f"oracle buildPoints restatement on the 256^3 instance of the 1024^3 scheme "
+                                     f"({len(vox)} voxels) x{reps}, {t_total:.1f}s",
+                           "host_nproc": os.cpu_count()}
+    try:
+        model = [l.split(":", 1)[1].strip() for l in open("/proc/cpuinfo") if l.startswith("model name")][0]
+        out["cpu_baseline"]["host_cpu"] = model
+    except Exception:
+        pass
+
+
+def probe_section(args, out):
+    """Copy-kernel calibration of the HBM counters (also a bandwidth probe)."""
+    from pyrit_amd import abi, lib
+    from pyrit_amd.device import Buffer, Events, synchronize
+
+    L = lib()
+    nbytes = 4 << 30
+    a, b = Buffer(nbytes), Buffer(nbytes)
+    abi.check(L, L.pyrhip_copy_probe(a.ptr, b.ptr, nbytes, None), "probe")
+    synchronize()
+    ev = Events()
+    ev.start()
+    for _ in range(5):
+        abi.check(L, L.pyrhip_copy_probe(a.ptr, b.ptr, nbytes, None), "probe")
+    ms = ev.stop() / 5
+    out["copy_probe"] = {"bytes_read": nbytes, "bytes_written": nbytes, "ms": ms,
+                         "GBps": 2 * nbytes / (ms * 1e-3) / 1e9}
+    a.free()
+    b.free()
+
+
+def main() -> int:
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+                          "n_gpus": args.gpus}), flush=True)
+        return 2
+    rank = int(os.environ.get("RANK", "0"))
+    from pyrit_amd import device
+
+    if not device.gpu_available():
+        print(json.dumps({"error": "no HIP device visible"}), flush=True)
+        return 1
+    only = set(args.only.split(",")) if args.only else {"build", "trace", "batch", "cpu", "probe"}
+    out = {}
+    n, ms = 0, float("nan")
+    if "probe" in only:
+        probe_section(args, out)
+        log(f"copy probe {out['copy_probe']['GBps']:.0f} GB/s")
+    if "build" in only:
+        n, ms = build_section(args, out)
+        log(f"build {out['build']['gvoxels_per_s']:.2f} Gvox/s, sort pass {out['roofline']['achieved']:.0f} GB/s")
+    if "batch" in only:
+        batch_section(args, out)
+        log(f"batched {out['batched']['batched_gvoxels_per_s']:.2f} Gvox/s, "
+            f"x{out['batched']['speedup_vs_per_chunk']:.1f} vs per chunk")
+    if "trace" in only:
+        trace_section(args, out)
+        log(f"primary {out['traversal']['primary_grays_per_s']:.2f} Grays/s")
+    if "cpu" in only and rank == 0:
+        cpu_section(args, out)
+    res = {
+        "metric": METRIC, "value": out.get("build", {}).get("gvoxels_per_s"), "unit": "Gvoxels/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "u64", "data": "synthetic",
+        "config": {"workload": "1024^3 heightfield grid (4 octaves, amplitude 256, 2000 caves), full-depth build",
+                   "voxels": n, "log2_size": args.log2, "parallelism": "single GPU"},
+    }
+    for k in ("roofline", "cpu_baseline", "build", "traversal", "batched", "copy_probe"):
+        if k in out:
+            res[k] = out[k]
+    print(json.dumps(res), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -90,6 +90,10 @@ typedef struct PyrhipBuildTimes {
     uint32_t sort_passes, radix_bits;
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
+/* Plain device-to-device copy kernel of `bytes` (multiple of 8): the
+ * calibration probe for HBM byte counters (reads = writes = bytes). No
+ * counterpart in the reference. */
+PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream);
 PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out);
 
 /* One DAG in device memory for pyrhip_trace_dag (a single instance with the

@@ -135,6 +135,7 @@ FUNCTIONS = {
                                   P(u32)]),
     "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),
     "pyrhip_set_timing": (None, [i32]),
+    "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),
     "pyrhip_last_build_times": (None, [P(PyrhipBuildTimes)]),
     "pyrhip_trace_dag": (i32, [P(PyrhipDag), u64, u64, u32, u32, vp]),
     "pyrhip_trace_camera": (i32, [P(PyrhipDag), P(PyrCamera), u64, u32, vp]),

@@ -292,30 +292,48 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         uint64_t* st = a.status + d;
         uint64_t excl = 0;
         if (tile == 0) {
-            __hip_atomic_store(&st[0], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELEASE,
+            __hip_atomic_store(&st[0], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELEASE,
+            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+            // predecessors are polled four at a time (independent loads in
+            // flight) and summed newest to oldest until an inclusive prefix
             int64_t t = (int64_t)tile - 1;
             uint32_t spins = 0;
             while (t >= 0) {
-                uint64_t w = __hip_atomic_load(&st[(uint64_t)t * R], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-                uint32_t f = status_flag(w, a.epoch);
-                if (f == 0) {
+                uint64_t w[4];
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    w[q] = t - q >= 0 ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
+                                                          __HIP_MEMORY_SCOPE_AGENT)
+                                      : status_word(kFlagPre, a.epoch, 0);
+                int used = 0;
+                bool done = false, wait = false;
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    if (done || wait) continue;
+                    const uint32_t f = status_flag(w[q], a.epoch);
+                    if (f == 0) {
+                        wait = true;
+                        continue;
+                    }
+                    excl += w[q] & kValueMask;
+                    used++;
+                    done = f == 2;
+                }
+                if (done) break;
+                t -= used;
+                if (wait) {
                     if (++spins > kSpinLimit) {
                         atomicOr(a.err, 2u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
-                    continue;
                 }
-                excl += w & kValueMask;
-                if (f == 2) break;
-                t--;
             }
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, excl + tot[j]),
-                               __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
         s_dst[d] = (uint64_t)a.digit_base[d] + excl;
     }
@@ -496,8 +514,18 @@ __global__ void __launch_bounds__(kBlock) k_hash_clear(const uint32_t* counters,
     }
 }
 
+// Owner = smallest index (gbuild.zig:450-454). The atomic is skipped when the
+// slot already holds an index <= i: a stale read can only be larger than the
+// current value, so skipping never loses a smaller index.
+__device__ __forceinline__ void owner_min(uint32_t* owner, uint64_t h, uint32_t i) {
+    if (load_agent(&owner[h]) > i) atomicMin(&owner[h], i);
+}
+
 // leaf_insert + owner (gbuild.zig:436-454): CAS of the mask (0 = empty slot),
-// then the smallest brick index with that mask becomes its owner
+// then the smallest brick index with that mask becomes its owner. Solid
+// terrain makes one mask (the full brick) dominate; lanes of a wave holding
+// the same mask as the wave's first lane reuse its slot, and slots are read
+// before any CAS, so a hot mask costs loads instead of serialised atomics.
 __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, const uint64_t* masks, uint64_t* table,
                                                          uint32_t* owner, uint32_t* slot, uint32_t* ch_count,
                                                          uint64_t max_cap, uint32_t brick_cap) {
@@ -507,18 +535,27 @@ __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, cons
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n; b += stride) {
         const uint64_t mask = masks[b];
+        const uint64_t m0 = first_lane64(mask);
+        const uint32_t b0 = first_lane32(b);
+        const bool follower = mask == m0 && b != b0;
         uint64_t h = mix64(mask) & m;
-        uint64_t probes = 0;
-        for (;;) {
-            uint64_t prev = atomicCAS((unsigned long long*)&table[h], 0ull, (unsigned long long)mask);
-            if (prev == 0 || prev == mask) break;
-            h = (h + 1) & m;
-            if (++probes > cap) {
-                atomicOr(&counters[C_ERR], 4u);
-                break;
+        if (!follower) {
+            uint64_t probes = 0;
+            for (;;) {
+                uint64_t cur = load_agent(&table[h]);
+                if (cur == 0) cur = atomicCAS((unsigned long long*)&table[h], 0ull, (unsigned long long)mask);
+                if (cur == 0 || cur == mask) break;
+                h = (h + 1) & m;
+                if (++probes > cap) {
+                    atomicOr(&counters[C_ERR], 4u);
+                    break;
+                }
             }
+            owner_min(owner, h, b);
         }
-        atomicMin(&owner[h], b);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1;
+        const uint64_t h0 = __shfl(h, leader, kWave);
+        if (follower) h = h0;  // b0 < b already competes for the owner
         slot[b] = (uint32_t)h;
         ch_count[b] = (uint32_t)__popcll(mask);
     }
@@ -601,8 +638,6 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
     __shared__ uint32_t s_tile, s_wave[4];
     __shared__ uint64_t s_b;
     const uint32_t n = min(sc.counters[C_NCH + a.lv], a.ch_cap);
-    const uint64_t want = hash_cap_for(n);
-    const uint64_t cap = want < a.table_cap ? want : a.table_cap, hm = cap - 1;
     const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
     for (;;) {
         const uint32_t tile = next_tile(sc, &s_tile);
@@ -653,31 +688,64 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
             a.node_len[p] = len;
             a.par.key[p] = pk;
             a.par.count[p] = count;
-            __threadfence();  // words visible before the index is published
+        }
+        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_NCH + a.lv + 1] = (uint32_t)(pre + total);
+    }
+}
+
+// node_insert/node_owner (gbuild.zig:527-554): the tuples written by
+// k_level_build (complete at this launch) are inserted into the level's hash
+// as index + 1; a collision compares the stored tuple word by word. Hot tuples
+// (identical full nodes of solid regions) are handled like hot leaf masks.
+__global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, LevelArgs a) {
+    const uint32_t np = min(counters[C_NCH + a.lv + 1], a.par_cap);
+    const uint32_t nch = min(counters[C_NCH + a.lv], a.ch_cap);
+    const uint64_t want = hash_cap_for(nch);
+    const uint64_t cap = want < a.table_cap ? want : a.table_cap, hm = cap - 1;
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) {
+        const uint32_t len = a.node_len[p];
+        const uint32_t* src = a.node_tmp + (uint64_t)p * kNodeStride;
+        uint32_t w[kNodeStride];
+#pragma unroll
+        for (int q = 0; q < kNodeStride; q++) w[q] = (uint32_t)q < len ? src[q] : 0u;
+        // same tuple as the wave's first lane?
+        const uint32_t p0 = first_lane32(p);
+        bool same0 = first_lane32(len) == len;
+#pragma unroll
+        for (int q = 0; q < kNodeStride; q++) same0 &= first_lane32(w[q]) == w[q];
+        const bool follower = same0 && p != p0;
+        uint64_t h = 0;
+        if (!follower) {
             uint64_t hv = len;
             for (uint32_t q = 0; q < len; q++) hv = mix64(hv ^ w[q]);
-            uint64_t h = hv & hm;
+            h = hv & hm;
             uint64_t probes = 0;
             for (;;) {
-                uint32_t prev = atomicCAS(&a.table[h], 0u, p + 1);
-                if (prev == 0) break;
-                const uint32_t other = prev - 1;
-                if (load_agent(&a.node_len[other]) == len) {
+                uint32_t cur = load_agent(&a.table[h]);
+                if (cur == 0) {
+                    cur = atomicCAS(&a.table[h], 0u, p + 1);
+                    if (cur == 0) break;
+                }
+                const uint32_t other = cur - 1;
+                if (a.node_len[other] == len) {
                     const uint32_t* ow = a.node_tmp + (uint64_t)other * kNodeStride;
                     bool same = true;
-                    for (uint32_t q = 0; q < len; q++) same &= load_agent(&ow[q]) == w[q];
+                    for (uint32_t q = 0; q < len; q++) same &= ow[q] == w[q];
                     if (same) break;
                 }
                 h = (h + 1) & hm;
                 if (++probes > cap) {
-                    atomicOr(&sc.counters[C_ERR], 8u);
+                    atomicOr(&counters[C_ERR], 8u);
                     break;
                 }
             }
-            atomicMin(&a.owner[h], p);
-            a.slot[p] = (uint32_t)h;
+            owner_min(a.owner, h, p);
         }
-        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_NCH + a.lv + 1] = (uint32_t)(pre + total);
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1;
+        const uint64_t h0 = __shfl(h, leader, kWave);
+        if (follower) h = h0;
+        a.slot[p] = (uint32_t)h;
     }
 }
 
@@ -750,6 +818,14 @@ __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t* counters, uint32_
         counters[C_ROOT] = root;
         counters[C_WORDS] = words;
     }
+}
+
+// plain streaming copy of n u64 words: the calibration kernel for the HBM
+// byte counters (bytes read = bytes written = 8n)
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+                                                        uint64_t n) {
+    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -1003,6 +1079,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                            counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, cap);
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_build, dim3(g), dim3(kBlock), 0, s, sc, la);
+        hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(nb, cells(lv + 1)), kBlock * 4, 4096)),
+                           dim3(kBlock), 0, s, counters, la);
         ScanCtx sc2{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_compact, dim3(grid_for(std::min(nb, cells(lv + 1)), kScanTile, 2048)),
                            dim3(kBlock), 0, s, sc2, la, nodes_out);
@@ -1108,6 +1186,17 @@ PYR_API void pyrhip_built_free(PyrhipBuilt* b, void* stream) {
     hipFree((void*)(uintptr_t)b->keys);
     hipFree((void*)(uintptr_t)b->attrs);
     memset(b, 0, sizeof(*b));
+}
+
+PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream) {
+    diag_clear();
+    if (!src || !dst || (bytes & 7)) return fail(PYR_ERROR_INVALID_ARGUMENT, "copy probe needs 8-byte multiples");
+    const uint64_t n = bytes / 8;
+    if (n)
+        hipLaunchKernelGGL(k_copy_probe, dim3(grid_for(n, kBlock * 8, 8192)), dim3(kBlock), 0, (hipStream_t)stream,
+                           (const uint64_t*)(uintptr_t)src, (uint64_t*)(uintptr_t)dst, n);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "copy probe: %s", hipGetErrorString(e));
 }
 
 PYR_API void pyrhip_set_timing(int enable) { g_timing = enable != 0; }

@@ -56,6 +56,15 @@ __device__ __forceinline__ void store_agent(T* p, T v) {
 
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 
+// value of the lowest active lane (readfirstlane returns int: convert through
+// uint32_t so nothing is sign-extended)
+__device__ __forceinline__ uint32_t first_lane32(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
+}
+__device__ __forceinline__ uint64_t first_lane64(uint64_t v) {
+    return ((uint64_t)first_lane32((uint32_t)(v >> 32)) << 32) | (uint64_t)first_lane32((uint32_t)v);
+}
+
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
     const uint32_t lane = lane_id();
 #pragma unroll
@@ -104,19 +113,19 @@ __device__ __forceinline__ uint64_t lookback_wave(uint64_t* status, uint32_t til
     const uint32_t lane = lane_id();
     if (tile == 0) {
         if (lane == 0)
-            __hip_atomic_store(&status[0], status_word(kFlagPre, epoch, aggregate), __ATOMIC_RELEASE,
+            __hip_atomic_store(&status[0], status_word(kFlagPre, epoch, aggregate), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         return 0;
     }
     if (lane == 0)
-        __hip_atomic_store(&status[tile], status_word(kFlagAgg, epoch, aggregate), __ATOMIC_RELEASE,
+        __hip_atomic_store(&status[tile], status_word(kFlagAgg, epoch, aggregate), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     uint64_t excl = 0;
     int64_t pos = (int64_t)tile - 1;
     uint32_t spins = 0;
     while (true) {
         int64_t idx = pos - (int64_t)lane;
-        uint64_t w = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT)
+        uint64_t w = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : status_word(kFlagPre, epoch, 0);
         uint32_t f = status_flag(w, epoch);
         if (__ballot(f == 0) != 0) {
@@ -138,7 +147,7 @@ __device__ __forceinline__ uint64_t lookback_wave(uint64_t* status, uint32_t til
         pos -= kWave;
     }
     if (lane == 0)
-        __hip_atomic_store(&status[tile], status_word(kFlagPre, epoch, excl + aggregate), __ATOMIC_RELEASE,
+        __hip_atomic_store(&status[tile], status_word(kFlagPre, epoch, excl + aggregate), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     return excl;
 }

@@ -79,6 +79,32 @@ def test_scene_dense_and_empty_and_single(hip_device):
     assert z.voxel_count == 0 and z.nodes().tolist() == [0, 0]
 
 
+def test_wave_leader_mask_high_bits(hip_device):
+    # regression: one wave of bricks whose first brick has mask 0x80000000 and
+    # whose others have 0xFFFFFFFF80000000 - distinct leaves that a
+    # sign-extended lane broadcast would merge
+    print("case: leaf masks differing only in the high dword", flush=True)
+    pts = []
+    for m in range(64):
+        bx, by, bz = _decode(m)
+        bits = [31] if m == 0 else [31] + list(range(32, 64))
+        for bit in bits:
+            pts.append((bx * 4 + (bit & 3), by * 4 + ((bit >> 2) & 3), bz * 4 + (bit >> 4), 5 + m))
+    pts = np.array(pts, O.VOXEL)
+    ref = O.gb_build(5, pts)
+    b = _gpu().build(5, pts)
+    assert_same_build(b, ref, 5)
+    assert len(ref["leaves"]) == 2
+
+
+def _decode(m):
+    import ctypes as C
+
+    out = (C.c_uint32 * 3)()
+    O.lib().or_morton_decode(m, out)
+    return tuple(out)
+
+
 def test_edit_equals_rebuild(hip_device):
     # cpu_test.zig:809-850 on the GPU path
     print("case: edit = rebuild", flush=True)
