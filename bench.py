@@ -50,6 +50,21 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic(kernel: str, voxels: int):
+    """HBM bytes per launch of `kernel` on this workload from the committed
+    rocprofv3 counter summary (profiles/traffic.json: FETCH_SIZE and WRITE_SIZE
+    scaled by the copy-probe calibration), or None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    for e in data.get("kernels", []):
+        if e.get("kernel") == kernel and e.get("voxels") == voxels:
+            return e.get("bytes_per_launch")
+    return None
+
+
 def build_section(args, out):
     from pyrit_amd import abi, gpu, lib, workloads
     from pyrit_amd.device import Events, synchronize
@@ -88,25 +103,31 @@ def build_section(args, out):
     gpu.set_timing(False)
     passes = phases[-1]["sort_passes"]
     rb = phases[-1]["radix_bits"]
+    rec = phases[-1]["record_bytes"]  # 8: packed key << 32 | attribute, 12: key + attribute arrays
+    items = phases[-1]["sort_items"]
     pass_ms = float(np.mean(sort_ms)) / passes
-    pass_bytes = 24.0 * n  # read + write of one (u64 key, u32 value) pair per voxel
+    pass_bytes = 2.0 * rec * n  # every record read once and written once per pass
     achieved = pass_bytes / (pass_ms * 1e-3) / 1e9
     U, W, Lc = info
-    B_est = None
     out["build"] = {
         "workload": w.name, "voxels": n, "unique": U, "node_words": W, "leaves": Lc, "ms": ms,
         "gvoxels_per_s": n / (ms * 1e-3) / 1e9, "sort_passes": passes, "radix_bits": rb,
+        "record_bytes": rec, "sort_items": items,
         "phase_ms": {k: float(np.mean([p[k] for p in phases])) for k in
                      ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "sort_pass_ms": pass_ms,
     }
-    # algorithmic bytes of the whole build (model: encode 16N read + 12N write,
-    # P passes x 24N, unique 12N read + 12U write, brick pass 8U, outputs)
-    alg = 28.0 * n + passes * 24.0 * n + 12.0 * n + 12.0 * U + 8.0 * U + 4.0 * W + 8.0 * Lc
+    # algorithmic bytes of the whole build: encode 16N read + rec*N write, P
+    # passes x 2*rec*N, unique rec*N read + 12U write, brick pass 8U read,
+    # outputs 4W + 8L (brick masks, tables and levels are small beside these)
+    alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 12.0 * U + 8.0 * U + 4.0 * W + 8.0 * Lc
+    out["build"]["algorithmic_bytes"] = alg
     out["build"]["algorithmic_GBps"] = alg / (ms * 1e-3) / 1e9
-    out["roofline"] = {"kernel": "k_sort_pass<%d>" % rb, "bound": "hbm", "achieved": achieved,
+    kname = "k_sort_pass<%d,%d,%s>" % (rb, items, "true" if rec == 8 else "false")
+    out["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-                       "traffic": None, "bytes_per_launch": pass_bytes, "avg_launch_ms": pass_ms}
+                       "traffic": measured_traffic(kname, n), "bytes_per_launch": pass_bytes,
+                       "avg_launch_ms": pass_ms}
     vox.free()
     return n, ms
 

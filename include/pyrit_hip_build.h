@@ -26,7 +26,8 @@ extern "C" {
 /* Built DAG in the GPU-native layout (src/gpu_build.zig:25-47): nodes level by
  * level bottom-up, word refs absolute, leaves in first-appearance Morton order,
  * attributes in DFS rank order. All buffers are device memory owned by the
- * caller (free with pyrhip_built_free). `keys`/`attrs` hold the sorted unique
+ * caller, taken from the device's stream-ordered memory pool: release them
+ * with pyrhip_built_free (hipFreeAsync), not hipFree. `keys`/`attrs` hold the sorted unique
  * voxels (brick Morton << 6 | bit) for later edits. */
 typedef struct PyrhipBuilt {
     uint32_t log2_size;
@@ -59,10 +60,14 @@ typedef struct PyrhipChunkBatch {
     uint64_t chunk_log2;
 } PyrhipChunkBatch;
 
-/* Tuning of one build; zeros select the defaults (8-bit digits). */
+/* Tuning of one build; zeros select the defaults. Every setting gives the
+ * same output; they trade passes, LDS footprint and occupancy. */
 typedef struct PyrhipBuildTuning {
-    uint32_t radix_bits;   /* 8 (default), 9, 10 or 11 */
-    uint32_t reserved[3];
+    uint32_t radix_bits;   /* digit width of the LSD passes: 8 (default), 9, 10 or 11 */
+    uint32_t sort_items;   /* keys per thread of a sort tile (256 threads): 8, 12, 16, 20 (default), 24 */
+    uint32_t pair_layout;  /* 0: pack key << 32 | attribute when keys fit 32 bits (8-bit digits);
+                              1: always separate key and attribute arrays */
+    uint32_t reserved;
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
@@ -88,6 +93,8 @@ PYR_API void pyrhip_built_free(PyrhipBuilt* built, void* stream);
 typedef struct PyrhipBuildTimes {
     float encode_ms, sort_ms, unique_ms, leaves_ms, levels_ms, total_ms;
     uint32_t sort_passes, radix_bits;
+    uint32_t record_bytes; /* bytes per record moved by a sort pass: 8 packed, 12 pairs */
+    uint32_t sort_items;
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 8): the

@@ -78,12 +78,13 @@ class PyrhipChunkBatch(C.Structure):
 
 
 class PyrhipBuildTuning(C.Structure):
-    _fields_ = [("radix_bits", u32), ("reserved", u32 * 3)]
+    _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("reserved", u32)]
 
 
 class PyrhipBuildTimes(C.Structure):
     _fields_ = [("encode_ms", f32), ("sort_ms", f32), ("unique_ms", f32), ("leaves_ms", f32), ("levels_ms", f32),
-                ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32)]
+                ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32), ("record_bytes", u32),
+                ("sort_items", u32)]
 
 
 class PyrhipDag(C.Structure):

@@ -206,11 +206,11 @@ PyrResult install(PyrContext* ctx, Geometry& g, const PyrhipBuilt& b, bool edita
         g.keys = (uint64_t*)(uintptr_t)b.keys;
         g.vals = (uint32_t*)(uintptr_t)b.attrs;
     } else {
-        (void)hipFree((void*)(uintptr_t)b.keys);
-        (void)hipFree((void*)(uintptr_t)b.attrs);
+        pool_free((void*)(uintptr_t)b.keys, s);
+        pool_free((void*)(uintptr_t)b.attrs, s);
     }
-    (void)hipFree((void*)(uintptr_t)b.nodes);
-    (void)hipFree((void*)(uintptr_t)b.leaves);
+    pool_free((void*)(uintptr_t)b.nodes, s);
+    pool_free((void*)(uintptr_t)b.leaves, s);
     return PYR_OK;
 }
 
@@ -302,9 +302,10 @@ PYR_API void pyr_destroy(PyrContext* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     for (Geometry& g : ctx->geos) {
         if (!g.live) continue;
-        (void)hipFree(g.keys);
-        (void)hipFree(g.vals);
+        pool_free(g.keys, ctx->stream);
+        pool_free(g.vals, ctx->stream);
     }
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->nodes.base);
     (void)hipFree(ctx->leaves.base);
     (void)hipFree(ctx->attrs.base);
@@ -391,8 +392,8 @@ PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const
         pyrhip_built_free(&b, ctx->stream);
         return r;
     }
-    (void)hipFree(old.keys);
-    (void)hipFree(old.vals);
+    pool_free(old.keys, ctx->stream);
+    pool_free(old.vals, ctx->stream);
     return PYR_OK;
 }
 
@@ -404,8 +405,8 @@ PYR_API PyrResult pyr_geometry_destroy(PyrContext* ctx, PyrGeometry geometry) {
     if (g->users) return fail(PYR_ERROR_IN_USE, "geometry still has %u instances", g->users);
     (void)hipStreamSynchronize(ctx->stream);
     release_geometry_ranges(ctx, *g);
-    (void)hipFree(g->keys);
-    (void)hipFree(g->vals);
+    pool_free(g->keys, ctx->stream);
+    pool_free(g->vals, ctx->stream);
     g->keys = nullptr;
     g->vals = nullptr;
     g->live = false;

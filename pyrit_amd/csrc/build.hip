@@ -86,9 +86,13 @@ struct EncodeArgs {
     uint32_t* vals;
     uint32_t* hist;           // [passes][radix]
     uint32_t passes, radix_bits;
+    uint32_t packed;          // keys[i] = key << 32 | attribute, vals unused
 };
 
-__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t i, uint64_t& key, uint32_t& val) {
+constexpr uint32_t kLdsOffsets = 8192;  // chunk prefixes staged in LDS up to this many
+
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, const uint32_t* offs, uint32_t i, uint64_t& key,
+                                           uint32_t& val) {
     const uint32_t lbits = local_bits(a.log2_size);
     const uint64_t invalid = 1ull << (lbits + a.chunk_bits);
     if (i < a.n_old) {
@@ -113,9 +117,9 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t i, uint
     uint32_t lo = 0, hi = a.chunk_count;
     while (hi - lo > 1) {
         uint32_t mid = (lo + hi) / 2;
-        if (a.offsets[mid] <= j) lo = mid; else hi = mid;
+        if (offs[mid] <= j) lo = mid; else hi = mid;
     }
-    const uint32_t c = lo, local = j - a.offsets[c];
+    const uint32_t c = lo, local = j - offs[c];
     if (a.capacity && local >= a.capacity) return;
     int4 v = a.voxels[a.capacity ? (uint64_t)c * a.capacity + local : (uint64_t)j];
     const int32_t n = 1 << (a.chunk_log2 ? a.chunk_log2[c] : a.log2_size);
@@ -128,17 +132,28 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t i, uint
 // keys/vals of every record plus, per block, LDS histograms of all radix
 // digits flushed to the global histogram with one atomic per non-zero bin
 __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, uint32_t n_total) {
-    extern __shared__ uint32_t s_hist[];
+    extern __shared__ uint32_t s_mem[];
     const uint32_t radix = 1u << a.radix_bits, bins = a.passes * radix;
+    uint32_t* s_hist = s_mem;
+    // chunk prefixes staged in LDS for the per-record binary search
+    const bool lds_offs = a.chunk_count && a.chunk_count + 1 <= kLdsOffsets;
+    uint32_t* s_offs = s_mem + bins;
     for (uint32_t i = threadIdx.x; i < bins; i += kBlock) s_hist[i] = 0;
+    if (lds_offs)
+        for (uint32_t i = threadIdx.x; i <= a.chunk_count; i += kBlock) s_offs[i] = a.offsets[i];
     __syncthreads();
+    const uint32_t* offs = lds_offs ? s_offs : a.offsets;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_total; i += stride) {
         uint64_t key;
         uint32_t val;
-        encode_one(a, i, key, val);
-        a.keys[i] = key;
-        a.vals[i] = val;
+        encode_one(a, offs, i, key, val);
+        if (a.packed) {
+            a.keys[i] = (key << 32) | val;
+        } else {
+            a.keys[i] = key;
+            a.vals[i] = val;
+        }
         for (uint32_t p = 0; p < a.passes; p++)
             atomicAdd(&s_hist[p * radix + (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1))], 1u);
     }
@@ -194,8 +209,13 @@ struct SortArgs {
     uint32_t* err;
 };
 
-template <int RB>
+// PACKED: one u64 word per record, key << 32 | attribute (keys of at most 32
+// bits); the digit is taken at a.shift >= 32 and there is no value stream.
+template <int RB, int ITEMS, bool PACKED>
 __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
+    // tile of kBlock * ITEMS keys (the names shadow the defaults above)
+    constexpr int kSortItems = ITEMS;
+    constexpr int kSortTile = kBlock * ITEMS;
     constexpr uint32_t R = 1u << RB;
     constexpr uint32_t DPT = R / kBlock;  // digits per thread
     constexpr int W = kBlock / kWave;
@@ -205,7 +225,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     __shared__ uint32_t s_part[kBlock];
     __shared__ uint64_t s_dst[R];          // global position of local slot 0 per digit
     __shared__ uint64_t s_keys[kSortTile];
-    __shared__ uint32_t s_vals[kSortTile];
+    __shared__ uint32_t s_vals[PACKED ? 1 : kSortTile];
 
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
@@ -222,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         uint64_t idx = wbase + (uint64_t)i * kWave + lane;
         bool ok = idx < a.n;
         k[i] = ok ? a.keys_in[idx] : 0;
-        v[i] = ok ? a.vals_in[idx] : 0;
+        if constexpr (!PACKED) v[i] = ok ? a.vals_in[idx] : 0;
     }
     // stable ranking inside the wave: lanes with equal digits found by RB
     // ballots; the lowest peer bumps the wave's counter for that digit
@@ -346,7 +366,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
             uint32_t pos = s_local[d] + s_wave_cnt[wave][d] + rank[i];
             s_keys[pos] = k[i];
-            s_vals[pos] = v[i];
+            if constexpr (!PACKED) s_vals[pos] = v[i];
         }
     }
     __syncthreads();
@@ -361,7 +381,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             continue;
         }
         a.keys_out[dst] = key;
-        a.vals_out[dst] = s_vals[j];
+        if constexpr (!PACKED) a.vals_out[dst] = s_vals[j];
     }
 }
 
@@ -402,7 +422,8 @@ __device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
 // unique (gbuild.zig:366-388): keep the last record of each key unless it is
 // invalid or deletes (attribute 0)
 __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, const uint64_t* keys, const uint32_t* vals,
-                                                    uint32_t n, uint64_t invalid, uint64_t* ukey, uint32_t* uval) {
+                                                    uint32_t n, uint64_t invalid, uint64_t* ukey, uint32_t* uval,
+                                                    bool packed) {
     __shared__ uint32_t s_tile, s_wave[4];
     __shared__ uint64_t s_b;
     const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
@@ -417,10 +438,18 @@ __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, const uint64_t* k
             uint64_t idx = scan_index(tile, i);
             f[i] = 0;
             if (idx < n) {
-                k[i] = keys[idx];
-                v[i] = vals[idx];
-                bool last = idx + 1 == n || keys[idx + 1] != k[i];
-                f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+                if (packed) {  // key << 32 | attribute
+                    const uint64_t w = keys[idx];
+                    k[i] = w >> 32;
+                    v[i] = (uint32_t)w;
+                    bool last = idx + 1 == n || (keys[idx + 1] >> 32) != k[i];
+                    f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+                } else {
+                    k[i] = keys[idx];
+                    v[i] = vals[idx];
+                    bool last = idx + 1 == n || keys[idx + 1] != k[i];
+                    f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+                }
             }
         }
         uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
@@ -841,6 +870,32 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
+bool pool_ready(int device) {
+    static bool done[64] = {};
+    if (device < 0 || device >= 64) return false;
+    if (done[device]) return true;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess) {
+        hipGetLastError();
+        return false;
+    }
+    uint64_t keep = ~0ull;
+    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+    done[device] = true;
+    return true;
+}
+
+hipError_t pool_alloc(void** p, uint64_t bytes, hipStream_t s) {
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    pool_ready(dev);
+    return hipMallocAsync(p, bytes, s);
+}
+
+void pool_free(void* p, hipStream_t s) {
+    if (p) (void)hipFreeAsync(p, s);
+}
+
 namespace {
 
 inline uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
@@ -863,7 +918,7 @@ struct Alloc {
     template <class T>
     T* get(uint64_t count) {
         void* p = nullptr;
-        if (hipMallocAsync(&p, std::max<uint64_t>(count * sizeof(T), 256), s) != hipSuccess) {
+        if (pool_alloc(&p, std::max<uint64_t>(count * sizeof(T), 256), s) != hipSuccess) {
             failed = true;
             return nullptr;
         }
@@ -871,7 +926,7 @@ struct Alloc {
         return (T*)p;
     }
     void release() {
-        for (void* p : ptrs) hipFreeAsync(p, s);
+        for (void* p : ptrs) pool_free(p, s);
         ptrs.clear();
     }
 };
@@ -899,9 +954,29 @@ struct Timer {
     }
 };
 
-template <int RB>
-void launch_sort(hipStream_t s, uint32_t tiles, const SortArgs& a) {
-    hipLaunchKernelGGL(k_sort_pass<RB>, dim3(tiles), dim3(kBlock), 0, s, a);
+template <int RB, bool PACKED>
+void launch_sort_rb(hipStream_t s, uint32_t items, uint32_t tiles, const SortArgs& a) {
+    switch (items) {
+        case 8: hipLaunchKernelGGL((k_sort_pass<RB, 8, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 12: hipLaunchKernelGGL((k_sort_pass<RB, 12, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 20: hipLaunchKernelGGL((k_sort_pass<RB, 20, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 24: hipLaunchKernelGGL((k_sort_pass<RB, 24, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_sort_pass<RB, 16, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+    }
+}
+
+// packed records are sorted with 8-bit digits only (the fastest width)
+void launch_sort(hipStream_t s, uint32_t rb, uint32_t items, bool packed, uint32_t tiles, const SortArgs& a) {
+    if (packed) {
+        launch_sort_rb<8, true>(s, items, tiles, a);
+        return;
+    }
+    switch (rb) {
+        case 8: launch_sort_rb<8, false>(s, items, tiles, a); break;
+        case 9: launch_sort_rb<9, false>(s, items, tiles, a); break;
+        case 10: launch_sort_rb<10, false>(s, items, tiles, a); break;
+        default: launch_sort_rb<11, false>(s, items, tiles, a); break;
+    }
 }
 
 }  // namespace
@@ -917,10 +992,16 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint32_t n = (uint32_t)n64, nmax = std::max<uint32_t>(n, 1);
     const uint32_t rb = tuning && tuning->radix_bits ? tuning->radix_bits : 8;
     if (rb < 8 || rb > 11) return fail(PYR_ERROR_INVALID_ARGUMENT, "radix_bits must be in [8, 11]");
+    const uint32_t sort_items = tuning && tuning->sort_items ? tuning->sort_items : 20;
+    if (sort_items != 8 && sort_items != 12 && sort_items != 16 && sort_items != 20 && sort_items != 24)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "sort_items must be 8, 12, 16, 20 or 24");
     const uint32_t lbits = local_bits(log2_size);
     const uint32_t key_bits = lbits + enc.chunk_bits + 1;
     if (key_bits > 64) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
     const uint32_t passes = (key_bits + rb - 1) / rb;
+    // keys of at most 32 bits travel packed with their attribute in one u64
+    // (16 instead of 24 bytes per record and pass)
+    const bool packed = key_bits <= 32 && rb == 8 && !(tuning && tuning->pair_layout);
     const uint32_t radix = 1u << rb;
     const uint64_t k = enc.chunk_count ? enc.chunk_count : 1;
 
@@ -939,7 +1020,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     uint32_t* counters = A.get<uint32_t>(C_COUNT);
     uint32_t* hist = A.get<uint32_t>((uint64_t)passes * radix * 2);
     uint32_t* digit_base = hist + (uint64_t)passes * radix;
-    const uint64_t sort_tiles = (nmax + kSortTile - 1) / kSortTile;
+    const uint64_t sort_tile = (uint64_t)kBlock * sort_items;
+    const uint64_t sort_tiles = (nmax + sort_tile - 1) / sort_tile;
     const uint64_t scan_tiles = (nmax + kScanTile - 1) / kScanTile;
     const uint64_t status_words = std::max<uint64_t>(sort_tiles * radix, scan_tiles);
     uint64_t* status = A.get<uint64_t>(status_words);
@@ -968,18 +1050,19 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     uint32_t* nodes_out = nullptr;
     bool out_ok = true;
     for (int q = 0; q < 2; q++) {
-        out_ok &= hipMalloc(&kb[q], (uint64_t)nmax * 8) == hipSuccess;
-        out_ok &= hipMalloc(&vb[q], (uint64_t)nmax * 4) == hipSuccess;
+        out_ok &= pool_alloc((void**)&kb[q], (uint64_t)nmax * 8, s) == hipSuccess;
+        // packed records need one value array (for the unique attributes)
+        if (q == 0 || !packed) out_ok &= pool_alloc((void**)&vb[q], (uint64_t)nmax * 4, s) == hipSuccess;
     }
-    out_ok &= hipMalloc(&leaves_out, nb * 8) == hipSuccess;
-    out_ok &= hipMalloc(&nodes_out, word_bound * 4) == hipSuccess;
+    out_ok &= pool_alloc((void**)&leaves_out, nb * 8, s) == hipSuccess;
+    out_ok &= pool_alloc((void**)&nodes_out, word_bound * 4, s) == hipSuccess;
     auto free_outputs = [&]() {
         for (int q = 0; q < 2; q++) {
-            hipFree(kb[q]);
-            hipFree(vb[q]);
+            pool_free(kb[q], s);
+            pool_free(vb[q], s);
         }
-        hipFree(leaves_out);
-        hipFree(nodes_out);
+        pool_free(leaves_out, s);
+        pool_free(nodes_out, s);
     };
     if (A.failed || !out_ok) {
         A.release();
@@ -1007,35 +1090,33 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     enc.hist = hist;
     enc.passes = passes;
     enc.radix_bits = rb;
+    enc.packed = packed;
     if (n) {
+        const uint32_t offs_lds =
+            enc.chunk_count && enc.chunk_count + 1 <= kLdsOffsets ? (enc.chunk_count + 1) * 4 : 0;
         hipLaunchKernelGGL(k_encode, dim3(grid_for(n, kBlock * 16, 2048)), dim3(kBlock),
-                           passes * radix * sizeof(uint32_t), s, enc, n);
+                           passes * radix * sizeof(uint32_t) + offs_lds, s, enc, n);
         hipLaunchKernelGGL(k_digit_scan, dim3(1), dim3(kBlock), 0, s, hist, digit_base, passes, radix);
     }
     tm.mark();
     // 2. radix passes
     uint32_t cur = 0;
     for (uint32_t p = 0; p < passes && n; p++) {
-        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, p * rb, digit_base + p * radix, status,
-                    ++epoch, tile_ctr(), counters + C_ERR};
-        switch (rb) {
-            case 8: launch_sort<8>(s, (uint32_t)sort_tiles, sa); break;
-            case 9: launch_sort<9>(s, (uint32_t)sort_tiles, sa); break;
-            case 10: launch_sort<10>(s, (uint32_t)sort_tiles, sa); break;
-            default: launch_sort<11>(s, (uint32_t)sort_tiles, sa); break;
-        }
+        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, (packed ? 32 : 0) + p * rb,
+                    digit_base + p * radix, status, ++epoch, tile_ctr(), counters + C_ERR};
+        launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
         cur ^= 1;
     }
     tm.mark();
     // 3. unique into the other buffer pair, which becomes Built.keys/attrs
     uint64_t* ukey = kb[cur ^ 1];
-    uint32_t* uval = vb[cur ^ 1];
+    uint32_t* uval = packed ? vb[0] : vb[cur ^ 1];
     const uint64_t invalid = 1ull << (lbits + enc.chunk_bits);
     const uint32_t scan_grid = grid_for(nmax, kScanTile, 2048);
     if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_unique, dim3(scan_grid), dim3(kBlock), 0, s, sc, kb[cur], vb[cur], n, invalid, ukey,
-                           uval);
+                           uval, packed);
         ScanCtx sb{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, ukey, lbits, brick_key, masks,
                            first_dev, nb, nmax);
@@ -1110,6 +1191,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         g_times.total_ms = tm.between(0, 5);
         g_times.sort_passes = passes;
         g_times.radix_bits = rb;
+        g_times.record_bytes = packed ? 8 : 12;
+        g_times.sort_items = sort_items;
     }
     if (e != hipSuccess || c[C_ERR] != 0) {
         free_outputs();
@@ -1126,8 +1209,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     out->attrs = (uint64_t)(uintptr_t)uval;
     out->voxel_count = c[C_U];
     // the buffer pair not handed out held the sorted records
-    hipFree(kb[cur]);
-    hipFree(vb[cur]);
+    pool_free(kb[cur], s);
+    if (!packed) pool_free(vb[cur], s);
     return PYR_OK;
 }
 
@@ -1180,11 +1263,11 @@ PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch
 
 PYR_API void pyrhip_built_free(PyrhipBuilt* b, void* stream) {
     if (!b) return;
-    (void)stream;
-    hipFree((void*)(uintptr_t)b->nodes);
-    hipFree((void*)(uintptr_t)b->leaves);
-    hipFree((void*)(uintptr_t)b->keys);
-    hipFree((void*)(uintptr_t)b->attrs);
+    hipStream_t s = (hipStream_t)stream;
+    pool_free((void*)(uintptr_t)b->nodes, s);
+    pool_free((void*)(uintptr_t)b->leaves, s);
+    pool_free((void*)(uintptr_t)b->keys, s);
+    pool_free((void*)(uintptr_t)b->attrs, s);
     memset(b, 0, sizeof(*b));
 }
 

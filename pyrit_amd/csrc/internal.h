@@ -1,5 +1,6 @@
 // internal.h - host-side helpers shared by the libpyrit translation units.
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdarg.h>
 #include <stdint.h>
 #include <stdio.h>
@@ -22,5 +23,13 @@ inline PyrResult fail(PyrResult code, const char* fmt, ...) {
     va_end(ap);
     return code;
 }
+
+// Device memory of builds and geometries comes from the device's stream-ordered
+// pool, configured once to keep freed memory (release threshold = max): a build
+// allocates tens of GB of outputs and scratch, and recycling them from the
+// pool replaces page mapping and unmapping on every call.
+bool pool_ready(int device);
+hipError_t pool_alloc(void** p, uint64_t bytes, hipStream_t s);
+void pool_free(void* p, hipStream_t s);
 
 }  // namespace pyr

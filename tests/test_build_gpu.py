@@ -62,6 +62,21 @@ def test_radix_widths_identical(hip_device, radix_bits):
     assert_same_build(b, ref, 9)
 
 
+@pytest.mark.parametrize("sort_items", [8, 12, 16, 20, 24])
+@pytest.mark.parametrize("pair_layout", [0, 1])
+def test_tile_sizes_and_layouts_identical(hip_device, sort_items, pair_layout):
+    # log2 9: 27-bit keys, so layout 0 sorts packed key << 32 | attribute words
+    print(f"case: sort_items {sort_items} pair_layout {pair_layout}", flush=True)
+    pts = random_points(90000, 9, 78, dup_frac=0.3, delete_frac=0.1, out_frac=0.01)
+    ref = O.gb_build(9, pts)
+    b = _gpu().build(9, pts, sort_items=sort_items, pair_layout=pair_layout)
+    assert_same_build(b, ref, 9)
+    # edits through the same tuning
+    ed = random_points(7000, 9, 79, dup_frac=0.2, delete_frac=0.3)
+    b2 = _gpu().build(9, ed, old=b, sort_items=sort_items, pair_layout=pair_layout)
+    assert_same_build(b2, O.gb_build(9, ed, ref["keys"], ref["attrs"]), 9)
+
+
 def test_scene_dense_and_empty_and_single(hip_device):
     print("case: scene 64^3 / empty / single voxel", flush=True)
     g = _gpu()
