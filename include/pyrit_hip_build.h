@@ -88,6 +88,14 @@ PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch
 
 PYR_API void pyrhip_built_free(PyrhipBuilt* built, void* stream);
 
+/* Sorted voxel keys of a build (Built.keys/attrs, or a chunk's slice of a
+ * batch) back to PyrVoxel records in chunk-local coordinates; log2_size is the
+ * edge the keys were built with (the batch's for a batch slice). The input of
+ * an edit rebuild of a chunk of a batch (the reference keeps old keys for the
+ * same purpose, src/gpu_build.zig:90-96). */
+PYR_API PyrResult pyrhip_decode_voxels(uint64_t keys, uint64_t attrs, uint32_t count, uint32_t log2_size,
+                                       uint64_t out, void* stream);
+
 /* Last build's kernel timings (device ms, from HIP events on the build
  * stream) when timing was enabled with pyrhip_set_timing(1). */
 typedef struct PyrhipBuildTimes {
@@ -135,6 +143,79 @@ PYR_API PyrResult pyrhip_camera_rays(const PyrCamera* camera, uint64_t rays, voi
  * 0xFFFFFFFF for a primary miss. */
 PYR_API PyrResult pyrhip_trace_shadow(const PyrhipDag* dag, uint64_t rays, uint64_t hits, const float sun[3],
                                       uint64_t out, uint32_t count, void* stream);
+
+/* ---------------------------------------------------------------------------
+ * Serving: a continuous-batching build server with resident chunk DAGs.
+ *
+ * The reference streams world chunks through a job pipeline on a worker
+ * thread that generates, applies edits and builds one batch at a time
+ * (src/world.zig:755-1124: submitJob / workerMain / gpuWorkImpl / finishJob,
+ * with capacity growth). Here requests arrive one by one: worker threads (one
+ * HIP stream each) repeatedly take every pending request that fits the device
+ * memory budget, build them as one mixed-size chunk batch, keep each chunk's
+ * DAG and sorted voxels resident, and complete the requests. Requests that
+ * arrive while batches are on the device join the next batch; with two or
+ * more workers batches overlap on the device. Requests on the same chunk are
+ * applied in submission order. Results are identical to building each
+ * request alone (the batch output is canonicalised per chunk by construction:
+ * same voxels, same canonical DAG).
+ * ------------------------------------------------------------------------- */
+
+typedef struct PyrhipServer PyrhipServer;
+
+typedef struct PyrhipServerInfo {
+    uint64_t memory_budget;     /* device bytes for resident DAGs plus batches in flight; 0 = 64 GiB */
+    uint32_t max_batch_chunks;  /* 0 = 1024 */
+    uint32_t max_batch_voxels;  /* records per batch, 0 = 2^28 */
+    uint32_t workers;           /* batches in flight (one stream each), 0 = 2 */
+    uint32_t flags;
+    PyrhipBuildTuning tuning;
+} PyrhipServerInfo;
+
+#define PYRHIP_REQUEST_BUILD 0u /* (re)build a chunk from voxel records (device memory) */
+#define PYRHIP_REQUEST_EDIT 1u  /* set / remove voxels of a resident chunk (host records, copied) */
+
+typedef struct PyrhipRequest {
+    uint64_t chunk;      /* chunk id */
+    uint32_t kind;       /* PYRHIP_REQUEST_* */
+    uint32_t log2_size;  /* chunk edge (build; edit of a chunk that is not resident) */
+    uint64_t voxels;     /* build: device PyrVoxel*; edit: host PyrVoxel* */
+    uint32_t count;
+    uint32_t reserved;
+    uint64_t user;
+} PyrhipRequest;
+
+typedef struct PyrhipCompletion {
+    uint64_t ticket, chunk, user;
+    int32_t result;          /* PYR_OK or the error of the batch / admission */
+    uint32_t voxel_count;    /* voxels of the chunk after the request */
+    double submit_s, start_s, done_s; /* steady clock: submitted, batch launched, DAG resident */
+    uint32_t batch_chunks;   /* size of the batch that carried the request */
+    uint32_t reserved;
+} PyrhipCompletion;
+
+typedef struct PyrhipServerStats {
+    uint64_t submitted, completed, rejected, batches;
+    uint64_t resident_chunks, resident_bytes, reserved_bytes, pending;
+    uint64_t batched_voxels;
+    double busy_s; /* summed device time of the batches (HIP events) */
+} PyrhipServerStats;
+
+PYR_API PyrResult pyrhip_server_create(const PyrhipServerInfo* info, PyrhipServer** out);
+PYR_API PyrResult pyrhip_server_submit(PyrhipServer* server, const PyrhipRequest* request, uint64_t* ticket);
+/* completed requests since the last poll (at most max); non-blocking */
+PYR_API uint32_t pyrhip_server_poll(PyrhipServer* server, PyrhipCompletion* out, uint32_t max);
+/* block until every request submitted so far has completed */
+PYR_API PyrResult pyrhip_server_drain(PyrhipServer* server);
+/* resident DAG of a chunk, valid until the chunk is rebuilt or the server is
+ * destroyed: the node and leaf arrays of the batch that built it (shared with
+ * the batch's other chunks; node_words / leaf_count give their lengths), the
+ * chunk's root and its attributes. An empty chunk has nodes == 0. */
+PYR_API PyrResult pyrhip_server_chunk(PyrhipServer* server, uint64_t chunk, PyrhipDag* out, uint32_t* voxel_count,
+                                      uint32_t* node_words, uint32_t* leaf_count);
+PYR_API double pyrhip_server_clock(void); /* the steady clock of the completion time stamps */
+PYR_API void pyrhip_server_stats(PyrhipServer* server, PyrhipServerStats* out);
+PYR_API void pyrhip_server_destroy(PyrhipServer* server);
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,31 @@ class PyrhipDag(C.Structure):
                 ("default_attribute", u32), ("reserved", u32)]
 
 
+class PyrhipServerInfo(C.Structure):
+    _fields_ = [("memory_budget", u64), ("max_batch_chunks", u32), ("max_batch_voxels", u32), ("workers", u32),
+                ("flags", u32), ("tuning", PyrhipBuildTuning)]
+
+
+class PyrhipRequest(C.Structure):
+    _fields_ = [("chunk", u64), ("kind", u32), ("log2_size", u32), ("voxels", u64), ("count", u32),
+                ("reserved", u32), ("user", u64)]
+
+
+class PyrhipCompletion(C.Structure):
+    _fields_ = [("ticket", u64), ("chunk", u64), ("user", u64), ("result", i32), ("voxel_count", u32),
+                ("submit_s", f64), ("start_s", f64), ("done_s", f64), ("batch_chunks", u32), ("reserved", u32)]
+
+
+class PyrhipServerStats(C.Structure):
+    _fields_ = [("submitted", u64), ("completed", u64), ("rejected", u64), ("batches", u64),
+                ("resident_chunks", u64), ("resident_bytes", u64), ("reserved_bytes", u64), ("pending", u64),
+                ("batched_voxels", u64), ("busy_s", f64)]
+
+
+PYRHIP_REQUEST_BUILD = 0
+PYRHIP_REQUEST_EDIT = 1
+
+
 class PyrTerrainCave(C.Structure):
     _fields_ = [("x", i32), ("y", i32), ("z", i32), ("r", i32)]
 
@@ -135,6 +160,7 @@ FUNCTIONS = {
     "pyrhip_build_chunks": (i32, [u32, P(PyrhipChunkBatch), P(PyrhipBuildTuning), vp, P(PyrhipBuilt), P(u32),
                                   P(u32)]),
     "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),
+    "pyrhip_decode_voxels": (i32, [u64, u64, u32, u32, u64, vp]),
     "pyrhip_set_timing": (None, [i32]),
     "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),
     "pyrhip_last_build_times": (None, [P(PyrhipBuildTimes)]),
@@ -142,6 +168,14 @@ FUNCTIONS = {
     "pyrhip_trace_camera": (i32, [P(PyrhipDag), P(PyrCamera), u64, u32, vp]),
     "pyrhip_camera_rays": (i32, [P(PyrCamera), u64, vp]),
     "pyrhip_trace_shadow": (i32, [P(PyrhipDag), u64, u64, P(f32), u64, u32, vp]),
+    "pyrhip_server_create": (i32, [P(PyrhipServerInfo), P(vp)]),
+    "pyrhip_server_submit": (i32, [vp, P(PyrhipRequest), P(u64)]),
+    "pyrhip_server_poll": (u32, [vp, P(PyrhipCompletion), u32]),
+    "pyrhip_server_drain": (i32, [vp]),
+    "pyrhip_server_chunk": (i32, [vp, u64, P(PyrhipDag), P(u32), P(u32), P(u32)]),
+    "pyrhip_server_clock": (f64, []),
+    "pyrhip_server_stats": (None, [vp, P(PyrhipServerStats)]),
+    "pyrhip_server_destroy": (None, [vp]),
 }
 
 TERRAIN_FUNCTIONS = {

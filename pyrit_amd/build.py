@@ -33,7 +33,7 @@ COMMON = [
 ]
 
 LIBS = {
-    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp"],
+    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp"],
     "libpyrit_terrain.so": ["terrain.hip"],
 }
 

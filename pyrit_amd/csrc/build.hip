@@ -29,6 +29,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "common.h"
@@ -849,6 +851,34 @@ __global__ void __launch_bounds__(kBlock) k_finalize(uint32_t* counters, uint32_
     }
 }
 
+__device__ __forceinline__ uint32_t compact3(uint64_t v) {
+    uint64_t x = v & 0x1249249249249249ull;
+    x = (x | (x >> 2)) & 0x10C30C30C30C30C3ull;
+    x = (x | (x >> 4)) & 0x100F00F00F00F00Full;
+    x = (x | (x >> 8)) & 0x1F0000FF0000FFull;
+    x = (x | (x >> 16)) & 0x1F00000000FFFFull;
+    x = (x | (x >> 32)) & 0x1FFFFFull;
+    return (uint32_t)x;
+}
+
+// sorted keys (brick Morton << 6 | bit, chunk bits above lbits dropped) back to
+// PyrVoxel records in chunk-local coordinates
+__global__ void __launch_bounds__(kBlock) k_decode(const uint64_t* keys, const uint32_t* attrs, uint32_t count,
+                                                    uint32_t lbits, int4* out) {
+    const uint32_t stride = gridDim.x * kBlock;
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
+        const uint64_t key = keys[i] & ((1ull << lbits) - 1);
+        const uint64_t brick = key >> 6;
+        const uint32_t bit = (uint32_t)(key & 63);
+        int4 v;
+        v.x = (int)((compact3(brick) << 2) | (bit & 3));
+        v.y = (int)((compact3(brick >> 1) << 2) | ((bit >> 2) & 3));
+        v.z = (int)((compact3(brick >> 2) << 2) | (bit >> 4));
+        v.w = (int)attrs[i];
+        out[i] = v;
+    }
+}
+
 // plain streaming copy of n u64 words: the calibration kernel for the HBM
 // byte counters (bytes read = bytes written = 8n)
 __global__ void __launch_bounds__(kBlock) k_copy_probe(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
@@ -871,9 +901,9 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
 // host orchestration
 // ---------------------------------------------------------------------------
 bool pool_ready(int device) {
-    static bool done[64] = {};
+    static std::atomic<bool> done[64] = {};  // several server workers allocate at once
     if (device < 0 || device >= 64) return false;
-    if (done[device]) return true;
+    if (done[device].load(std::memory_order_acquire)) return true;
     hipMemPool_t pool;
     if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess) {
         hipGetLastError();
@@ -881,7 +911,7 @@ bool pool_ready(int device) {
     }
     uint64_t keep = ~0ull;
     (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    done[device] = true;
+    done[device].store(true, std::memory_order_release);
     return true;
 }
 
@@ -931,7 +961,10 @@ struct Alloc {
     }
 };
 
-bool g_timing = false;
+// timing switch and the last build's phase times (builds may run on several
+// threads at once, e.g. the build server's workers)
+std::atomic<bool> g_timing{false};
+std::mutex g_times_mutex;
 PyrhipBuildTimes g_times{};
 
 struct Timer {
@@ -1183,6 +1216,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e == hipSuccess) e = hipGetLastError();
     if (g_timing && e == hipSuccess) {
+        std::lock_guard<std::mutex> lk(g_times_mutex);
         g_times.encode_ms = tm.between(0, 1);
         g_times.sort_ms = tm.between(1, 2);
         g_times.unique_ms = tm.between(2, 3);
@@ -1271,6 +1305,19 @@ PYR_API void pyrhip_built_free(PyrhipBuilt* b, void* stream) {
     memset(b, 0, sizeof(*b));
 }
 
+PYR_API PyrResult pyrhip_decode_voxels(uint64_t keys, uint64_t attrs, uint32_t count, uint32_t log2_size,
+                                       uint64_t out, void* stream) {
+    diag_clear();
+    if (count && (!keys || !attrs || !out)) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if (count)
+        hipLaunchKernelGGL(k_decode, dim3(grid_for(count, kBlock * 4, 4096)), dim3(kBlock), 0, (hipStream_t)stream,
+                           (const uint64_t*)(uintptr_t)keys, (const uint32_t*)(uintptr_t)attrs, count,
+                           local_bits(log2_size), (int4*)(uintptr_t)out);
+    hipError_t e = hipGetLastError();
+    return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "decode: %s", hipGetErrorString(e));
+}
+
 PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream) {
     diag_clear();
     if (!src || !dst || (bytes & 7)) return fail(PYR_ERROR_INVALID_ARGUMENT, "copy probe needs 8-byte multiples");
@@ -1285,7 +1332,9 @@ PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, 
 PYR_API void pyrhip_set_timing(int enable) { g_timing = enable != 0; }
 
 PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out) {
-    if (out) *out = g_times;
+    if (!out) return;
+    std::lock_guard<std::mutex> lk(g_times_mutex);
+    *out = g_times;
 }
 
 }  // extern "C"
