@@ -48,6 +48,7 @@ enum : uint32_t {
     C_LEAVES = 3,  // distinct leaves
     C_ROOT = 4,
     C_WORDS = 5,
+    C_TILES = 6,   // record tiles of a batch (segment-aligned)
     C_NCH = 8,     // + lv: elements of level lv (children of the level-lv parents)
     C_CUR = 40,    // + lv: node-word cursor when level lv's parents are written
     C_TILE = 72,   // tile counters, one per launch
@@ -68,6 +69,89 @@ __host__ __device__ inline uint64_t hash_cap_for(uint64_t n) {
 }
 
 // ---------------------------------------------------------------------------
+// record segments and tiles
+//
+// The record array is cut into segments: chunk c of a batch owns positions
+// [offsets[c], offsets[c+1]), a single build is one segment [0, n). Records
+// are sorted by their chunk-local key inside their segment only, so the chunk
+// index never enters the sort key (fewer digits, and local keys of up to 32
+// bits travel packed with their attribute). Tiles of `tile` records never
+// cross a segment boundary; a batch describes its tiles in device memory
+// ({segment, first position, length, first tile of the segment}), a single
+// build derives them from the tile index.
+// ---------------------------------------------------------------------------
+struct Segs {
+    uint32_t single;          // 1: one segment [0, n), no descriptors
+    uint32_t count;           // segments
+    uint32_t n;               // records
+    uint32_t tile;            // records per tile
+    const uint32_t* off;      // [count + 1] segment starts (batches)
+    const uint4* desc;        // [tiles] (batches)
+    const uint32_t* tiles;    // device tile count (batches)
+};
+
+__device__ __forceinline__ uint32_t tile_count(const Segs& g) {
+    return g.single ? (g.n + g.tile - 1) / g.tile : *g.tiles;
+}
+
+__device__ __forceinline__ uint4 tile_desc(const Segs& g, uint32_t t) {
+    if (g.single) {
+        const uint32_t start = t * g.tile;
+        return make_uint4(0u, start, min(g.tile, g.n - start), 0u);
+    }
+    return g.desc[t];
+}
+
+__device__ __forceinline__ uint32_t seg_end(const Segs& g, uint32_t seg) { return g.single ? g.n : g.off[seg + 1]; }
+
+// tile_first[s] = tiles of segments before s (one block; segments are
+// scanned 256 at a time), tile_first[count] = *total = the tile count.
+// Offsets that are not a partition of [0, n) raise error bit 32 and leave no
+// tiles, so no later kernel touches a record.
+__global__ void __launch_bounds__(kBlock) k_seg_tiles(const uint32_t* off, uint32_t count, uint32_t n,
+                                                       uint32_t tile, uint32_t* tile_first, uint32_t* total,
+                                                       uint32_t* err) {
+    __shared__ uint32_t s_wave[kBlock / kWave];
+    __shared__ uint32_t s_bad;
+    if (threadIdx.x == 0) s_bad = off[0] != 0 || off[count] != n;
+    __syncthreads();
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < count; b += kBlock) {
+        const uint32_t s = b + threadIdx.x;
+        uint32_t len = 0;
+        if (s < count) {
+            const uint32_t lo = off[s], hi = off[s + 1];
+            if (hi < lo) s_bad = 1;
+            len = hi >= lo ? hi - lo : 0;
+        }
+        uint32_t v[1] = {(uint32_t)(((uint64_t)len + tile - 1) / tile)}, e[1];
+        const uint32_t sum = tile_scan<1>(v, e, s_wave);
+        if (s < count) tile_first[s] = carry + e[0];
+        carry += sum;
+        __syncthreads();  // s_wave is reused by the next round
+    }
+    if (threadIdx.x == 0) {
+        tile_first[count] = carry;
+        *total = s_bad ? 0u : carry;
+        if (s_bad) atomicOr(err, 32u);
+    }
+}
+
+// tile descriptors, one wave per segment (nothing is written for offsets
+// that k_seg_tiles rejected)
+__global__ void __launch_bounds__(kBlock) k_seg_desc(const uint32_t* off, const uint32_t* tile_first, uint32_t count,
+                                                      uint32_t tile, const uint32_t* total, uint4* desc) {
+    if (*total == 0) return;
+    const uint32_t waves = gridDim.x * (kBlock / kWave);
+    for (uint32_t s = blockIdx.x * (kBlock / kWave) + threadIdx.x / kWave; s < count; s += waves) {
+        const uint32_t first = tile_first[s], nt = tile_first[s + 1] - first;
+        const uint32_t start = off[s], len = off[s + 1] - start;
+        for (uint32_t t = lane_id(); t < nt; t += kWave)
+            desc[first + t] = make_uint4(s, start + t * tile, min(tile, len - t * tile), first);
+    }
+}
+
+// ---------------------------------------------------------------------------
 // input description shared by the encode kernel
 // ---------------------------------------------------------------------------
 struct EncodeArgs {
@@ -80,117 +164,114 @@ struct EncodeArgs {
     // chunk batches
     uint32_t chunk_count;     // 0 = single build
     uint32_t capacity;        // 0 = packed segments
-    const uint32_t* offsets;  // [chunk_count + 1]
+    const uint32_t* offsets;  // [chunk_count + 1], offsets[0] = 0
     const uint8_t* chunk_log2;
     uint32_t chunk_bits;
     // outputs
     uint64_t* keys;
     uint32_t* vals;
-    uint32_t* hist;           // [passes][radix]
+    uint32_t* hist;           // [passes][segments][radix]
     uint32_t passes, radix_bits;
     uint32_t packed;          // keys[i] = key << 32 | attribute, vals unused
 };
 
-constexpr uint32_t kLdsOffsets = 8192;  // chunk prefixes staged in LDS up to this many
-
-__device__ __forceinline__ void encode_one(const EncodeArgs& a, const uint32_t* offs, uint32_t i, uint64_t& key,
-                                           uint32_t& val) {
+// Chunk-local sort key and attribute of record i (position i of the record
+// array, the `local`-th record of segment `seg`); out-of-range records get
+// the invalid bit just above the local key bits, so they sort to the end of
+// their segment. encode (gbuild.zig:184-199) and pack_chunks
+// (gbuild.zig:203-230) without the chunk index in the key.
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, uint32_t local, uint32_t i,
+                                           uint64_t& key, uint32_t& val) {
     const uint32_t lbits = local_bits(a.log2_size);
-    const uint64_t invalid = 1ull << (lbits + a.chunk_bits);
-    if (i < a.n_old) {
+    key = 1ull << lbits;
+    val = 0;
+    if (i < a.n_old) {  // earlier build's sorted unique voxels (single builds)
         key = a.old_keys[i];
         val = a.old_vals[i];
         return;
     }
-    uint32_t j = i - a.n_old;
-    key = invalid;
-    val = 0;
+    int4 v;
+    int32_t n;
     if (a.chunk_count == 0) {
-        // encode (gbuild.zig:184-199)
-        int4 v = a.voxels[j];
-        const int32_t n = 1 << a.log2_size;
-        if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
-            key = local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
-            val = (uint32_t)v.w;
-        }
-        return;
+        v = a.voxels[i - a.n_old];
+        n = 1 << a.log2_size;
+    } else {
+        if (a.capacity && local >= a.capacity) return;
+        v = a.voxels[a.capacity ? (uint64_t)seg * a.capacity + local : (uint64_t)i];
+        n = 1 << (a.chunk_log2 ? a.chunk_log2[seg] : a.log2_size);
     }
-    // pack_chunks (gbuild.zig:203-230): chunk by binary search in the prefixes
-    uint32_t lo = 0, hi = a.chunk_count;
-    while (hi - lo > 1) {
-        uint32_t mid = (lo + hi) / 2;
-        if (offs[mid] <= j) lo = mid; else hi = mid;
-    }
-    const uint32_t c = lo, local = j - offs[c];
-    if (a.capacity && local >= a.capacity) return;
-    int4 v = a.voxels[a.capacity ? (uint64_t)c * a.capacity + local : (uint64_t)j];
-    const int32_t n = 1 << (a.chunk_log2 ? a.chunk_log2[c] : a.log2_size);
     if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
-        key = ((uint64_t)c << lbits) | local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
+        key = local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
         val = (uint32_t)v.w;
     }
 }
 
-// keys/vals of every record plus, per block, LDS histograms of all radix
-// digits flushed to the global histogram with one atomic per non-zero bin
-__global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, uint32_t n_total) {
-    extern __shared__ uint32_t s_mem[];
+// Records of a contiguous run of tiles per block, plus the digit histograms
+// of every pass per segment: counted in LDS and flushed to the global
+// histogram (one atomic per non-zero bin) whenever the block's run moves to
+// the next segment.
+__global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_t tiles_per_block) {
+    extern __shared__ uint32_t s_hist[];
     const uint32_t radix = 1u << a.radix_bits, bins = a.passes * radix;
-    uint32_t* s_hist = s_mem;
-    // chunk prefixes staged in LDS for the per-record binary search
-    const bool lds_offs = a.chunk_count && a.chunk_count + 1 <= kLdsOffsets;
-    uint32_t* s_offs = s_mem + bins;
     for (uint32_t i = threadIdx.x; i < bins; i += kBlock) s_hist[i] = 0;
-    if (lds_offs)
-        for (uint32_t i = threadIdx.x; i <= a.chunk_count; i += kBlock) s_offs[i] = a.offsets[i];
-    __syncthreads();
-    const uint32_t* offs = lds_offs ? s_offs : a.offsets;
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n_total; i += stride) {
-        uint64_t key;
-        uint32_t val;
-        encode_one(a, offs, i, key, val);
-        if (a.packed) {
-            a.keys[i] = (key << 32) | val;
-        } else {
-            a.keys[i] = key;
-            a.vals[i] = val;
-        }
-        for (uint32_t p = 0; p < a.passes; p++)
-            atomicAdd(&s_hist[p * radix + (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1))], 1u);
-    }
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < bins; i += kBlock)
-        if (s_hist[i]) atomicAdd(&a.hist[i], s_hist[i]);
-}
-
-// exclusive scan of each pass's digit histogram (one block)
-__global__ void __launch_bounds__(kBlock) k_digit_scan(const uint32_t* hist, uint32_t* base, uint32_t passes,
-                                                       uint32_t radix) {
-    __shared__ uint32_t s_part[kBlock];
-    for (uint32_t p = 0; p < passes; p++) {
-        const uint32_t per = radix / kBlock;
-        const uint32_t* h = hist + p * radix;
-        uint32_t sum = 0;
-        for (uint32_t k = 0; k < per; k++) sum += h[threadIdx.x * per + k];
-        s_part[threadIdx.x] = sum;
+    const uint32_t tiles = tile_count(g);
+    const uint32_t t0 = blockIdx.x * tiles_per_block;
+    const uint32_t t1 = min(tiles, t0 + tiles_per_block);
+    uint32_t cur = 0xFFFFFFFFu;
+    auto flush = [&]() {
         __syncthreads();
-        if (threadIdx.x == 0) {
-            uint32_t acc = 0;
-            for (int t = 0; t < kBlock; t++) {
-                uint32_t v = s_part[t];
-                s_part[t] = acc;
-                acc += v;
+        for (uint32_t b = threadIdx.x; b < bins; b += kBlock) {
+            const uint32_t c = s_hist[b];
+            if (c) {
+                const uint32_t p = b / radix, d = b % radix;
+                atomicAdd(&a.hist[((uint64_t)p * g.count + cur) * radix + d], c);
+                s_hist[b] = 0;
             }
         }
         __syncthreads();
-        uint32_t acc = s_part[threadIdx.x];
-        for (uint32_t k = 0; k < per; k++) {
-            uint32_t d = threadIdx.x * per + k;
-            base[p * radix + d] = acc;
-            acc += h[d];
+    };
+    __syncthreads();
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint4 d = tile_desc(g, t);
+        if (d.x != cur) {
+            if (cur != 0xFFFFFFFFu) flush();
+            cur = d.x;
         }
-        __syncthreads();
+        const uint32_t local0 = (t - d.w) * g.tile;
+        for (uint32_t j = threadIdx.x; j < d.z; j += kBlock) {
+            const uint32_t i = d.y + j;
+            uint64_t key;
+            uint32_t val;
+            encode_one(a, d.x, local0 + j, i, key, val);
+            if (a.packed) {
+                a.keys[i] = (key << 32) | val;
+            } else {
+                a.keys[i] = key;
+                a.vals[i] = val;
+            }
+            for (uint32_t p = 0; p < a.passes; p++)
+                atomicAdd(&s_hist[p * radix + (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1))], 1u);
+        }
+    }
+    if (cur != 0xFFFFFFFFu) flush();
+}
+
+// Per (pass, segment): exclusive scan of the digit counts plus the segment's
+// start, the first output position of each digit in that pass.
+__global__ void __launch_bounds__(kBlock) k_digit_scan(const uint32_t* hist, uint32_t* base, Segs g,
+                                                       uint32_t passes, uint32_t radix) {
+    __shared__ uint32_t s_wave[kBlock / kWave];
+    const uint32_t seg = blockIdx.x % g.count;
+    const uint64_t row = (uint64_t)blockIdx.x * radix;  // blockIdx.x = pass * count + seg
+    const uint32_t per = radix / kBlock;
+    uint32_t v[1] = {0}, e[1];
+    for (uint32_t k = 0; k < per; k++) v[0] += hist[row + threadIdx.x * per + k];
+    tile_scan<1>(v, e, s_wave);
+    uint32_t acc = (g.single ? 0u : g.off[seg]) + e[0];
+    for (uint32_t k = 0; k < per; k++) {
+        const uint64_t d = row + threadIdx.x * per + k;
+        base[d] = acc;
+        acc += hist[d];
     }
 }
 
@@ -202,9 +283,9 @@ struct SortArgs {
     const uint32_t* vals_in;
     uint64_t* keys_out;
     uint32_t* vals_out;
-    uint32_t n;
+    Segs g;
     uint32_t shift;
-    const uint32_t* digit_base;  // [radix] global exclusive offsets of this pass
+    const uint32_t* digit_base;  // [segments][radix] first position of each digit in each segment, this pass
     uint64_t* status;            // [tiles][radix]
     uint32_t epoch;
     uint32_t* tile_ctr;
@@ -234,17 +315,21 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     for (uint32_t i = tid; i < W * R; i += kBlock) (&s_wave_cnt[0][0])[i] = 0;
     __syncthreads();
     const uint32_t tile = s_tile;
-    const uint64_t base = (uint64_t)tile * kSortTile;
+    if (tile >= tile_count(a.g)) return;  // grids are sized by an upper bound of the tile count
+    // tile of segment td.x: records [td.y, td.y + td.z); td.w = the segment's first tile
+    const uint4 td = tile_desc(a.g, tile);
+    const uint32_t count = td.z;
+    const uint32_t* seg_base = a.digit_base + (uint64_t)td.x * R;
 
     uint64_t k[kSortItems];
     uint32_t v[kSortItems], rank[kSortItems];
-    const uint64_t wbase = base + (uint64_t)wave * kWave * kSortItems;
+    const uint32_t wbase = wave * kWave * kSortItems;
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
-        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-        bool ok = idx < a.n;
-        k[i] = ok ? a.keys_in[idx] : 0;
-        if constexpr (!PACKED) v[i] = ok ? a.vals_in[idx] : 0;
+        const uint32_t li = wbase + i * kWave + lane;
+        bool ok = li < count;
+        k[i] = ok ? a.keys_in[td.y + li] : 0;
+        if constexpr (!PACKED) v[i] = ok ? a.vals_in[td.y + li] : 0;
     }
     // stable ranking inside the wave: lanes with equal digits found by RB
     // ballots; the lowest peer bumps the wave's counter for that digit
@@ -252,8 +337,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     const uint64_t lt = (1ull << lane) - 1;
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
-        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-        bool ok = idx < a.n;
+        bool ok = wbase + i * kWave + lane < count;
         uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
         uint64_t peers = __ballot(ok);
 #pragma unroll
@@ -307,14 +391,16 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             acc += tot[j];
         }
     }
-    // decoupled look-back per digit: publish the tile count, then the prefix
+    // decoupled look-back per digit over the tiles of the segment: publish
+    // the tile count, then the prefix
+    const int64_t first = td.w;
 #pragma unroll
     for (uint32_t j = 0; j < DPT; j++) {
         const uint32_t d = tid * DPT + j;
         uint64_t* st = a.status + d;
         uint64_t excl = 0;
-        if (tile == 0) {
-            __hip_atomic_store(&st[0], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELAXED,
+        if (tile == td.w) {  // first tile of its segment
+            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
         } else {
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELAXED,
@@ -322,14 +408,14 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             // predecessors are polled four at a time (independent loads in
             // flight) and summed newest to oldest until an inclusive prefix
             int64_t t = (int64_t)tile - 1;
-            uint32_t spins = 0;
-            while (t >= 0) {
+            Wait wt;
+            while (t >= first) {
                 uint64_t w[4];
 #pragma unroll
                 for (int q = 0; q < 4; q++)
-                    w[q] = t - q >= 0 ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
-                                                          __HIP_MEMORY_SCOPE_AGENT)
-                                      : status_word(kFlagPre, a.epoch, 0);
+                    w[q] = t - q >= first ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
+                                                              __HIP_MEMORY_SCOPE_AGENT)
+                                          : status_word(kFlagPre, a.epoch, 0);
                 int used = 0;
                 bool done = false, wait = false;
 #pragma unroll
@@ -346,25 +432,18 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
                 }
                 if (done) break;
                 t -= used;
-                if (wait) {
-                    if (++spins > kSpinLimit) {
-                        atomicOr(a.err, 2u);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                }
+                if (wait && !wait_more(wt, a.err, 2u)) break;
             }
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, excl + tot[j]),
                                __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         }
-        s_dst[d] = (uint64_t)a.digit_base[d] + excl;
+        s_dst[d] = (uint64_t)seg_base[d] + excl;
     }
     __syncthreads();
     // stage the tile in LDS in digit order
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
-        uint64_t idx = wbase + (uint64_t)i * kWave + lane;
-        if (idx < a.n) {
+        if (wbase + i * kWave + lane < count) {
             uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
             uint32_t pos = s_local[d] + s_wave_cnt[wave][d] + rank[i];
             s_keys[pos] = k[i];
@@ -373,12 +452,11 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     }
     __syncthreads();
     // coalesced write-out: consecutive slots of one digit are consecutive in HBM
-    const uint32_t count = a.n - base < (uint64_t)kSortTile ? (uint32_t)(a.n - base) : (uint32_t)kSortTile;
     for (uint32_t j = tid; j < count; j += kBlock) {
         uint64_t key = s_keys[j];
         uint32_t d = (uint32_t)(key >> a.shift) & (R - 1);
         uint64_t dst = s_dst[d] + (j - s_local[d]);
-        if (dst >= a.n) {  // only after a failed look-back
+        if (dst >= a.g.n) {  // only after a failed look-back
             atomicOr(a.err, 16u);
             continue;
         }
@@ -421,35 +499,49 @@ __device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
     return (uint64_t)tile * kScanTile + (uint64_t)wave * kWave * kScanItems + (uint64_t)i * kWave + lane;
 }
 
-// unique (gbuild.zig:366-388): keep the last record of each key unless it is
-// invalid or deletes (attribute 0)
-__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, const uint64_t* keys, const uint32_t* vals,
-                                                    uint32_t n, uint64_t invalid, uint64_t* ukey, uint32_t* uval,
-                                                    bool packed) {
+// unique (gbuild.zig:366-388): keep the last record of each key of a segment
+// unless it is invalid or deletes (attribute 0); the kept keys get their
+// chunk index back above the local bits. Scan tiles are the sort tiles cut
+// into `subs` pieces of kScanTile records (never across a segment),
+// compacted in tile order.
+__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t subs, const uint64_t* keys,
+                                                    const uint32_t* vals, uint32_t lbits, uint64_t* ukey,
+                                                    uint32_t* uval, bool packed) {
     __shared__ uint32_t s_tile, s_wave[4];
     __shared__ uint64_t s_b;
-    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
+    const uint64_t invalid = 1ull << lbits;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t ntiles = tile_count(g) * subs;
     for (;;) {
         const uint32_t tile = next_tile(sc, &s_tile);
         if (tile >= ntiles) break;
+        const uint4 td = tile_desc(g, tile / subs);
+        const uint32_t lo = (tile % subs) * kScanTile;
+        const uint32_t len = td.z > lo ? min((uint32_t)kScanTile, td.z - lo) : 0u;
+        const uint32_t base = td.y + lo;
+        const uint32_t end = seg_end(g, td.x);  // records from `end` on belong to the next chunk
+        const uint64_t chunk = (uint64_t)td.x << lbits;
         uint32_t f[kScanItems], e[kScanItems];
         uint64_t k[kScanItems];
         uint32_t v[kScanItems];
 #pragma unroll
         for (int i = 0; i < kScanItems; i++) {
-            uint64_t idx = scan_index(tile, i);
+            const uint32_t li = wave * kWave * kScanItems + i * kWave + lane;
+            const uint32_t idx = base + li;
             f[i] = 0;
-            if (idx < n) {
+            k[i] = 0;
+            v[i] = 0;
+            if (li < len) {
                 if (packed) {  // key << 32 | attribute
                     const uint64_t w = keys[idx];
                     k[i] = w >> 32;
                     v[i] = (uint32_t)w;
-                    bool last = idx + 1 == n || (keys[idx + 1] >> 32) != k[i];
+                    const bool last = idx + 1 == end || (keys[idx + 1] >> 32) != k[i];
                     f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
                 } else {
                     k[i] = keys[idx];
                     v[i] = vals[idx];
-                    bool last = idx + 1 == n || keys[idx + 1] != k[i];
+                    const bool last = idx + 1 == end || keys[idx + 1] != k[i];
                     f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
                 }
             }
@@ -460,11 +552,11 @@ __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, const uint64_t* k
         for (int i = 0; i < kScanItems; i++)
             if (f[i]) {
                 const uint64_t pos = pre + e[i];
-                if (pos >= n) {
+                if (pos >= g.n) {
                     atomicOr(&sc.counters[C_ERR], 16u);
                     continue;
                 }
-                ukey[pos] = k[i];
+                ukey[pos] = chunk | k[i];
                 uval[pos] = v[i];
             }
         if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_U] = (uint32_t)(pre + total);
@@ -1029,8 +1121,11 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (sort_items != 8 && sort_items != 12 && sort_items != 16 && sort_items != 20 && sort_items != 24)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "sort_items must be 8, 12, 16, 20 or 24");
     const uint32_t lbits = local_bits(log2_size);
-    const uint32_t key_bits = lbits + enc.chunk_bits + 1;
-    if (key_bits > 64) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
+    // records sort by chunk-local key inside their chunk's segment: the key
+    // is the local bits plus the invalid bit; the chunk index comes back into
+    // the keys at the unique step (chunk << lbits | local, which must fit)
+    if (lbits + enc.chunk_bits > 63) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
+    const uint32_t key_bits = lbits + 1;
     const uint32_t passes = (key_bits + rb - 1) / rb;
     // keys of at most 32 bits travel packed with their attribute in one u64
     // (16 instead of 24 bytes per record and pass)
@@ -1051,12 +1146,29 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
 
     Alloc A{{}, s};
     uint32_t* counters = A.get<uint32_t>(C_COUNT);
-    uint32_t* hist = A.get<uint32_t>((uint64_t)passes * radix * 2);
-    uint32_t* digit_base = hist + (uint64_t)passes * radix;
-    const uint64_t sort_tile = (uint64_t)kBlock * sort_items;
-    const uint64_t sort_tiles = (nmax + sort_tile - 1) / sort_tile;
+    const uint32_t segs = enc.chunk_count ? enc.chunk_count : 1;
+    const uint64_t hist_words = (uint64_t)passes * segs * radix;
+    uint32_t* hist = A.get<uint32_t>(hist_words * 2);
+    uint32_t* digit_base = hist + hist_words;
+    const uint32_t sort_tile = kBlock * sort_items;
+    // record tiles: a single build has exactly ceil(n / tile); a batch's are
+    // counted on the device, at most one partial tile more per chunk
+    const uint64_t sort_tiles = (nmax + sort_tile - 1) / sort_tile + (enc.chunk_count ? enc.chunk_count : 0);
+    if (sort_tiles * sort_tile >= 0xFFFFFFFFull)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "too many voxels and chunks for one build");
+    Segs seg{};
+    seg.single = enc.chunk_count == 0;
+    seg.count = segs;
+    seg.n = n;
+    seg.tile = sort_tile;
+    seg.off = enc.offsets;
+    uint32_t* tile_first = enc.chunk_count ? A.get<uint32_t>(enc.chunk_count + 1) : nullptr;
+    seg.desc = enc.chunk_count ? A.get<uint4>(sort_tiles) : nullptr;
+    seg.tiles = counters + C_TILES;
     const uint64_t scan_tiles = (nmax + kScanTile - 1) / kScanTile;
-    const uint64_t status_words = std::max<uint64_t>(sort_tiles * radix, scan_tiles);
+    // sort passes: one word per (tile, digit); unique: one per kScanTile piece of a tile
+    const uint64_t status_words =
+        std::max<uint64_t>(sort_tiles * std::max<uint32_t>(radix, (sort_tile + kScanTile - 1) / kScanTile), scan_tiles);
     uint64_t* status = A.get<uint64_t>(status_words);
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
@@ -1109,7 +1221,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     tm.mark();
     // zero state
     fill32(s, counters, 0, C_COUNT);
-    fill32(s, hist, 0, (uint64_t)passes * radix);
+    fill32(s, hist, 0, hist_words);
     fill64(s, status, 0, status_words);
     fill64(s, masks, 0, nb);
     if (enc.chunk_count) fill32(s, roots_dev, 0xFFFFFFFFu, enc.chunk_count * 2);
@@ -1125,18 +1237,26 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     enc.radix_bits = rb;
     enc.packed = packed;
     if (n) {
-        const uint32_t offs_lds =
-            enc.chunk_count && enc.chunk_count + 1 <= kLdsOffsets ? (enc.chunk_count + 1) * 4 : 0;
-        hipLaunchKernelGGL(k_encode, dim3(grid_for(n, kBlock * 16, 2048)), dim3(kBlock),
-                           passes * radix * sizeof(uint32_t) + offs_lds, s, enc, n);
-        hipLaunchKernelGGL(k_digit_scan, dim3(1), dim3(kBlock), 0, s, hist, digit_base, passes, radix);
+        if (enc.chunk_count) {
+            hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(kBlock), 0, s, enc.offsets, enc.chunk_count, n, sort_tile,
+                               tile_first, counters + C_TILES, counters + C_ERR);
+            hipLaunchKernelGGL(k_seg_desc, dim3(grid_for(enc.chunk_count, kBlock / kWave, 1024)), dim3(kBlock), 0, s,
+                               enc.offsets, tile_first, enc.chunk_count, sort_tile, counters + C_TILES,
+                               (uint4*)seg.desc);
+        }
+        const uint32_t eblocks = (uint32_t)std::min<uint64_t>(sort_tiles, 2048);
+        const uint32_t per_block = (uint32_t)((sort_tiles + eblocks - 1) / eblocks);
+        hipLaunchKernelGGL(k_encode, dim3(eblocks), dim3(kBlock), passes * radix * sizeof(uint32_t), s, enc, seg,
+                           per_block);
+        hipLaunchKernelGGL(k_digit_scan, dim3(passes * segs), dim3(kBlock), 0, s, hist, digit_base, seg, passes,
+                           radix);
     }
     tm.mark();
     // 2. radix passes
     uint32_t cur = 0;
     for (uint32_t p = 0; p < passes && n; p++) {
-        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], n, (packed ? 32 : 0) + p * rb,
-                    digit_base + p * radix, status, ++epoch, tile_ctr(), counters + C_ERR};
+        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], seg, (packed ? 32 : 0) + p * rb,
+                    digit_base + (uint64_t)p * segs * radix, status, ++epoch, tile_ctr(), counters + C_ERR};
         launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
         cur ^= 1;
     }
@@ -1144,12 +1264,12 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // 3. unique into the other buffer pair, which becomes Built.keys/attrs
     uint64_t* ukey = kb[cur ^ 1];
     uint32_t* uval = packed ? vb[0] : vb[cur ^ 1];
-    const uint64_t invalid = 1ull << (lbits + enc.chunk_bits);
     const uint32_t scan_grid = grid_for(nmax, kScanTile, 2048);
     if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
-        hipLaunchKernelGGL(k_unique, dim3(scan_grid), dim3(kBlock), 0, s, sc, kb[cur], vb[cur], n, invalid, ukey,
-                           uval, packed);
+        const uint32_t subs = (sort_tile + kScanTile - 1) / kScanTile;
+        hipLaunchKernelGGL(k_unique, dim3((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048)), dim3(kBlock), 0, s,
+                           sc, seg, subs, kb[cur], vb[cur], lbits, ukey, uval, packed);
         ScanCtx sb{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, ukey, lbits, brick_key, masks,
                            first_dev, nb, nmax);

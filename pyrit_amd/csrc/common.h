@@ -88,13 +88,35 @@ __device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
 // prefix), [61:40] epoch of the launch, [39:0] value. A word from another
 // launch (other epoch) reads as "not ready", so one zero-initialised status
 // array serves every launch of a build without clearing. Waits are bounded:
-// after kSpinLimit polls the launch records an error and goes on, so every
-// wave terminates even if a predecessor never publishes.
+// a wait gives up after kWaitTicks of the constant 100 MHz clock, or as soon
+// as any other wait of the build has given up (its error word is non-zero),
+// records an error and goes on, so every wave terminates quickly even if a
+// predecessor never publishes.
 // ---------------------------------------------------------------------------
 constexpr uint64_t kFlagAgg = 1ull << 62;
 constexpr uint64_t kFlagPre = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 40) - 1;
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint64_t kWaitTicks = 100000000ull;  // 1 s at 100 MHz
+
+struct Wait {
+    uint64_t t0 = 0;
+    uint32_t polls = 0;
+};
+
+// One more poll of a bounded wait: false = give up (error bit `bit` is set
+// in *err unless another wait set an error first).
+__device__ __forceinline__ bool wait_more(Wait& w, uint32_t* err, uint32_t bit) {
+    if (w.polls == 0) w.t0 = wall_clock64();
+    if ((++w.polls & 31) == 0) {
+        if (__hip_atomic_load(err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return false;
+        if (wall_clock64() - w.t0 > kWaitTicks) {
+            atomicOr(err, bit);
+            return false;
+        }
+    }
+    __builtin_amdgcn_s_sleep(1);
+    return true;
+}
 
 __device__ __forceinline__ uint64_t status_word(uint64_t flag, uint32_t epoch, uint64_t v) {
     return flag | ((uint64_t)(epoch & 0x3FFFFFu) << 40) | (v & kValueMask);
@@ -122,21 +144,19 @@ __device__ __forceinline__ uint64_t lookback_wave(uint64_t* status, uint32_t til
                            __HIP_MEMORY_SCOPE_AGENT);
     uint64_t excl = 0;
     int64_t pos = (int64_t)tile - 1;
-    uint32_t spins = 0;
+    Wait wt;
     while (true) {
         int64_t idx = pos - (int64_t)lane;
         uint64_t w = idx >= 0 ? __hip_atomic_load(&status[idx], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
                               : status_word(kFlagPre, epoch, 0);
         uint32_t f = status_flag(w, epoch);
-        if (__ballot(f == 0) != 0) {
-            if (++spins > kSpinLimit) {
-                if (lane == 0) atomicOr(err, 1u);
-                break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+        const uint64_t pre = __ballot(f == 2);
+        // only the lanes up to the newest inclusive prefix are needed
+        const uint64_t need = pre ? (pre & (~pre + 1)) * 2 - 1 : ~0ull;
+        if (__ballot(f == 0) & need) {
+            if (__ballot(!wait_more(wt, err, 1u)) != 0) break;
             continue;
         }
-        uint64_t pre = __ballot(f == 2);
         uint64_t v = w & kValueMask;
         if (pre) {
             uint32_t first = (uint32_t)__ffsll((unsigned long long)pre) - 1;
