@@ -101,7 +101,7 @@ def build_section(args, out):
     ms = ev.stop() / args.steps
     synchronize()
     gpu.set_timing(False)
-    passes = phases[-1]["sort_passes"]
+    passes = max(1, phases[-1]["sort_passes"])  # passes that moved records
     rb = phases[-1]["radix_bits"]
     rec = phases[-1]["record_bytes"]  # 8: packed key << 32 | attribute, 12: key + attribute arrays
     items = phases[-1]["sort_items"]
@@ -209,10 +209,41 @@ def trace_section(args, out):
         "primary_plus_shadow_mrays_per_s": (nrays + hit_total * poses) / (both_ms * 1e-3) / 1e6,
         "last_frame_hit_fraction": hit_total / (W * H),
     }
+    if args.cpu_trace:
+        out["traversal"]["cpu_baseline"] = cpu_trace_baseline(args, built, cams[0], W, H)
     L.pyrhip_built_free(C.byref(built), None)
     for b in (hits, rays, occ):
         b.free()
     return dag, w, cams
+
+
+def cpu_trace_baseline(args, built, cam, W, H):
+    """The oracle's restatement of the reference traversal (walk + traceScene)
+    on the same DAG (downloaded from the GPU build) and the first pose's primary
+    rays at half resolution, on 1 host core and on every host core (rays are
+    independent, as in the reference's CPU tests)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_py as O
+    from pyrit_amd.device import download
+
+    nodes = download(built.nodes, np.uint32, built.node_words)
+    leaves = download(built.leaves, np.uint64, built.leaf_count)
+    attrs = download(built.attrs, np.uint32, built.voxel_count)
+    dag = O.Dag(built.log2_size, built.root, nodes, leaves, attrs, built.voxel_count)
+    scene = O.single_scene(dag)
+    oc = O.OrCamera.from_buffer_copy(bytes(cam))
+    oc.width, oc.height = W // 2, H // 2  # same aspect: the projection scale is unchanged
+    rays = O.camera_rays(oc)
+    res = {"sample": f"pose 0 primary rays at {W // 2}x{H // 2} ({len(rays)} rays), same DAG", "unit": "Mrays/s",
+           "kind": "port"}
+    for label, threads in (("one_core", 1), ("all_cores", os.cpu_count() or 1)):
+        t0 = time.perf_counter()
+        O.trace_rays(scene, rays, threads=threads)
+        dt = time.perf_counter() - t0
+        res[label] = {"value": len(rays) / dt / 1e6, "cores": threads, "seconds": dt}
+    log(f"cpu traversal {res['one_core']['value']:.2f} Mrays/s (1 core), "
+        f"{res['all_cores']['value']:.1f} Mrays/s ({res['all_cores']['cores']} cores)")
+    return res
 
 
 def batch_section(args, out):
@@ -375,6 +406,7 @@ def main() -> int:
         print(json.dumps({"error": "no HIP device visible"}), flush=True)
         return 1
     only = set(args.only.split(",")) if args.only else {"build", "trace", "batch", "serve", "cpu", "probe"}
+    args.cpu_trace = "cpu" in only
     out = {}
     n, ms = 0, float("nan")
     if "probe" in only:

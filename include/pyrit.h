@@ -261,6 +261,105 @@ PYR_API void pyr_camera_look_at(PyrCamera* camera, const float eye[3], const flo
 PYR_API void pyr_camera_perspective(PyrCamera* camera, float fov_y_radians, uint32_t width, uint32_t height,
                                     float near_plane);
 
+/* ---- streamed worlds (reference pyrit.h:789-910) --------------------------
+ *
+ * A world keeps a LOD octree of chunks around the camera: every chunk has
+ * 2^chunk_log2 cells per edge, and a cell of level `lod` spans 2^lod base
+ * voxels. The application's generator writes each requested chunk's voxels on
+ * the GPU; the chunks of one update are built as one GPU chunk batch and
+ * entered into the context scene as instances (scale 2^lod, placed relative
+ * to the render origin). A node is refined while its cells would look larger
+ * than voxel_pixels on screen; children replace their parent only when all
+ * eight are built (or empty). Generation and builds run on a worker thread
+ * with its own stream unless PYR_WORLD_SYNC is set. Destroy a world before
+ * its context. */
+
+typedef struct PyrWorld PyrWorld;
+
+typedef struct PyrChunkKey {
+    int32_t x, y, z; /* chunk origin = (x, y, z) * 2^(chunk_log2 + lod) base voxels */
+    uint32_t lod;
+} PyrChunkKey;
+
+/* One generator request. For every chunk c, write voxels (chunk-local cell
+ * coordinates in [0, 2^chunk_log2), one cell = 2^lod base voxels) as
+ *     uint32_t k = atomicAdd(&counts[c], 1);
+ *     if (k < capacity) voxels[c * capacity + k] = v;
+ * counts are zero on entry. Emit the visible skin, not full volumes. */
+typedef struct PyrWorldGenParams {
+    uint64_t chunks; /* device PyrChunkKey[count] */
+    uint64_t voxels; /* device PyrVoxel[count * capacity] */
+    uint64_t counts; /* device uint32_t[count] */
+    uint32_t count, capacity, chunk_log2, reserved;
+    uint64_t user;   /* PyrWorldInfo.user */
+} PyrWorldGenParams;
+
+/* Launches the application's kernels on `stream` (a hipStream_t); must not
+ * synchronise. */
+typedef void (*PyrWorldGenFn)(void* user, const PyrWorldGenParams* params, void* stream);
+
+/* generate and build inside pyr_world_update / pyr_world_wait instead of on
+ * the worker thread (deterministic; for tests and tools) */
+#define PYR_WORLD_SYNC 0x1u
+
+typedef struct PyrWorldInfo {
+    uint32_t chunk_log2;        /* 3..8, 0 = 5 */
+    uint32_t max_lod;           /* deepest level, 0 = 20 */
+    uint32_t view_chunks;       /* radius of the coarsest level in chunks, 0 = 2 */
+    float voxel_pixels;         /* target on-screen size of a cell, 0 = 4 */
+    float view_distance;        /* base voxels, 0 = 16384 */
+    uint32_t reserved0;
+    uint64_t memory_budget;     /* bytes of chunk DAGs, 0 = 256 MiB */
+    int32_t y_min, y_max;       /* vertical extent in base voxels (required) */
+    uint32_t chunks_per_update; /* chunks per generated batch, 0 = chosen from the capacity */
+    uint32_t mask;              /* instance mask, 0 = 0x1 */
+    uint32_t secondary_mask;    /* accepted; this implementation keeps one selection */
+    float secondary_pixels;
+    uint32_t chunk_capacity;    /* generator slots per chunk, 0 = 8 * 4^chunk_log2 (at most 8^chunk_log2) */
+    uint32_t rt_leaf_log2;      /* accepted (no RT cores on this device) */
+    uint32_t keep_frames;       /* frames an unused chunk stays resident, 0 = 8 */
+    uint32_t flags;             /* PYR_WORLD_* */
+    PyrWorldGenFn generate;     /* required */
+    void* user;
+} PyrWorldInfo;
+
+typedef struct PyrWorldStats {
+    uint32_t visible_chunks, resident_chunks, pending_chunks;
+    uint32_t built_chunks, built_voxels; /* in the last update */
+    uint32_t overflow_chunks;            /* chunks whose generator output exceeded the capacity (total) */
+    float voxel_pixels;
+    uint32_t top_lod;
+    float secondary_bias;
+    uint32_t path_edited;
+    uint64_t bytes;                      /* device bytes of the resident chunk DAGs */
+} PyrWorldStats;
+
+/* one base voxel set (attribute != 0) or removed (attribute 0) */
+typedef struct PyrWorldEdit {
+    int64_t x, y, z;
+    uint32_t attribute;
+    uint32_t reserved;
+} PyrWorldEdit;
+
+PYR_API PyrResult pyr_world_create(PyrContext* ctx, const PyrWorldInfo* info, PyrWorld** out);
+PYR_API PyrResult pyr_world_destroy(PyrContext* ctx, PyrWorld* world);
+/* position: camera in base voxels; origin: render origin as in PyrFrameInfo
+ * (NULL = 0); camera: next frame's camera for the screen-size LOD choice
+ * (NULL = 1080p, 60 degrees). Never waits for the GPU (unless PYR_WORLD_SYNC). */
+PYR_API PyrResult pyr_world_update(PyrContext* ctx, PyrWorld* world, const double position[3], const double origin[3],
+                                   const PyrCamera* camera);
+/* waits for the batch being generated and built, and takes it over */
+PYR_API PyrResult pyr_world_wait(PyrContext* ctx, PyrWorld* world, const double camera[3]);
+/* Edits overlay the generator: affected resident chunks are rebuilt at the
+ * next update, and every later generation of a chunk applies them again. On
+ * coarser levels a set fills its cell; a removal empties a cell only once all
+ * of the cell's base voxels are removed. */
+PYR_API PyrResult pyr_world_edit(PyrContext* ctx, PyrWorld* world, const PyrWorldEdit* edits, uint32_t count);
+PYR_API uint64_t pyr_world_edits_bytes(PyrWorld* world);
+PYR_API PyrResult pyr_world_edits_save(PyrWorld* world, void* dst, uint64_t size);
+PYR_API PyrResult pyr_world_edits_load(PyrContext* ctx, PyrWorld* world, const void* src, uint64_t size);
+PYR_API PyrResult pyr_world_stats(PyrWorld* world, PyrWorldStats* out);
+
 #ifdef __cplusplus
 }
 #endif

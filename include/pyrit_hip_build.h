@@ -26,8 +26,8 @@ extern "C" {
 /* Built DAG in the GPU-native layout (src/gpu_build.zig:25-47): nodes level by
  * level bottom-up, word refs absolute, leaves in first-appearance Morton order,
  * attributes in DFS rank order. All buffers are device memory owned by the
- * caller, taken from the device's stream-ordered memory pool: release them
- * with pyrhip_built_free (hipFreeAsync), not hipFree. `keys`/`attrs` hold the sorted unique
+ * caller, taken from the library's device-memory cache: release them with
+ * pyrhip_built_free, not hipFree. `keys`/`attrs` hold the sorted unique
  * voxels (brick Morton << 6 | bit) for later edits. */
 typedef struct PyrhipBuilt {
     uint32_t log2_size;
@@ -103,6 +103,9 @@ typedef struct PyrhipBuildTimes {
     uint32_t sort_passes, radix_bits;
     uint32_t record_bytes; /* bytes per record moved by a sort pass: 8 packed, 12 pairs */
     uint32_t sort_items;
+    /* host milliseconds: allocating device memory, enqueueing the kernels,
+     * waiting for the final read-back */
+    float host_alloc_ms, host_enqueue_ms, host_wait_ms;
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 8): the
@@ -110,6 +113,13 @@ PYR_API void pyrhip_set_timing(int enable);
  * counterpart in the reference. */
 PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream);
 PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out);
+/* The device-memory cache that builds, their outputs and server generations
+ * allocate from (freed blocks stay mapped for reuse): out = {limit of cached
+ * free bytes, bytes held, bytes in use, bytes cached free}. pyrhip_trim gives
+ * every cached free block back to the driver. No counterpart in the
+ * reference (its builds allocate from the CUDA stream-ordered pool). */
+PYR_API PyrResult pyrhip_pool_stats(uint64_t out[4]);
+PYR_API void pyrhip_trim(void);
 
 /* One DAG in device memory for pyrhip_trace_dag (a single instance with the
  * identity transform; src/device/dag.zig:24-36 Dag). */

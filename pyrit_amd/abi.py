@@ -84,7 +84,7 @@ class PyrhipBuildTuning(C.Structure):
 class PyrhipBuildTimes(C.Structure):
     _fields_ = [("encode_ms", f32), ("sort_ms", f32), ("unique_ms", f32), ("leaves_ms", f32), ("levels_ms", f32),
                 ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32), ("record_bytes", u32),
-                ("sort_items", u32)]
+                ("sort_items", u32), ("host_alloc_ms", f32), ("host_enqueue_ms", f32), ("host_wait_ms", f32)]
 
 
 class PyrhipDag(C.Structure):
@@ -115,6 +115,41 @@ class PyrhipServerStats(C.Structure):
 
 PYRHIP_REQUEST_BUILD = 0
 PYRHIP_REQUEST_EDIT = 1
+
+
+class PyrChunkKey(C.Structure):
+    _fields_ = [("x", i32), ("y", i32), ("z", i32), ("lod", u32)]
+
+
+class PyrWorldGenParams(C.Structure):
+    _fields_ = [("chunks", u64), ("voxels", u64), ("counts", u64), ("count", u32), ("capacity", u32),
+                ("chunk_log2", u32), ("reserved", u32), ("user", u64)]
+
+
+class PyrWorldInfo(C.Structure):
+    _fields_ = [("chunk_log2", u32), ("max_lod", u32), ("view_chunks", u32), ("voxel_pixels", f32),
+                ("view_distance", f32), ("reserved0", u32), ("memory_budget", u64), ("y_min", i32), ("y_max", i32),
+                ("chunks_per_update", u32), ("mask", u32), ("secondary_mask", u32), ("secondary_pixels", f32),
+                ("chunk_capacity", u32), ("rt_leaf_log2", u32), ("keep_frames", u32), ("flags", u32),
+                ("generate", vp), ("user", vp)]
+
+
+class PyrWorldStats(C.Structure):
+    _fields_ = [("visible_chunks", u32), ("resident_chunks", u32), ("pending_chunks", u32), ("built_chunks", u32),
+                ("built_voxels", u32), ("overflow_chunks", u32), ("voxel_pixels", f32), ("top_lod", u32),
+                ("secondary_bias", f32), ("path_edited", u32), ("bytes", u64)]
+
+
+class PyrWorldEdit(C.Structure):
+    _fields_ = [("x", C.c_int64), ("y", C.c_int64), ("z", C.c_int64), ("attribute", u32), ("reserved", u32)]
+
+
+PYR_WORLD_SYNC = 0x1
+
+
+class PyrTerrainWorld(C.Structure):
+    _fields_ = [("seed", u32), ("mean", i32), ("amplitude", i32), ("octaves", u32), ("wavelength_log2", u32),
+                ("reserved", u32)]
 
 
 class PyrTerrainCave(C.Structure):
@@ -164,6 +199,8 @@ FUNCTIONS = {
     "pyrhip_set_timing": (None, [i32]),
     "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),
     "pyrhip_last_build_times": (None, [P(PyrhipBuildTimes)]),
+    "pyrhip_pool_stats": (i32, [P(u64)]),
+    "pyrhip_trim": (None, []),
     "pyrhip_trace_dag": (i32, [P(PyrhipDag), u64, u64, u32, u32, vp]),
     "pyrhip_trace_camera": (i32, [P(PyrhipDag), P(PyrCamera), u64, u32, vp]),
     "pyrhip_camera_rays": (i32, [P(PyrCamera), u64, vp]),
@@ -176,6 +213,15 @@ FUNCTIONS = {
     "pyrhip_server_clock": (f64, []),
     "pyrhip_server_stats": (None, [vp, P(PyrhipServerStats)]),
     "pyrhip_server_destroy": (None, [vp]),
+    "pyr_world_create": (i32, [vp, P(PyrWorldInfo), P(vp)]),
+    "pyr_world_destroy": (i32, [vp, vp]),
+    "pyr_world_update": (i32, [vp, vp, P(f64), P(f64), P(PyrCamera)]),
+    "pyr_world_wait": (i32, [vp, vp, P(f64)]),
+    "pyr_world_edit": (i32, [vp, vp, vp, u32]),
+    "pyr_world_edits_bytes": (u64, [vp]),
+    "pyr_world_edits_save": (i32, [vp, vp, u64]),
+    "pyr_world_edits_load": (i32, [vp, vp, vp, u64]),
+    "pyr_world_stats": (i32, [vp, P(PyrWorldStats)]),
 }
 
 TERRAIN_FUNCTIONS = {
@@ -183,6 +229,8 @@ TERRAIN_FUNCTIONS = {
     "pyrgen_scratch_bytes": (u64, [u64]),
     "pyrgen_height_host": (i32, [P(PyrTerrainChunk), i32, i32]),
     "pyrgen_generate_device": (i32, [u64, u32, u64, u64, u64, u64, u64, u64, u64, u64, vp]),
+    "pyrgen_world_generate": (None, [vp, P(PyrWorldGenParams), vp]),
+    "pyrgen_world_chunk_host": (u32, [P(PyrTerrainWorld), P(PyrChunkKey), u32, vp, u32]),
 }
 
 

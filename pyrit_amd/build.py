@@ -33,7 +33,7 @@ COMMON = [
 ]
 
 LIBS = {
-    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp"],
+    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp"],
     "libpyrit_terrain.so": ["terrain.hip"],
 }
 

@@ -33,6 +33,15 @@ struct Geometry {
     uint32_t* vals = nullptr;
     uint32_t voxel_count = 0;
     uint32_t users = 0;  // instances referring to it
+    int32_t shared = -1; // >= 0: pool ranges of a chunk batch shared with other geometries
+};
+
+// pool ranges of one installed chunk batch; every chunk's geometry refers to
+// the batch's node and leaf ranges and to its own slice of the attributes
+struct SharedRange {
+    uint64_t node_offset = 0, leaf_offset = 0, attr_offset = 0;
+    uint64_t node_words = 0, leaf_count = 0, attr_count = 0;
+    uint32_t refs = 0;
 };
 
 struct Instance {
@@ -103,6 +112,7 @@ struct PyrContext {
     bool own_stream = false;
     Pool nodes, leaves, attrs;
     std::vector<Geometry> geos;
+    std::vector<SharedRange> shared;
     std::vector<Instance> insts;
     uint32_t max_instances = 0, max_geometries = 0;
     uint64_t frame = 0;
@@ -152,7 +162,21 @@ PyrResult pool_init(Pool& p, uint64_t bytes, uint64_t elem) {
     return hip_check(hipMalloc(&p.base, bytes), "pool allocation");
 }
 
+void release_shared(PyrContext* ctx, int32_t id) {
+    SharedRange& r = ctx->shared[(size_t)id];
+    if (r.refs == 0 || --r.refs > 0) return;
+    ctx->nodes.ra.release(r.node_offset, r.node_words);
+    ctx->leaves.ra.release(r.leaf_offset, r.leaf_count);
+    ctx->attrs.ra.release(r.attr_offset, r.attr_count);
+    r = SharedRange();
+}
+
 void release_geometry_ranges(PyrContext* ctx, Geometry& g) {
+    if (g.shared >= 0) {
+        release_shared(ctx, g.shared);
+        g.shared = -1;
+        return;
+    }
     ctx->nodes.ra.release(g.data.node_offset, g.node_words);
     ctx->leaves.ra.release(g.data.leaf_offset, g.leaf_count);
     ctx->attrs.ra.release(g.data.attribute_offset, g.attr_count);
@@ -219,7 +243,7 @@ PyrResult stage_voxels(PyrContext* ctx, const void* voxels, uint32_t count, uint
     *owned = false;
     *dev = (void*)voxels;
     if (!(flags & PYR_BUILD_HOST_INPUT) || count == 0) return PYR_OK;
-    PyrResult r = hip_check(hipMalloc(dev, (uint64_t)count * sizeof(PyrVoxel)), "voxel upload buffer");
+    PyrResult r = hip_check(pool_alloc(dev, (uint64_t)count * sizeof(PyrVoxel), ctx->stream), "voxel upload buffer");
     if (r != PYR_OK) return r;
     *owned = true;
     return hip_check(hipMemcpyAsync(*dev, voxels, (uint64_t)count * sizeof(PyrVoxel), hipMemcpyHostToDevice,
@@ -228,6 +252,75 @@ PyrResult stage_voxels(PyrContext* ctx, const void* voxels, uint32_t count, uint
 }
 
 }  // namespace
+
+// ---- chunk batches for streamed worlds (context.zig:1287+ installChunkBatch) ----
+
+namespace pyr {
+
+hipStream_t ctx_stream(PyrContext* ctx) { return ctx->stream; }
+
+// Copies a finished chunk batch into the pools once (nodes, leaves and all
+// attributes); on success the batch buffers are released and *id names the
+// shared ranges, held by one reference until ctx_release_batch.
+PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* b, int32_t* id) {
+    Geometry tmp;
+    PyrResult r = install(ctx, tmp, *b, false);
+    if (r != PYR_OK) return r;
+    memset(b, 0, sizeof(*b));  // install released the batch buffers
+    size_t slot = 0;
+    while (slot < ctx->shared.size() && ctx->shared[slot].refs) slot++;
+    if (slot == ctx->shared.size()) ctx->shared.emplace_back();
+    SharedRange& sr = ctx->shared[slot];
+    sr.node_offset = tmp.data.node_offset;
+    sr.leaf_offset = tmp.data.leaf_offset;
+    sr.attr_offset = tmp.data.attribute_offset;
+    sr.node_words = tmp.node_words;
+    sr.leaf_count = tmp.leaf_count;
+    sr.attr_count = tmp.attr_count;
+    sr.refs = 1;
+    *id = (int32_t)slot;
+    return PYR_OK;
+}
+
+void ctx_release_batch(PyrContext* ctx, int32_t id) {
+    if (id >= 0 && (size_t)id < ctx->shared.size()) release_shared(ctx, id);
+}
+
+uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t id) {
+    const SharedRange& r = ctx->shared[(size_t)id];
+    return r.node_words * 4 + r.leaf_count * 8 + r.attr_count * 4;
+}
+
+// A geometry for one chunk of an installed batch: the batch's node and leaf
+// ranges (word refs are relative to the batch start), the chunk's root and
+// its attributes from rank first_attr of the batch on.
+PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, uint32_t first_attr,
+                                  uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out) {
+    uint32_t slot = 0;
+    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
+    if (slot == ctx->geos.size()) ctx->geos.emplace_back();
+    SharedRange& sr = ctx->shared[(size_t)id];
+    Geometry& g = ctx->geos[slot];
+    const uint32_t gen = g.gen + 1;
+    g = Geometry();
+    g.gen = gen;
+    g.data.node_offset = (uint32_t)sr.node_offset;
+    g.data.leaf_offset = (uint32_t)sr.leaf_offset;
+    g.data.attribute_offset = (uint32_t)(sr.attr_offset + first_attr);
+    g.data.root = root;
+    g.data.log2_size = log2_size;
+    g.data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;
+    g.data.default_attribute = 1;
+    g.voxel_count = voxel_count;
+    g.shared = id;
+    sr.refs++;
+    g.live = true;
+    *out = (PyrGeometry)(uintptr_t)make_handle(slot, g.gen);
+    return PYR_OK;
+}
+
+}  // namespace pyr
 
 extern "C" {
 
@@ -345,7 +438,7 @@ PYR_API PyrResult pyr_geometry_build(PyrContext* ctx, uint32_t log2_size, const 
     PyrResult r = stage_voxels(ctx, voxels, count, flags, &dev, &owned);
     PyrhipBuilt b{};
     if (r == PYR_OK) r = pyrhip_build(log2_size, 0, 0, 0, (uint64_t)(uintptr_t)dev, count, nullptr, ctx->stream, &b);
-    if (owned) (void)hipFree(dev);
+    if (owned) pool_free(dev, ctx->stream);
     if (r != PYR_OK) return r;
     if (slot == ctx->geos.size()) ctx->geos.emplace_back();
     Geometry& g = ctx->geos[slot];
@@ -378,7 +471,7 @@ PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const
     if (r == PYR_OK)
         r = pyrhip_build(g->data.log2_size, (uint64_t)(uintptr_t)g->keys, (uint64_t)(uintptr_t)g->vals,
                          g->voxel_count, (uint64_t)(uintptr_t)dev, count, nullptr, ctx->stream, &b);
-    if (owned) (void)hipFree(dev);
+    if (owned) pool_free(dev, ctx->stream);
     if (r != PYR_OK) return r;
     Geometry old = *g;
     release_geometry_ranges(ctx, *g);

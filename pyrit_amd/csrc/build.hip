@@ -30,6 +30,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <mutex>
 #include <vector>
 
@@ -51,6 +52,9 @@ enum : uint32_t {
     C_TILES = 6,   // record tiles of a batch (segment-aligned)
     C_NCH = 8,     // + lv: elements of level lv (children of the level-lv parents)
     C_CUR = 40,    // + lv: node-word cursor when level lv's parents are written
+    C_PASS = 64,   // bit p: radix pass p moves records (some segment has two digits)
+    C_PSRC = 65,   // bit p: record buffer pass p reads (it writes the other)
+    C_FINAL = 66,  // record buffer that holds the sorted records
     C_TILE = 72,   // tile counters, one per launch
     C_COUNT = 256,
 };
@@ -175,41 +179,69 @@ struct EncodeArgs {
     uint32_t packed;          // keys[i] = key << 32 | attribute, vals unused
 };
 
-// Chunk-local sort key and attribute of record i (position i of the record
-// array, the `local`-th record of segment `seg`); out-of-range records get
-// the invalid bit just above the local key bits, so they sort to the end of
-// their segment. encode (gbuild.zig:184-199) and pack_chunks
-// (gbuild.zig:203-230) without the chunk index in the key.
-__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, uint32_t local, uint32_t i,
-                                           uint64_t& key, uint32_t& val) {
+// Input record i of the record array (the `local`-th record of segment
+// `seg`), or x = -1 (out of range) when it has none: a batch record past its
+// chunk's capacity. Old keys of an edit build are not read here.
+__device__ __forceinline__ int4 fetch_voxel(const EncodeArgs& a, uint32_t seg, uint32_t local, uint32_t i) {
+    if (i < a.n_old || (a.capacity && local >= a.capacity)) return make_int4(-1, -1, -1, 0);
+    const uint64_t src = a.chunk_count == 0 ? (uint64_t)(i - a.n_old)
+                                            : (a.capacity ? (uint64_t)seg * a.capacity + local : (uint64_t)i);
+    return a.voxels[src];
+}
+
+// Chunk-local sort key and attribute of record i from its fetched voxel;
+// out-of-range records get the invalid bit just above the local key bits, so
+// they sort to the end of their segment. encode (gbuild.zig:184-199) and
+// pack_chunks (gbuild.zig:203-230) without the chunk index in the key.
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, uint32_t i, int4 v, uint64_t& key,
+                                           uint32_t& val) {
     const uint32_t lbits = local_bits(a.log2_size);
-    key = 1ull << lbits;
-    val = 0;
     if (i < a.n_old) {  // earlier build's sorted unique voxels (single builds)
         key = a.old_keys[i];
         val = a.old_vals[i];
         return;
     }
-    int4 v;
-    int32_t n;
-    if (a.chunk_count == 0) {
-        v = a.voxels[i - a.n_old];
-        n = 1 << a.log2_size;
-    } else {
-        if (a.capacity && local >= a.capacity) return;
-        v = a.voxels[a.capacity ? (uint64_t)seg * a.capacity + local : (uint64_t)i];
-        n = 1 << (a.chunk_log2 ? a.chunk_log2[seg] : a.log2_size);
-    }
+    const int32_t n = 1 << (a.chunk_count && a.chunk_log2 ? a.chunk_log2[seg] : a.log2_size);
+    key = 1ull << lbits;
+    val = 0;
     if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
         key = local_key((uint32_t)v.x, (uint32_t)v.y, (uint32_t)v.z);
         val = (uint32_t)v.w;
     }
 }
 
+constexpr int kEncodeUnroll = 4;  // records per thread whose loads are in flight together
+
 // Records of a contiguous run of tiles per block, plus the digit histograms
 // of every pass per segment: counted in LDS and flushed to the global
 // histogram (one atomic per non-zero bin) whenever the block's run moves to
 // the next segment.
+// Writes record i and counts its digits of every pass in the LDS histogram.
+// Neighbouring input records often share their high digits: a digit common
+// to every active lane of the wave costs one LDS atomic instead of up to 64
+// serialised ones on the same bin.
+__device__ __forceinline__ void emit_record(const EncodeArgs& a, uint32_t* s_hist, uint32_t radix, uint32_t i,
+                                            uint64_t key, uint32_t val) {
+    if (a.packed) {
+        a.keys[i] = (key << 32) | val;
+    } else {
+        a.keys[i] = key;
+        a.vals[i] = val;
+    }
+    const uint64_t active = __ballot(1);
+    for (uint32_t p = 0; p < a.passes; p++) {
+        const uint32_t dg = (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1));
+        const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
+        const uint64_t same = __ballot(dg == d0);
+        if (same == active) {
+            if (lane_id() == (uint32_t)__ffsll((unsigned long long)active) - 1)
+                atomicAdd(&s_hist[p * radix + d0], (uint32_t)__popcll(same));
+        } else {
+            atomicAdd(&s_hist[p * radix + dg], 1u);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_t tiles_per_block) {
     extern __shared__ uint32_t s_hist[];
     const uint32_t radix = 1u << a.radix_bits, bins = a.passes * radix;
@@ -238,19 +270,24 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
             cur = d.x;
         }
         const uint32_t local0 = (t - d.w) * g.tile;
-        for (uint32_t j = threadIdx.x; j < d.z; j += kBlock) {
-            const uint32_t i = d.y + j;
-            uint64_t key;
-            uint32_t val;
-            encode_one(a, d.x, local0 + j, i, key, val);
-            if (a.packed) {
-                a.keys[i] = (key << 32) | val;
-            } else {
-                a.keys[i] = key;
-                a.vals[i] = val;
+        for (uint32_t j0 = 0; j0 < d.z; j0 += kBlock * kEncodeUnroll) {
+            // the input loads of kEncodeUnroll records are issued before any is used
+            int4 raw[kEncodeUnroll];
+#pragma unroll
+            for (int u = 0; u < kEncodeUnroll; u++) {
+                const uint32_t j = j0 + u * kBlock + threadIdx.x;
+                raw[u] = j < d.z ? fetch_voxel(a, d.x, local0 + j, d.y + j) : make_int4(-1, -1, -1, 0);
             }
-            for (uint32_t p = 0; p < a.passes; p++)
-                atomicAdd(&s_hist[p * radix + (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1))], 1u);
+#pragma unroll
+            for (int u = 0; u < kEncodeUnroll; u++) {
+                const uint32_t j = j0 + u * kBlock + threadIdx.x;
+                if (j < d.z) {
+                    uint64_t key;
+                    uint32_t val;
+                    encode_one(a, d.x, d.y + j, raw[u], key, val);
+                    emit_record(a, s_hist, radix, d.y + j, key, val);
+                }
+            }
         }
     }
     if (cur != 0xFFFFFFFFu) flush();
@@ -258,31 +295,58 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
 
 // Per (pass, segment): exclusive scan of the digit counts plus the segment's
 // start, the first output position of each digit in that pass.
+// A pass in which every record of a segment has the same digit leaves that
+// segment as it is; passes that move no record in any segment are marked
+// skippable in counters[C_PASS] (one bit per pass that moves records).
 __global__ void __launch_bounds__(kBlock) k_digit_scan(const uint32_t* hist, uint32_t* base, Segs g,
-                                                       uint32_t passes, uint32_t radix) {
+                                                       uint32_t passes, uint32_t radix, uint32_t* counters) {
     __shared__ uint32_t s_wave[kBlock / kWave];
-    const uint32_t seg = blockIdx.x % g.count;
+    const uint32_t seg = blockIdx.x % g.count, pass = blockIdx.x / g.count;
     const uint64_t row = (uint64_t)blockIdx.x * radix;  // blockIdx.x = pass * count + seg
     const uint32_t per = radix / kBlock;
     uint32_t v[1] = {0}, e[1];
     for (uint32_t k = 0; k < per; k++) v[0] += hist[row + threadIdx.x * per + k];
-    tile_scan<1>(v, e, s_wave);
+    const uint32_t len = tile_scan<1>(v, e, s_wave);
     uint32_t acc = (g.single ? 0u : g.off[seg]) + e[0];
+    bool whole = false;  // one digit holds the whole segment
     for (uint32_t k = 0; k < per; k++) {
         const uint64_t d = row + threadIdx.x * per + k;
         base[d] = acc;
         acc += hist[d];
+        whole |= hist[d] == len;
     }
+    const int stays = __syncthreads_or(whole ? 1 : 0);
+    if (threadIdx.x == 0 && len > 0 && !stays) atomicOr(&counters[C_PASS], 1u << pass);
+}
+
+// Source buffer of every radix pass and the buffer that ends up sorted:
+// skipped passes do not swap the buffers.
+__global__ void k_pass_plan(uint32_t passes, uint32_t* counters) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const uint32_t moves = counters[C_PASS];
+    uint32_t cur = 0, src = 0;
+    for (uint32_t p = 0; p < passes; p++)
+        if ((moves >> p) & 1) {
+            src |= cur << p;
+            cur ^= 1;
+        }
+    counters[C_PSRC] = src;
+    counters[C_FINAL] = cur;
 }
 
 // ---------------------------------------------------------------------------
 // onesweep LSD pass
 // ---------------------------------------------------------------------------
+// the two record buffers of the ping-pong (keys, and attributes when not packed)
+struct RecordBufs {
+    uint64_t* keys[2];
+    uint32_t* vals[2];
+};
+
 struct SortArgs {
-    const uint64_t* keys_in;
-    const uint32_t* vals_in;
-    uint64_t* keys_out;
-    uint32_t* vals_out;
+    RecordBufs b;
+    uint32_t pass;
+    const uint32_t* plan;  // counters: C_PASS, C_PSRC
     Segs g;
     uint32_t shift;
     const uint32_t* digit_base;  // [segments][radix] first position of each digit in each segment, this pass
@@ -291,6 +355,9 @@ struct SortArgs {
     uint32_t* tile_ctr;
     uint32_t* err;
 };
+
+// predecessor status words a sort tile's digit thread loads per round
+constexpr int kPoll = 16;
 
 // PACKED: one u64 word per record, key << 32 | attribute (keys of at most 32
 // bits); the digit is taken at a.shift >= 32 and there is no value stream.
@@ -310,6 +377,12 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_vals[PACKED ? 1 : kSortTile];
 
+    if (!((a.plan[C_PASS] >> a.pass) & 1)) return;  // no segment has two digits: nothing moves
+    const uint32_t src = (a.plan[C_PSRC] >> a.pass) & 1;
+    const uint64_t* keys_in = a.b.keys[src];
+    const uint32_t* vals_in = a.b.vals[src];
+    uint64_t* keys_out = a.b.keys[src ^ 1];
+    uint32_t* vals_out = a.b.vals[src ^ 1];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
     if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
     for (uint32_t i = tid; i < W * R; i += kBlock) (&s_wave_cnt[0][0])[i] = 0;
@@ -328,8 +401,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     for (int i = 0; i < kSortItems; i++) {
         const uint32_t li = wbase + i * kWave + lane;
         bool ok = li < count;
-        k[i] = ok ? a.keys_in[td.y + li] : 0;
-        if constexpr (!PACKED) v[i] = ok ? a.vals_in[td.y + li] : 0;
+        k[i] = ok ? keys_in[td.y + li] : 0;
+        if constexpr (!PACKED) v[i] = ok ? vals_in[td.y + li] : 0;
     }
     // stable ranking inside the wave: lanes with equal digits found by RB
     // ballots; the lowest peer bumps the wave's counter for that digit
@@ -405,21 +478,21 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         } else {
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            // predecessors are polled four at a time (independent loads in
+            // predecessors are polled kPoll at a time (independent loads in
             // flight) and summed newest to oldest until an inclusive prefix
             int64_t t = (int64_t)tile - 1;
             Wait wt;
             while (t >= first) {
-                uint64_t w[4];
+                uint64_t w[kPoll];
 #pragma unroll
-                for (int q = 0; q < 4; q++)
+                for (int q = 0; q < kPoll; q++)
                     w[q] = t - q >= first ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
                                                               __HIP_MEMORY_SCOPE_AGENT)
                                           : status_word(kFlagPre, a.epoch, 0);
                 int used = 0;
                 bool done = false, wait = false;
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
+                for (int q = 0; q < kPoll; q++) {
                     if (done || wait) continue;
                     const uint32_t f = status_flag(w[q], a.epoch);
                     if (f == 0) {
@@ -460,8 +533,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             atomicOr(a.err, 16u);
             continue;
         }
-        a.keys_out[dst] = key;
-        if constexpr (!PACKED) a.vals_out[dst] = s_vals[j];
+        keys_out[dst] = key;
+        if constexpr (!PACKED) vals_out[dst] = s_vals[j];
     }
 }
 
@@ -504,11 +577,16 @@ __device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
 // chunk index back above the local bits. Scan tiles are the sort tiles cut
 // into `subs` pieces of kScanTile records (never across a segment),
 // compacted in tile order.
-__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t subs, const uint64_t* keys,
-                                                    const uint32_t* vals, uint32_t lbits, uint64_t* ukey,
-                                                    uint32_t* uval, bool packed) {
+__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t subs, RecordBufs b,
+                                                    uint32_t lbits, bool packed) {
     __shared__ uint32_t s_tile, s_wave[4];
     __shared__ uint64_t s_b;
+    // sorted records in buffer `fin`; the unique voxels go to the other one
+    const uint32_t fin = sc.counters[C_FINAL];
+    const uint64_t* keys = b.keys[fin];
+    const uint32_t* vals = b.vals[fin];
+    uint64_t* ukey = b.keys[fin ^ 1];
+    uint32_t* uval = packed ? b.vals[0] : b.vals[fin ^ 1];
     const uint64_t invalid = 1ull << lbits;
     const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
     const uint32_t ntiles = tile_count(g) * subs;
@@ -524,26 +602,45 @@ __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t 
         uint32_t f[kScanItems], e[kScanItems];
         uint64_t k[kScanItems];
         uint32_t v[kScanItems];
+        // all loads first (no dependent loads): the sort key of each record
+        // (packed: key << 32 | attribute)
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            const uint32_t li = wave * kWave * kScanItems + i * kWave + lane;
+            k[i] = 0;
+            v[i] = 0;
+            if (li < len) {
+                if (packed) {
+                    const uint64_t w = keys[base + li];
+                    k[i] = w >> 32;
+                    v[i] = (uint32_t)w;
+                } else {
+                    k[i] = keys[base + li];
+                    v[i] = vals[base + li];
+                }
+            }
+        }
+        // the next record's key comes from the next lane (lane 63: lane 0 of
+        // the next item); only the piece's last element and the wave's last
+        // element read it from memory
 #pragma unroll
         for (int i = 0; i < kScanItems; i++) {
             const uint32_t li = wave * kWave * kScanItems + i * kWave + lane;
             const uint32_t idx = base + li;
+            const uint64_t from_lane = __shfl_down(k[i], 1, kWave);
+            const uint64_t from_next = __shfl(k[i + 1 < kScanItems ? i + 1 : i], 0, kWave);
             f[i] = 0;
-            k[i] = 0;
-            v[i] = 0;
             if (li < len) {
-                if (packed) {  // key << 32 | attribute
-                    const uint64_t w = keys[idx];
-                    k[i] = w >> 32;
-                    v[i] = (uint32_t)w;
-                    const bool last = idx + 1 == end || (keys[idx + 1] >> 32) != k[i];
-                    f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
-                } else {
-                    k[i] = keys[idx];
-                    v[i] = vals[idx];
-                    const bool last = idx + 1 == end || keys[idx + 1] != k[i];
-                    f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+                bool last = idx + 1 == end;
+                if (!last) {
+                    uint64_t nk;
+                    if (li + 1 == len || (lane == kWave - 1 && i == kScanItems - 1))
+                        nk = packed ? keys[idx + 1] >> 32 : keys[idx + 1];
+                    else
+                        nk = lane < kWave - 1 ? from_lane : from_next;
+                    last = nk != k[i];
                 }
+                f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
             }
         }
         uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
@@ -566,11 +663,12 @@ __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t 
 // bricks (gbuild.zig:394-428): head of each run of equal key >> 6; the run's
 // mask is OR-reduced inside each 64-element round of a wave and merged into
 // masks[] with one 64-bit atomicOr per (brick, round)
-__global__ void __launch_bounds__(kBlock) k_bricks(ScanCtx sc, const uint64_t* ukey, uint32_t lbits,
+__global__ void __launch_bounds__(kBlock) k_bricks(ScanCtx sc, RecordBufs b, uint32_t lbits,
                                                     uint64_t* brick_key, uint64_t* masks, uint32_t* first_voxel,
                                                     uint64_t brick_cap, uint32_t n_cap) {
     __shared__ uint32_t s_tile, s_wave[4];
     __shared__ uint64_t s_b;
+    const uint64_t* ukey = b.keys[sc.counters[C_FINAL] ^ 1];  // the unique voxels
     const uint32_t n = min(sc.counters[C_U], n_cap);
     const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
     const uint32_t lane = lane_id();
@@ -581,12 +679,23 @@ __global__ void __launch_bounds__(kBlock) k_bricks(ScanCtx sc, const uint64_t* u
         uint64_t k[kScanItems];
 #pragma unroll
         for (int i = 0; i < kScanItems; i++) {
-            uint64_t idx = scan_index(tile, i);
+            const uint64_t idx = scan_index(tile, i);
+            k[i] = idx < n ? ukey[idx] : ~0ull;
+        }
+        // the previous key comes from the previous lane (lane 0: lane 63 of
+        // the previous item); the wave's first element reads it from memory
+        const uint64_t wave_prev = [&]() {
+            const uint64_t idx0 = scan_index(tile, 0);
+            return idx0 > 0 && idx0 <= n ? ukey[idx0 - 1] : ~0ull;
+        }();
+#pragma unroll
+        for (int i = 0; i < kScanItems; i++) {
+            const uint64_t idx = scan_index(tile, i);
+            const uint64_t from_lane = __shfl_up(k[i], 1, kWave);
+            const uint64_t from_prev = __shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave);
+            const uint64_t prev = lane > 0 ? from_lane : (i > 0 ? from_prev : wave_prev);
             f[i] = 0;
-            k[i] = ~0ull;
             if (idx < n) {
-                k[i] = ukey[idx];
-                uint64_t prev = idx ? ukey[idx - 1] : ~0ull;
                 f[i] = idx == 0 || (prev >> 6) != (k[i] >> 6);
                 if (first_voxel && (idx == 0 || (prev >> lbits) != (k[i] >> lbits)))
                     first_voxel[(uint32_t)(k[i] >> lbits)] = (uint32_t)idx;
@@ -992,32 +1101,6 @@ __global__ void k_fill_u64(uint64_t* p, uint64_t v, uint64_t n) {
 // ---------------------------------------------------------------------------
 // host orchestration
 // ---------------------------------------------------------------------------
-bool pool_ready(int device) {
-    static std::atomic<bool> done[64] = {};  // several server workers allocate at once
-    if (device < 0 || device >= 64) return false;
-    if (done[device].load(std::memory_order_acquire)) return true;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, device) != hipSuccess) {
-        hipGetLastError();
-        return false;
-    }
-    uint64_t keep = ~0ull;
-    (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
-    done[device].store(true, std::memory_order_release);
-    return true;
-}
-
-hipError_t pool_alloc(void** p, uint64_t bytes, hipStream_t s) {
-    int dev = 0;
-    (void)hipGetDevice(&dev);
-    pool_ready(dev);
-    return hipMallocAsync(p, bytes, s);
-}
-
-void pool_free(void* p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
-}
-
 namespace {
 
 inline uint32_t grid_for(uint64_t work, uint32_t per_block, uint32_t cap) {
@@ -1144,6 +1227,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     for (uint32_t lv = 2; lv < log2_size; lv++) word_bound += std::min(nb, cells(lv + 1)) * kNodeStride;
     const uint64_t cap = hash_cap_for(nb);
 
+    const auto host_t0 = std::chrono::steady_clock::now();
     Alloc A{{}, s};
     uint32_t* counters = A.get<uint32_t>(C_COUNT);
     const uint32_t segs = enc.chunk_count ? enc.chunk_count : 1;
@@ -1216,6 +1300,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         return fail(PYR_ERROR_OUT_OF_MEMORY, "device memory for a build of %u voxels", n);
     }
 
+    const auto host_t1 = std::chrono::steady_clock::now();  // allocations done
     Timer tm;
     tm.s = s;
     tm.mark();
@@ -1249,29 +1334,30 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         hipLaunchKernelGGL(k_encode, dim3(eblocks), dim3(kBlock), passes * radix * sizeof(uint32_t), s, enc, seg,
                            per_block);
         hipLaunchKernelGGL(k_digit_scan, dim3(passes * segs), dim3(kBlock), 0, s, hist, digit_base, seg, passes,
-                           radix);
+                           radix, counters);
+        hipLaunchKernelGGL(k_pass_plan, dim3(1), dim3(kWave), 0, s, passes, counters);
     }
+    const RecordBufs bufs{{kb[0], kb[1]}, {vb[0], vb[1]}};
     tm.mark();
     // 2. radix passes
-    uint32_t cur = 0;
+    // (the device skips passes that move nothing and picks each pass's
+    // source buffer, so the buffers are named by index here)
     for (uint32_t p = 0; p < passes && n; p++) {
-        SortArgs sa{kb[cur], vb[cur], kb[cur ^ 1], vb[cur ^ 1], seg, (packed ? 32 : 0) + p * rb,
-                    digit_base + (uint64_t)p * segs * radix, status, ++epoch, tile_ctr(), counters + C_ERR};
+        SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + p * rb, digit_base + (uint64_t)p * segs * radix,
+                    status, ++epoch, tile_ctr(), counters + C_ERR};
         launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
-        cur ^= 1;
     }
     tm.mark();
-    // 3. unique into the other buffer pair, which becomes Built.keys/attrs
-    uint64_t* ukey = kb[cur ^ 1];
-    uint32_t* uval = packed ? vb[0] : vb[cur ^ 1];
+    // 3. unique into the buffer pair that does not hold the sorted records;
+    // it becomes Built.keys/attrs
     const uint32_t scan_grid = grid_for(nmax, kScanTile, 2048);
     if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         const uint32_t subs = (sort_tile + kScanTile - 1) / kScanTile;
         hipLaunchKernelGGL(k_unique, dim3((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048)), dim3(kBlock), 0, s,
-                           sc, seg, subs, kb[cur], vb[cur], lbits, ukey, uval, packed);
+                           sc, seg, subs, bufs, lbits, packed);
         ScanCtx sb{status, ++epoch, tile_ctr(), counters};
-        hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, ukey, lbits, brick_key, masks,
+        hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, bufs, lbits, brick_key, masks,
                            first_dev, nb, nmax);
     }
     tm.mark();
@@ -1333,7 +1419,9 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
             e = hipMemcpyAsync(first_host, first_dev, enc.chunk_count * 4, hipMemcpyDeviceToHost, s);
     }
     A.release();
+    const auto host_t2 = std::chrono::steady_clock::now();  // everything enqueued
     if (e == hipSuccess) e = hipStreamSynchronize(s);
+    const auto host_t3 = std::chrono::steady_clock::now();
     if (e == hipSuccess) e = hipGetLastError();
     if (g_timing && e == hipSuccess) {
         std::lock_guard<std::mutex> lk(g_times_mutex);
@@ -1343,10 +1431,14 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         g_times.leaves_ms = tm.between(3, 4);
         g_times.levels_ms = tm.between(4, 5);
         g_times.total_ms = tm.between(0, 5);
-        g_times.sort_passes = passes;
+        g_times.sort_passes = (uint32_t)__builtin_popcount(c[C_PASS]);  // passes that moved records
         g_times.radix_bits = rb;
         g_times.record_bytes = packed ? 8 : 12;
         g_times.sort_items = sort_items;
+        using ms = std::chrono::duration<float, std::milli>;
+        g_times.host_alloc_ms = ms(host_t1 - host_t0).count();
+        g_times.host_enqueue_ms = ms(host_t2 - host_t1).count();
+        g_times.host_wait_ms = ms(host_t3 - host_t2).count();
     }
     if (e != hipSuccess || c[C_ERR] != 0) {
         free_outputs();
@@ -1359,12 +1451,12 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     out->node_words = c[C_WORDS];
     out->leaves = (uint64_t)(uintptr_t)leaves_out;
     out->leaf_count = c[C_LEAVES];
-    out->keys = (uint64_t)(uintptr_t)ukey;
-    out->attrs = (uint64_t)(uintptr_t)uval;
+    const uint32_t fin = c[C_FINAL] & 1;  // buffer pair of the sorted records
+    out->keys = (uint64_t)(uintptr_t)kb[fin ^ 1];
+    out->attrs = (uint64_t)(uintptr_t)(packed ? vb[0] : vb[fin ^ 1]);
     out->voxel_count = c[C_U];
-    // the buffer pair not handed out held the sorted records
-    pool_free(kb[cur], s);
-    if (!packed) pool_free(vb[cur], s);
+    pool_free(kb[fin], s);
+    if (!packed) pool_free(vb[fin], s);
     return PYR_OK;
 }
 
@@ -1450,6 +1542,7 @@ PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, 
 }
 
 PYR_API void pyrhip_set_timing(int enable) { g_timing = enable != 0; }
+
 
 PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out) {
     if (!out) return;

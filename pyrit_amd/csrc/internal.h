@@ -24,12 +24,21 @@ inline PyrResult fail(PyrResult code, const char* fmt, ...) {
     return code;
 }
 
-// Device memory of builds and geometries comes from the device's stream-ordered
-// pool, configured once to keep freed memory (release threshold = max): a build
-// allocates tens of GB of outputs and scratch, and recycling them from the
-// pool replaces page mapping and unmapping on every call.
-bool pool_ready(int device);
+// Device memory of builds, build outputs and resident server generations
+// comes from the library's stream-ordered cache (devcache.cpp): freed blocks
+// stay mapped and serve later requests of their size class, reuse on another
+// stream waits for the freeing stream on the device.
 hipError_t pool_alloc(void** p, uint64_t bytes, hipStream_t s);
 void pool_free(void* p, hipStream_t s);
+
+// Context internals used by streamed worlds (api.cpp): a chunk batch is
+// installed into the geometry pools once and every chunk gets a geometry
+// that refers to the batch's shared ranges.
+hipStream_t ctx_stream(PyrContext* ctx);
+PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* built, int32_t* batch_id);
+void ctx_release_batch(PyrContext* ctx, int32_t batch_id);
+uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t batch_id);
+PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t batch_id, uint32_t root, uint32_t first_attr,
+                                  uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out);
 
 }  // namespace pyr

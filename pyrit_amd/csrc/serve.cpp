@@ -286,7 +286,8 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
             ok &= hipMemcpyAsync(g->attrs, (void*)(uintptr_t)b.attrs, ab, hipMemcpyDeviceToDevice, s) == hipSuccess;
         }
         g->bytes = nb + lb + kb + ab;
-        pyrhip_built_free(&b, s);
+        const uint32_t batch_voxels = b.voxel_count;
+        pyrhip_built_free(&b, s);  // zeroes b
         if (ok && hipStreamSynchronize(s) != hipSuccess) ok = false;
         if (!ok) {
             hipGetLastError();
@@ -300,7 +301,7 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         // per-chunk voxel counts: ranks are contiguous per chunk in batch order
         counts.assign(k, 0);
         if (r == PYR_OK) {
-            uint32_t next = b.voxel_count;
+            uint32_t next = batch_voxels;
             for (int64_t i = (int64_t)k - 1; i >= 0; i--) {
                 if (first[i] == 0xFFFFFFFFu) continue;
                 counts[i] = next - first[i];

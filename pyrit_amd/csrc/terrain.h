@@ -217,3 +217,56 @@ PYRGEN_HD uint32_t pyrgen_column(const PyrTerrainChunk* c, const PyrTerrainCave*
     }
     return emitted;
 }
+
+// ---------------------------------------------------------------------------
+// World chunks: the same heightfield sampled per level of detail. A chunk of
+// level `lod` has cells of 2^lod base voxels; a cell is solid when its lowest
+// base voxel is below the surface, and only the skin is emitted: in every
+// column the cells from one below the lowest neighbouring surface up to the
+// column's own surface. Materials by depth band in cells.
+// ---------------------------------------------------------------------------
+typedef struct PyrTerrainWorld {
+    uint32_t seed;
+    int32_t mean;             // mean surface height (base voxels)
+    int32_t amplitude;
+    uint32_t octaves;
+    uint32_t wavelength_log2;
+    uint32_t reserved;
+} PyrTerrainWorld;
+
+// surface of cell column (X, Z) of level lod, in cells: cells y < result are solid
+PYRGEN_HD int64_t pyrgen_world_cell_height(const PyrTerrainWorld* w, int64_t X, int64_t Z, uint32_t lod) {
+    const int32_t bx = (int32_t)(X << lod), bz = (int32_t)(Z << lod);
+    const int64_t f = pyrgen_fbm(bx, bz, w->octaves, w->wavelength_log2, w->seed);
+    const int64_t h = (int64_t)w->mean + (((int64_t)w->amplitude * f) >> 16);
+    return (h + ((int64_t)1 << lod) - 1) >> lod;  // ceil(h / 2^lod)
+}
+
+// Emits the skin of local column (lx, lz) of chunk (kx, ky, kz, lod) with
+// edge 2^chunk_log2 through emit(x, y, z, attribute); returns the count.
+template <class Emit>
+PYRGEN_HD uint32_t pyrgen_world_column(const PyrTerrainWorld* w, int32_t kx, int32_t ky, int32_t kz, uint32_t lod,
+                                       uint32_t chunk_log2, uint32_t lx, uint32_t lz, Emit emit) {
+    const int64_t n = (int64_t)1 << chunk_log2;
+    const int64_t X = (int64_t)kx * n + lx, Z = (int64_t)kz * n + lz, Y0 = (int64_t)ky * n;
+    const int64_t h = pyrgen_world_cell_height(w, X, Z, lod);
+    int64_t m = pyrgen_world_cell_height(w, X - 1, Z, lod);
+    int64_t t = pyrgen_world_cell_height(w, X + 1, Z, lod);
+    m = t < m ? t : m;
+    t = pyrgen_world_cell_height(w, X, Z - 1, lod);
+    m = t < m ? t : m;
+    t = pyrgen_world_cell_height(w, X, Z + 1, lod);
+    m = t < m ? t : m;
+    int64_t lo = m - 1;
+    if (lo > h - 1) lo = h - 1;
+    int64_t ya = lo - Y0, yb = h - Y0;
+    if (ya < 0) ya = 0;
+    if (yb > n) yb = n;
+    uint32_t emitted = 0;
+    for (int64_t ly = ya; ly < yb; ly++) {
+        const int64_t depth = h - 1 - (Y0 + ly);
+        emit(lx, (uint32_t)ly, lz, depth < 1 ? PYRGEN_GRASS : depth < 4 ? PYRGEN_DIRT : PYRGEN_STONE);
+        emitted++;
+    }
+    return emitted;
+}

@@ -11,6 +11,7 @@
 #include <stdint.h>
 #include <string.h>
 
+#include "pyrit.h"
 #include "terrain.h"
 
 namespace {
@@ -103,9 +104,71 @@ __global__ void k_chunk_offsets(const uint64_t* col_start, const uint64_t* col_o
     if (c == count) offsets[count] = *total;
 }
 
+// world chunks: one thread per (chunk, column), voxels appended with an
+// atomic counter per chunk into the generator's capacity layout
+struct CapEmit {
+    PyrVoxel* out;
+    uint32_t* count;
+    uint32_t capacity;
+    __device__ void operator()(uint32_t x, uint32_t y, uint32_t z, uint32_t a) const {
+        const uint32_t k = atomicAdd(count, 1u);
+        if (k < capacity) out[k] = PyrVoxel{(int32_t)x, (int32_t)y, (int32_t)z, a};
+    }
+};
+
+__global__ void __launch_bounds__(256) k_world(PyrTerrainWorld w, const PyrChunkKey* keys, uint32_t count,
+                                               uint32_t chunk_log2, uint32_t capacity, PyrVoxel* voxels,
+                                               uint32_t* counts) {
+    const uint64_t cols = (uint64_t)1 << (2 * chunk_log2);
+    const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= cols * count) return;
+    const uint32_t c = (uint32_t)(i / cols), col = (uint32_t)(i % cols);
+    const PyrChunkKey k = keys[c];
+    const uint32_t n = 1u << chunk_log2;
+    CapEmit e{voxels + (uint64_t)c * capacity, &counts[c], capacity};
+    pyrgen_world_column(&w, k.x, k.y, k.z, k.lod, chunk_log2, col % n, col / n, e);
+}
+
+struct HostEmit {
+    PyrVoxel* out;
+    uint32_t capacity;
+    uint32_t* k;
+    void operator()(uint32_t x, uint32_t y, uint32_t z, uint32_t a) const {
+        if (*k < capacity && out) out[*k] = PyrVoxel{(int32_t)x, (int32_t)y, (int32_t)z, a};
+        (*k)++;
+    }
+};
+
 } // namespace
 
 extern "C" {
+
+// PyrWorldGenFn of the streamed-world API for the terrain above; `user`
+// points to a PyrTerrainWorld in host memory.
+__attribute__((visibility("default"))) void pyrgen_world_generate(void* user, const PyrWorldGenParams* p,
+                                                                  void* stream) {
+    const PyrTerrainWorld w = *(const PyrTerrainWorld*)user;
+    const uint64_t threads = ((uint64_t)1 << (2 * p->chunk_log2)) * p->count;
+    if (threads == 0) return;
+    hipLaunchKernelGGL(k_world, dim3((uint32_t)((threads + 255) / 256)), dim3(256), 0, (hipStream_t)stream, w,
+                       (const PyrChunkKey*)(uintptr_t)p->chunks, p->count, p->chunk_log2, p->capacity,
+                       (PyrVoxel*)(uintptr_t)p->voxels, (uint32_t*)(uintptr_t)p->counts);
+}
+
+// The same chunk on the host (columns z-major, x fastest, cells bottom-up);
+// writes at most `capacity` voxels and returns how many the chunk has.
+__attribute__((visibility("default"))) uint32_t pyrgen_world_chunk_host(const PyrTerrainWorld* w,
+                                                                        const PyrChunkKey* key,
+                                                                        uint32_t chunk_log2, PyrVoxel* out,
+                                                                        uint32_t capacity) {
+    const uint32_t n = 1u << chunk_log2;
+    uint32_t k = 0;
+    HostEmit e{out, capacity, &k};
+    for (uint32_t lz = 0; lz < n; lz++)
+        for (uint32_t lx = 0; lx < n; lx++)
+            pyrgen_world_column(w, key->x, key->y, key->z, key->lod, chunk_log2, lx, lz, e);
+    return k;
+}
 
 // Host generation. col_offsets[count+1] = exclusive prefix of 4^log2 columns.
 // voxels == NULL: only offsets[count+1] (voxel prefix per chunk) is filled.

@@ -1,10 +1,12 @@
-"""Summarise rocprofv3 CSV output into the short text files kept under
-profiles/.
+"""Summarise rocprofv3 output into the short text files kept under profiles/.
 
-  python -m pyrit_amd.profsum stats <kernel_stats.csv> --out profiles/X.md --cmd "..."
-  python -m pyrit_amd.profsum traffic --fetch <fetch_counter_collection.csv>
-         --write <write_counter_collection.csv> --voxels N --probe-bytes B
-         --out profiles/traffic.json --cmd "..." --cmd "..."
+  python -m pyrit_amd.profsum stats <kernel_stats.csv | results.db> --out profiles/X.md --cmd "..."
+  python -m pyrit_amd.profsum traffic --fetch <counter csv | db> --write <counter csv | db>
+         --voxels N --probe-bytes B --out profiles/traffic.json --cmd "..." --cmd "..."
+
+Inputs are either rocprofv3's CSV files (`--output-format csv`) or its rocpd
+SQLite database (`*_results.db`, the default output format), read with the
+standard library's sqlite3.
 
 `traffic` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the same
 counters read for k_copy_probe, a plain copy kernel that reads and writes
@@ -17,27 +19,54 @@ import collections
 import csv
 import json
 import re
+import sqlite3
 
 
 def short_name(name: str) -> str:
-    n = name.split("(")[0].replace("void ", "").replace("pyr::", "").strip()
+    n = name.replace("(anonymous namespace)::", "").split("(")[0].replace("void ", "").replace("pyr::", "").strip()
     return re.sub(r"\s+", "", n)
 
 
+def _kernel_rows(path: str) -> list[dict]:
+    """[{name, calls, total_ns, avg_ns, min_ns, max_ns, vgpr, lds}] sorted by total time"""
+    if path.endswith(".db"):
+        db = sqlite3.connect(path)
+        q = ("select name, count(*), sum(duration), avg(duration), min(duration), max(duration), max(vgpr_count), "
+             "max(lds_size) from kernels group by name order by 3 desc")
+        return [{"name": r[0], "calls": r[1], "total_ns": float(r[2]), "avg_ns": float(r[3]), "min_ns": float(r[4]),
+                 "max_ns": float(r[5]), "vgpr": r[6], "lds": r[7]} for r in db.execute(q)]
+    rows = []
+    for r in csv.DictReader(open(path)):
+        rows.append({"name": r["Name"], "calls": int(r["Calls"]), "total_ns": float(r["TotalDurationNs"]),
+                     "avg_ns": float(r["AverageNs"]), "min_ns": float(r["MinNs"]), "max_ns": float(r["MaxNs"]),
+                     "vgpr": None, "lds": None})
+    return rows
+
+
 def stats(path: str, out: str, cmds: list[str], top: int = 25) -> None:
-    rows = list(csv.DictReader(open(path)))
+    rows = _kernel_rows(path)
+    total = sum(r["total_ns"] for r in rows) or 1.0
     lines = ["# rocprofv3 kernel statistics", "", "Commands:", ""]
     lines += [f"    {c}" for c in cmds]
-    lines += ["", "| kernel | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|"]
+    lines += ["", "| kernel | calls | total ms | avg us | min us | max us | % | VGPRs | LDS B |",
+              "|---|---|---|---|---|---|---|---|---|"]
     for r in rows[:top]:
-        lines.append(f"| {short_name(r['Name']) or '(unnamed)'} | {r['Calls']} | "
-                     f"{float(r['TotalDurationNs']) / 1e6:.3f} | {float(r['AverageNs']) / 1e3:.1f} | "
-                     f"{float(r['MinNs']) / 1e3:.1f} | {float(r['MaxNs']) / 1e3:.1f} | {float(r['Percentage']):.2f} |")
+        lines.append(f"| {short_name(r['name']) or '(unnamed)'} | {r['calls']} | {r['total_ns'] / 1e6:.3f} | "
+                     f"{r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} | "
+                     f"{100.0 * r['total_ns'] / total:.2f} | {r['vgpr'] if r['vgpr'] is not None else ''} | "
+                     f"{r['lds'] if r['lds'] is not None else ''} |")
     open(out, "w").write("\n".join(lines) + "\n")
 
 
 def _counter(path: str, name: str) -> dict:
+    """kernel short name -> [counter value per dispatch]"""
     agg = collections.OrderedDict()
+    if path.endswith(".db"):
+        db = sqlite3.connect(path)
+        q = "select kernel_name, value from counters_collection where counter_name = ? order by dispatch_id"
+        for k, v in db.execute(q, (name,)):
+            agg.setdefault(short_name(k), []).append(float(v))
+        return agg
     for r in csv.DictReader(open(path)):
         if r["Counter_Name"] != name:
             continue
@@ -71,7 +100,7 @@ def main() -> None:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
     s = sub.add_parser("stats")
-    s.add_argument("csv")
+    s.add_argument("input")
     s.add_argument("--out", required=True)
     s.add_argument("--cmd", action="append", default=[])
     t = sub.add_parser("traffic")
@@ -83,7 +112,7 @@ def main() -> None:
     t.add_argument("--cmd", action="append", default=[])
     a = ap.parse_args()
     if a.what == "stats":
-        stats(a.csv, a.out, a.cmd)
+        stats(a.input, a.out, a.cmd)
     else:
         traffic(a.fetch, a.write, a.voxels, a.probe_bytes, a.out, a.cmd)
 

@@ -59,6 +59,9 @@ LAYOUT_C = r"""
 int main(void) {
   S(PyrRay) S(PyrHit) S(PyrCamera) S(PyrGeometryData) S(PyrInstanceData) S(PyrScene) S(PyrCreateInfo)
   S(PyrVoxel) S(PyrFrameInfo) S(PyrTargets) S(PyrhipBuilt) S(PyrhipChunkBatch) S(PyrhipDag)
+  S(PyrChunkKey) S(PyrWorldGenParams) S(PyrWorldInfo) S(PyrWorldStats) S(PyrWorldEdit)
+  O(PyrWorldInfo, memory_budget) O(PyrWorldInfo, y_min) O(PyrWorldInfo, flags) O(PyrWorldInfo, generate)
+  O(PyrWorldInfo, user) O(PyrWorldStats, voxel_pixels) O(PyrWorldStats, bytes) O(PyrWorldGenParams, user)
   O(PyrInstanceData, bounds_min) O(PyrInstanceData, geometry) O(PyrInstanceData, bounds_max)
   O(PyrInstanceData, mask) O(PyrInstanceData, flags)
   O(PyrScene, instance_count) O(PyrScene, frame) O(PyrScene, origin) O(PyrScene, materials)
@@ -74,7 +77,13 @@ int main(void) {
 # sizes the reference's ABI fixes (reference include/pyrit.h, checked there by
 # tests/abi_test.zig); computed from its field lists on LP64
 REFERENCE_SIZES = {"PyrRay": 32, "PyrHit": 16, "PyrCamera": 92, "PyrGeometryData": 32, "PyrInstanceData": 144,
-                   "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80}
+                   "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80,
+                   "PyrChunkKey": 16, "PyrWorldGenParams": 48, "PyrWorldInfo": 88, "PyrWorldStats": 48,
+                   "PyrWorldEdit": 32}
+# field offsets the reference's world ABI fixes (reference include/pyrit.h:815-875)
+REFERENCE_OFFSETS = {"PyrWorldInfo.memory_budget": 24, "PyrWorldInfo.y_min": 32, "PyrWorldInfo.flags": 68,
+                     "PyrWorldInfo.generate": 72, "PyrWorldInfo.user": 80, "PyrWorldStats.voxel_pixels": 24,
+                     "PyrWorldStats.bytes": 40, "PyrWorldGenParams.user": 40}
 
 
 def test_struct_layouts_match_reference_and_mirrors():
@@ -92,10 +101,14 @@ def test_struct_layouts_match_reference_and_mirrors():
         vals[k] = int(v)
     for name, size in REFERENCE_SIZES.items():
         assert vals[name] == size, (name, vals[name], size)
+    for name, off in REFERENCE_OFFSETS.items():
+        assert vals[name] == off, (name, vals[name], off)
     mirrors = {"PyrCamera": abi.PyrCamera, "PyrGeometryData": abi.PyrGeometryData,
                "PyrInstanceData": abi.PyrInstanceData, "PyrScene": abi.PyrScene, "PyrCreateInfo": abi.PyrCreateInfo,
                "PyrFrameInfo": abi.PyrFrameInfo, "PyrTargets": abi.PyrTargets, "PyrhipBuilt": abi.PyrhipBuilt,
-               "PyrhipChunkBatch": abi.PyrhipChunkBatch, "PyrhipDag": abi.PyrhipDag}
+               "PyrhipChunkBatch": abi.PyrhipChunkBatch, "PyrhipDag": abi.PyrhipDag, "PyrChunkKey": abi.PyrChunkKey,
+               "PyrWorldGenParams": abi.PyrWorldGenParams, "PyrWorldInfo": abi.PyrWorldInfo,
+               "PyrWorldStats": abi.PyrWorldStats, "PyrWorldEdit": abi.PyrWorldEdit}
     for name, cls in mirrors.items():
         assert C.sizeof(cls) == vals[name], name
     for key, v in vals.items():

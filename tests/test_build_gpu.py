@@ -53,6 +53,37 @@ def test_random_points_bit_exact(hip_device, log2, n, seed):
     assert O.canon(b.nodes(), b.leaves(), b.root, log2).same(cpu)
 
 
+@pytest.mark.parametrize("case", ["low corner", "two corners", "no invalid 256", "with invalid 256"])
+def test_passes_that_move_nothing(hip_device, case):
+    """Radix passes whose digit is the same for every record are skipped on
+    the device; the result must not depend on which passes ran."""
+    print(f"case: skipped passes, {case}", flush=True)
+    rng = np.random.default_rng(11)
+    if case == "low corner":  # 1024^3 grid, coordinates < 8: only the lowest digits vary
+        log2, pts = 10, random_points(4000, 3, 12, dup_frac=0.3, delete_frac=0.1)
+    elif case == "two corners":  # middle digits constant, the top one varies
+        log2 = 10
+        a = random_points(3000, 2, 13, dup_frac=0.3, delete_frac=0.1)
+        b = random_points(3000, 2, 14, dup_frac=0.3, delete_frac=0.1)
+        for c in ("x", "y", "z"):
+            b[c] += 512
+        pts = np.concatenate([a, b])
+        pts = pts[rng.permutation(len(pts))]
+    elif case == "no invalid 256":  # 25-bit keys: the top pass only holds the invalid bit
+        log2, pts = 8, random_points(100000, 8, 15, dup_frac=0.2, delete_frac=0.05)
+    else:
+        log2, pts = 8, random_points(100000, 8, 16, dup_frac=0.2, delete_frac=0.05, out_frac=0.01)
+    ref = O.gb_build(log2, pts)
+    b = _gpu().build(log2, pts)
+    assert_same_build(b, ref, log2)
+    # a batch whose chunks all skip the same passes
+    offs = np.array([0, len(pts) // 3, len(pts)], np.uint32)
+    rb_, rr, rf = O.gb_build_chunks(log2, pts, offs)
+    bb, roots, first = _gpu().build_chunks(log2, pts, offs)
+    assert_same_build(bb, rb_, log2)
+    assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+
+
 @pytest.mark.parametrize("radix_bits", [8, 9, 10, 11])
 def test_radix_widths_identical(hip_device, radix_bits):
     print(f"case: radix bits {radix_bits}", flush=True)
