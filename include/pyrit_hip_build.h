@@ -67,7 +67,7 @@ typedef struct PyrhipBuildTuning {
     uint32_t sort_items;   /* keys per thread of a sort tile (256 threads): 8, 12, 16, 20 (default), 24 */
     uint32_t pair_layout;  /* 0: pack key << 32 | attribute when keys fit 32 bits (8-bit digits);
                               1: always separate key and attribute arrays */
-    uint32_t reserved;
+    uint32_t scan_items;   /* records per thread of a unique / brick scan tile: 8 (default), 12, 16, 20 */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique

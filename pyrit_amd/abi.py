@@ -78,7 +78,7 @@ class PyrhipChunkBatch(C.Structure):
 
 
 class PyrhipBuildTuning(C.Structure):
-    _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("reserved", u32)]
+    _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("scan_items", u32)]
 
 
 class PyrhipBuildTimes(C.Structure):

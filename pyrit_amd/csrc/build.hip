@@ -356,7 +356,9 @@ struct SortArgs {
     uint32_t* err;
 };
 
-// predecessor status words a sort tile's digit thread loads per round
+// predecessor status words a sort tile's digit thread loads in the first
+// look-back round and in every later one
+constexpr int kPollFirst = 16;
 constexpr int kPoll = 16;
 
 // PACKED: one u64 word per record, key << 32 | attribute (keys of at most 32
@@ -478,22 +480,26 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         } else {
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            // predecessors are polled kPoll at a time (independent loads in
-            // flight) and summed newest to oldest until an inclusive prefix
+            // predecessors are polled newest to oldest and summed until an
+            // inclusive prefix: kPollFirst in the first round (the previous
+            // tile's prefix is often there already), kPoll (independent loads
+            // in flight) in every later one
             int64_t t = (int64_t)tile - 1;
             Wait wt;
+            int width = kPollFirst;
             while (t >= first) {
                 uint64_t w[kPoll];
 #pragma unroll
                 for (int q = 0; q < kPoll; q++)
-                    w[q] = t - q >= first ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
-                                                              __HIP_MEMORY_SCOPE_AGENT)
-                                          : status_word(kFlagPre, a.epoch, 0);
+                    w[q] = q < width && t - q >= first
+                               ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT)
+                               : status_word(kFlagPre, a.epoch, 0);  // before the segment: prefix 0
                 int used = 0;
                 bool done = false, wait = false;
 #pragma unroll
                 for (int q = 0; q < kPoll; q++) {
-                    if (done || wait) continue;
+                    if (done || wait || q >= width) continue;
                     const uint32_t f = status_flag(w[q], a.epoch);
                     if (f == 0) {
                         wait = true;
@@ -505,6 +511,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
                 }
                 if (done) break;
                 t -= used;
+                width = kPoll;
                 if (wait && !wait_more(wt, a.err, 2u)) break;
             }
             __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, excl + tot[j]),
@@ -567,47 +574,63 @@ __device__ __forceinline__ uint32_t next_tile(const ScanCtx& sc, uint32_t* s_til
     return *s_tile;
 }
 
+template <int ITEMS = kScanItems>
 __device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
     const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
-    return (uint64_t)tile * kScanTile + (uint64_t)wave * kWave * kScanItems + (uint64_t)i * kWave + lane;
+    return (uint64_t)tile * (kBlock * ITEMS) + (uint64_t)wave * kWave * ITEMS + (uint64_t)i * kWave + lane;
 }
 
-// unique (gbuild.zig:366-388): keep the last record of each key of a segment
-// unless it is invalid or deletes (attribute 0); the kept keys get their
-// chunk index back above the local bits. Scan tiles are the sort tiles cut
-// into `subs` pieces of kScanTile records (never across a segment),
-// compacted in tile order.
-__global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t subs, RecordBufs b,
-                                                    uint32_t lbits, bool packed) {
-    __shared__ uint32_t s_tile, s_wave[4];
-    __shared__ uint64_t s_b;
-    // sorted records in buffer `fin`; the unique voxels go to the other one
-    const uint32_t fin = sc.counters[C_FINAL];
+// Kept records and brick heads of one segment-aligned scan piece in a single
+// pass (gbuild.zig:366-428: uniqueFlags/uniqueScatter, brickHeads/brickBuild).
+//
+// A record is kept when it is the last of its key in its segment, valid and
+// not a delete. A kept record heads its brick when no earlier kept record of
+// the segment has the same brick (key >> 6). Records of one brick are
+// adjacent in the sorted array, so "earlier kept record of the brick" means
+// "kept record earlier in the same brick run": wave ballots give it inside a
+// 64-record row, a carry passes it from row to row and from wave to wave, and
+// for the run that reaches into the piece from before, wave 0 looks back
+// through the preceding records of that run in memory. Two exclusive scans
+// (kept, heads) and two decoupled look-backs, run by waves 0 and 1 at the
+// same time, give every kept record its unique-voxel position and its brick
+// index; bricks' masks are OR-reduced per row and merged with one atomicOr
+// per (brick, row).
+template <int ITEMS>
+__global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_t* status_heads, Segs g, uint32_t subs,
+                                                           RecordBufs b, uint32_t lbits, bool packed,
+                                                           uint64_t* brick_key, uint64_t* masks,
+                                                           uint32_t* first_voxel, uint64_t brick_cap) {
+    constexpr int W = kBlock / kWave;
+    __shared__ uint32_t s_tile, s_wave[W], s_wave2[W];
+    __shared__ uint64_t s_pre[2];
+    __shared__ uint32_t s_any_start[W], s_any_kept[W], s_out[W], s_carry0;
+    const uint64_t invalid = 1ull << lbits;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint32_t fin = su.counters[C_FINAL];
     const uint64_t* keys = b.keys[fin];
     const uint32_t* vals = b.vals[fin];
     uint64_t* ukey = b.keys[fin ^ 1];
     uint32_t* uval = packed ? b.vals[0] : b.vals[fin ^ 1];
-    const uint64_t invalid = 1ull << lbits;
-    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
     const uint32_t ntiles = tile_count(g) * subs;
+    auto key_at = [&](uint32_t idx) -> uint64_t { return packed ? keys[idx] >> 32 : keys[idx]; };
     for (;;) {
-        const uint32_t tile = next_tile(sc, &s_tile);
+        const uint32_t tile = next_tile(su, &s_tile);
         if (tile >= ntiles) break;
         const uint4 td = tile_desc(g, tile / subs);
-        const uint32_t lo = (tile % subs) * kScanTile;
-        const uint32_t len = td.z > lo ? min((uint32_t)kScanTile, td.z - lo) : 0u;
+        const uint32_t lo = (tile % subs) * (kBlock * ITEMS);
+        const uint32_t len = td.z > lo ? min((uint32_t)(kBlock * ITEMS), td.z - lo) : 0u;
         const uint32_t base = td.y + lo;
-        const uint32_t end = seg_end(g, td.x);  // records from `end` on belong to the next chunk
+        const uint32_t seg_start = g.single ? 0u : g.off[td.x];
+        const uint32_t end = seg_end(g, td.x);
         const uint64_t chunk = (uint64_t)td.x << lbits;
-        uint32_t f[kScanItems], e[kScanItems];
-        uint64_t k[kScanItems];
-        uint32_t v[kScanItems];
-        // all loads first (no dependent loads): the sort key of each record
-        // (packed: key << 32 | attribute)
+        const uint32_t w0 = wave * kWave * ITEMS;  // this wave's first element in the piece
+        uint64_t k[ITEMS];
+        uint32_t v[ITEMS], f[ITEMS], e[ITEMS], h[ITEMS], eh[ITEMS];
 #pragma unroll
-        for (int i = 0; i < kScanItems; i++) {
-            const uint32_t li = wave * kWave * kScanItems + i * kWave + lane;
-            k[i] = 0;
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t li = w0 + i * kWave + lane;
+            k[i] = ~0ull;
             v[i] = 0;
             if (li < len) {
                 if (packed) {
@@ -620,115 +643,169 @@ __global__ void __launch_bounds__(kBlock) k_unique(ScanCtx sc, Segs g, uint32_t 
                 }
             }
         }
-        // the next record's key comes from the next lane (lane 63: lane 0 of
-        // the next item); only the piece's last element and the wave's last
-        // element read it from memory
+        // the key before this wave's first element (memory) and before the piece
+        const uint64_t wave_prev = w0 < len && base + w0 > seg_start ? key_at(base + w0 - 1) : ~0ull;
+        // kept flags (next key from the next lane, lane 63: lane 0 of the next item)
+        uint64_t run_start_mask[ITEMS], kept_mask[ITEMS];
 #pragma unroll
-        for (int i = 0; i < kScanItems; i++) {
-            const uint32_t li = wave * kWave * kScanItems + i * kWave + lane;
+        for (int i = 0; i < ITEMS; i++) {
+            const uint32_t li = w0 + i * kWave + lane;
             const uint32_t idx = base + li;
             const uint64_t from_lane = __shfl_down(k[i], 1, kWave);
-            const uint64_t from_next = __shfl(k[i + 1 < kScanItems ? i + 1 : i], 0, kWave);
+            const uint64_t from_next = __shfl(k[i + 1 < ITEMS ? i + 1 : i], 0, kWave);
+            const uint64_t prev_lane = __shfl_up(k[i], 1, kWave);
+            const uint64_t prev_item = __shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave);
             f[i] = 0;
+            bool rs = false;
             if (li < len) {
                 bool last = idx + 1 == end;
                 if (!last) {
                     uint64_t nk;
-                    if (li + 1 == len || (lane == kWave - 1 && i == kScanItems - 1))
-                        nk = packed ? keys[idx + 1] >> 32 : keys[idx + 1];
+                    if (li + 1 == len || (lane == kWave - 1 && i == ITEMS - 1))
+                        nk = key_at(idx + 1);
                     else
                         nk = lane < kWave - 1 ? from_lane : from_next;
                     last = nk != k[i];
                 }
                 f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
+                const uint64_t pk = lane > 0 ? prev_lane : (i > 0 ? prev_item : wave_prev);
+                // a new brick run starts here (or the segment starts here)
+                rs = idx == seg_start || (pk >> 6) != (k[i] >> 6);
             }
+            run_start_mask[i] = __ballot(rs);
+            kept_mask[i] = __ballot(f[i] != 0);
         }
-        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
-        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
+        // wave summary: any run start, carry out when it has one, any kept
+        {
+            bool any_start = false, any_kept = false, out = false;
 #pragma unroll
-        for (int i = 0; i < kScanItems; i++)
-            if (f[i]) {
-                const uint64_t pos = pre + e[i];
-                if (pos >= g.n) {
-                    atomicOr(&sc.counters[C_ERR], 16u);
-                    continue;
+            for (int i = 0; i < ITEMS; i++) {
+                const uint64_t S = run_start_mask[i], F = kept_mask[i];
+                any_kept |= F != 0;
+                if (S) {
+                    const int r = 63 - __clzll((long long)S);  // last run start of the row
+                    out = (F >> r) != 0;
+                    any_start = true;
+                } else if (any_start) {
+                    out = out || F != 0;
                 }
-                ukey[pos] = chunk | k[i];
-                uval[pos] = v[i];
             }
-        if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_U] = (uint32_t)(pre + total);
-    }
-}
-
-// bricks (gbuild.zig:394-428): head of each run of equal key >> 6; the run's
-// mask is OR-reduced inside each 64-element round of a wave and merged into
-// masks[] with one 64-bit atomicOr per (brick, round)
-__global__ void __launch_bounds__(kBlock) k_bricks(ScanCtx sc, RecordBufs b, uint32_t lbits,
-                                                    uint64_t* brick_key, uint64_t* masks, uint32_t* first_voxel,
-                                                    uint64_t brick_cap, uint32_t n_cap) {
-    __shared__ uint32_t s_tile, s_wave[4];
-    __shared__ uint64_t s_b;
-    const uint64_t* ukey = b.keys[sc.counters[C_FINAL] ^ 1];  // the unique voxels
-    const uint32_t n = min(sc.counters[C_U], n_cap);
-    const uint32_t ntiles = (n + kScanTile - 1) / kScanTile;
-    const uint32_t lane = lane_id();
-    for (;;) {
-        const uint32_t tile = next_tile(sc, &s_tile);
-        if (tile >= ntiles) break;
-        uint32_t f[kScanItems], e[kScanItems];
-        uint64_t k[kScanItems];
-#pragma unroll
-        for (int i = 0; i < kScanItems; i++) {
-            const uint64_t idx = scan_index(tile, i);
-            k[i] = idx < n ? ukey[idx] : ~0ull;
-        }
-        // the previous key comes from the previous lane (lane 0: lane 63 of
-        // the previous item); the wave's first element reads it from memory
-        const uint64_t wave_prev = [&]() {
-            const uint64_t idx0 = scan_index(tile, 0);
-            return idx0 > 0 && idx0 <= n ? ukey[idx0 - 1] : ~0ull;
-        }();
-#pragma unroll
-        for (int i = 0; i < kScanItems; i++) {
-            const uint64_t idx = scan_index(tile, i);
-            const uint64_t from_lane = __shfl_up(k[i], 1, kWave);
-            const uint64_t from_prev = __shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave);
-            const uint64_t prev = lane > 0 ? from_lane : (i > 0 ? from_prev : wave_prev);
-            f[i] = 0;
-            if (idx < n) {
-                f[i] = idx == 0 || (prev >> 6) != (k[i] >> 6);
-                if (first_voxel && (idx == 0 || (prev >> lbits) != (k[i] >> lbits)))
-                    first_voxel[(uint32_t)(k[i] >> lbits)] = (uint32_t)idx;
+            if (lane == 0) {
+                s_any_start[wave] = any_start;
+                s_any_kept[wave] = any_kept;
+                s_out[wave] = out;
             }
         }
-        uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
-        uint64_t pre = tile_prefix(sc, tile, total, &s_b);
-#pragma unroll
-        for (int i = 0; i < kScanItems; i++) {
-            uint64_t idx = scan_index(tile, i);
-            uint64_t b = pre + e[i] + f[i] - 1;  // brick of this voxel
-            bool ok = idx < n;
-            if (ok && b >= brick_cap) {  // only after a failed look-back
-                atomicOr(&sc.counters[C_ERR], 16u);
-                ok = false;
+        // carry into the piece: the run reaching into element 0 from before
+        if (wave == 0) {
+            bool carry = false;
+            const uint64_t k0 = __shfl(k[0], 0, kWave);
+            // (out-of-range records never head a brick: no look through them)
+            if (len > 0 && (k0 & invalid) == 0 && base > seg_start && (key_at(base - 1) >> 6) == (k0 >> 6)) {
+                // records base-1, base-2, ... of the same brick, 64 at a time;
+                // the record after each one decides whether it is the last of its key
+                int64_t top = (int64_t)base - 1;
+                uint64_t after = k0;  // key of the record just after the window
+                for (;;) {
+                    const int64_t j = top - (int64_t)lane;
+                    const bool in = j >= (int64_t)seg_start;
+                    uint64_t kj = ~0ull;
+                    uint32_t vj = 0;
+                    if (in) {
+                        if (packed) {
+                            const uint64_t w = keys[j];
+                            kj = w >> 32;
+                            vj = (uint32_t)w;
+                        } else {
+                            kj = keys[j];
+                            vj = vals[j];
+                        }
+                    }
+                    // key of record j + 1 (the shuffle runs in every lane: a
+                    // shuffle from an inactive lane yields no defined value)
+                    const uint64_t up = __shfl_up(kj, 1, kWave);
+                    const uint64_t nk = lane == 0 ? after : up;
+                    const bool same = in && (kj >> 6) == (k0 >> 6);
+                    const bool kept = same && nk != kj && (kj & invalid) == 0 && vj != 0;
+                    if (__ballot(kept)) {
+                        carry = true;
+                        break;
+                    }
+                    const uint64_t sm = __ballot(same);
+                    if (sm != ~0ull) break;  // the run starts inside this window
+                    after = __shfl(kj, kWave - 1, kWave);
+                    top -= kWave;
+                }
             }
-            if (ok && f[i]) brick_key[b] = k[i] >> 6;
-            // segmented OR over equal b (equal b are adjacent lanes)
+            if (lane == 0) s_carry0 = carry;
+        }
+        __syncthreads();
+        // this wave's carry in, then head flags row by row
+        bool carry = s_carry0;
+        for (uint32_t w = 0; w < wave; w++) carry = s_any_start[w] ? s_out[w] : (carry || s_any_kept[w]);
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            const uint64_t S = run_start_mask[i], F = kept_mask[i];
+            // kept records before this lane in its run, inside the row
+            const uint64_t starts_upto = S & (lt | (1ull << lane));
+            const int r = starts_upto ? 63 - __clzll((long long)starts_upto) : -1;
+            const uint64_t span = r >= 0 ? lt & ~((1ull << r) - 1) : lt;
+            const bool before = (F & span) != 0 || (r < 0 && carry);
+            h[i] = f[i] && !before;
+            // carry out of the row
+            if (S) {
+                const int rl = 63 - __clzll((long long)S);
+                carry = (F >> rl) != 0;
+            } else {
+                carry = carry || F != 0;
+            }
+        }
+        const uint32_t tot_u = tile_scan<ITEMS>(f, e, s_wave);
+        const uint32_t tot_b = tile_scan<ITEMS>(h, eh, s_wave2);
+        // both look-backs at once: wave 0 kept records, wave 1 brick heads
+        if (wave == 0) {
+            const uint64_t p = lookback_wave(su.status, tile, tot_u, su.epoch, &su.counters[C_ERR]);
+            if (lane == 0) s_pre[0] = p;
+        } else if (wave == 1) {
+            const uint64_t p = lookback_wave(status_heads, tile, tot_b, su.epoch, &su.counters[C_ERR]);
+            if (lane == 0) s_pre[1] = p;
+        }
+        __syncthreads();
+        const uint64_t pre_u = s_pre[0], pre_b = s_pre[1];
+#pragma unroll
+        for (int i = 0; i < ITEMS; i++) {
+            // brick index of a kept record: heads up to it, minus one
+            const uint64_t bi = pre_b + eh[i] + h[i] - 1;
+            bool ok = f[i] != 0;
+            if (ok) {
+                const uint64_t pos = pre_u + e[i];
+                if (pos >= g.n || bi >= brick_cap) {  // only after a failed look-back
+                    atomicOr(&su.counters[C_ERR], 16u);
+                    ok = false;
+                } else {
+                    ukey[pos] = chunk | k[i];
+                    uval[pos] = v[i];
+                    if (h[i]) brick_key[bi] = (chunk | k[i]) >> 6;
+                    if (first_voxel && e[i] == 0) atomicMin(&first_voxel[td.x], (uint32_t)pos);
+                }
+            }
+            // OR of the row's mask bits per brick, merged at each brick's last kept lane
             uint64_t m = ok ? 1ull << (k[i] & 63) : 0;
+            const uint64_t bk = ok ? bi : ~0ull;
 #pragma unroll
             for (int o = 1; o < kWave; o <<= 1) {
-                uint64_t y = __shfl_up(m, o, kWave);
-                uint64_t yb = __shfl_up(b, o, kWave);
-                if (lane >= (uint32_t)o && yb == b) m |= y;
+                const uint64_t y = __shfl_up(m, o, kWave);
+                const uint64_t yb = __shfl_up(bk, o, kWave);
+                if (lane >= (uint32_t)o && yb == bk) m |= y;
             }
-            uint64_t nb = __shfl_down(b, 1, kWave);
-            bool nok = __shfl_down((uint32_t)ok, 1, kWave);
-            bool run_end = lane == kWave - 1 || nb != b || !nok;
-            if (ok && run_end) atomicOr((unsigned long long*)&masks[b], (unsigned long long)m);
+            const uint64_t nbk = __shfl_down(bk, 1, kWave);
+            const bool run_end = lane == kWave - 1 || nbk != bk;
+            if (ok && run_end) atomicOr((unsigned long long*)&masks[bi], (unsigned long long)m);
         }
         if (tile == ntiles - 1 && threadIdx.x == 0) {
-            sc.counters[C_B] = (uint32_t)(pre + total);
-            sc.counters[C_NCH + 2] = (uint32_t)(pre + total);
+            su.counters[C_U] = (uint32_t)(pre_u + tot_u);
+            su.counters[C_B] = (uint32_t)(pre_b + tot_b);
+            su.counters[C_NCH + 2] = (uint32_t)(pre_b + tot_b);
         }
     }
 }
@@ -1187,6 +1264,31 @@ void launch_sort(hipStream_t s, uint32_t rb, uint32_t items, bool packed, uint32
     }
 }
 
+template <int ITEMS>
+void launch_unique_bricks_t(hipStream_t s, uint32_t sort_tile, uint64_t sort_tiles, const ScanCtx& sc,
+                            const Segs& seg, const RecordBufs& bufs, uint32_t lbits, bool packed, uint64_t* brick_key,
+                            uint64_t* masks, uint32_t* first_dev, uint64_t nb) {
+    constexpr uint32_t tile = kBlock * ITEMS;
+    const uint32_t subs = (sort_tile + tile - 1) / tile;
+    // brick-head look-back statuses follow the kept-record ones
+    uint64_t* status_heads = sc.status + sort_tiles * subs;
+    hipLaunchKernelGGL(k_unique_bricks<ITEMS>, dim3((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048)),
+                       dim3(kBlock), 0, s, sc, status_heads, seg, subs, bufs, lbits, packed, brick_key, masks,
+                       first_dev, nb);
+}
+
+// unique + bricks with scan tiles of 256 * items records
+void launch_unique_bricks(hipStream_t s, uint32_t items, uint32_t sort_tile, uint64_t sort_tiles, const ScanCtx& sc,
+                          const Segs& seg, const RecordBufs& bufs, uint32_t lbits, bool packed, uint64_t* brick_key,
+                          uint64_t* masks, uint32_t* first_dev, uint64_t nb) {
+    switch (items) {
+        case 12: launch_unique_bricks_t<12>(s, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks, first_dev, nb); break;
+        case 16: launch_unique_bricks_t<16>(s, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks, first_dev, nb); break;
+        case 20: launch_unique_bricks_t<20>(s, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks, first_dev, nb); break;
+        default: launch_unique_bricks_t<8>(s, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks, first_dev, nb); break;
+    }
+}
+
 }  // namespace
 
 // One build: single geometry (enc.chunk_count == 0) or chunk batch.
@@ -1203,6 +1305,9 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint32_t sort_items = tuning && tuning->sort_items ? tuning->sort_items : 20;
     if (sort_items != 8 && sort_items != 12 && sort_items != 16 && sort_items != 20 && sort_items != 24)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "sort_items must be 8, 12, 16, 20 or 24");
+    const uint32_t scan_items = tuning && tuning->scan_items ? tuning->scan_items : kScanItems;
+    if (scan_items != 8 && scan_items != 12 && scan_items != 16 && scan_items != 20)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "scan_items must be 8, 12, 16 or 20");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -1251,8 +1356,10 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     seg.tiles = counters + C_TILES;
     const uint64_t scan_tiles = (nmax + kScanTile - 1) / kScanTile;
     // sort passes: one word per (tile, digit); unique: one per kScanTile piece of a tile
+    // (sort passes: one word per tile and digit; unique/bricks: two per scan piece)
     const uint64_t status_words =
-        std::max<uint64_t>(sort_tiles * std::max<uint32_t>(radix, (sort_tile + kScanTile - 1) / kScanTile), scan_tiles);
+        std::max<uint64_t>(sort_tiles * std::max<uint32_t>(radix, 2 * ((sort_tile + kScanTile - 1) / kScanTile)),
+                           scan_tiles);
     uint64_t* status = A.get<uint64_t>(status_words);
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
@@ -1350,15 +1457,10 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     tm.mark();
     // 3. unique into the buffer pair that does not hold the sorted records;
     // it becomes Built.keys/attrs
-    const uint32_t scan_grid = grid_for(nmax, kScanTile, 2048);
     if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
-        const uint32_t subs = (sort_tile + kScanTile - 1) / kScanTile;
-        hipLaunchKernelGGL(k_unique, dim3((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048)), dim3(kBlock), 0, s,
-                           sc, seg, subs, bufs, lbits, packed);
-        ScanCtx sb{status, ++epoch, tile_ctr(), counters};
-        hipLaunchKernelGGL(k_bricks, dim3(scan_grid), dim3(kBlock), 0, s, sb, bufs, lbits, brick_key, masks,
-                           first_dev, nb, nmax);
+        launch_unique_bricks(s, scan_items, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks,
+                             first_dev, nb);
     }
     tm.mark();
     // 4. leaves

@@ -57,9 +57,9 @@ class Built:
             pass
 
 
-def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0) -> abi.PyrhipBuildTuning:
+def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0) -> abi.PyrhipBuildTuning:
     t = abi.PyrhipBuildTuning()
-    t.radix_bits, t.sort_items, t.pair_layout = radix_bits, sort_items, pair_layout
+    t.radix_bits, t.sort_items, t.pair_layout, t.scan_items = radix_bits, sort_items, pair_layout, scan_items
     return t
 
 
@@ -72,7 +72,7 @@ def _ptr(x) -> int:
 
 
 def build(log2_size: int, voxels, count: int | None = None, old: Built | None = None, radix_bits: int = 0,
-          stream: int | None = None, sort_items: int = 0, pair_layout: int = 0) -> Built:
+          stream: int | None = None, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0) -> Built:
     """Build a DAG from PyrVoxel records (numpy VOXEL array or device Buffer).
     `old` supplies the sorted voxels of an earlier editable build (edits)."""
     keep = None
@@ -85,7 +85,7 @@ def build(log2_size: int, voxels, count: int | None = None, old: Built | None = 
         vptr = _ptr(voxels)
         assert count is not None
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits, sort_items, pair_layout)
+    t = _tuning(radix_bits, sort_items, pair_layout, scan_items)
     L = lib()
     code = L.pyrhip_build(log2_size, old.raw.keys if old else 0, old.raw.attrs if old else 0,
                           old.voxel_count if old else 0, vptr, count, C.byref(t), stream, C.byref(out))

@@ -108,6 +108,18 @@ def test_tile_sizes_and_layouts_identical(hip_device, sort_items, pair_layout):
     assert_same_build(b2, O.gb_build(9, ed, ref["keys"], ref["attrs"]), 9)
 
 
+@pytest.mark.parametrize("scan_items", [8, 12, 16, 20])
+@pytest.mark.parametrize("sort_items", [12, 20])
+def test_scan_tile_sizes_identical(hip_device, scan_items, sort_items):
+    # unique / brick scan tiles of 256 * scan_items records, cut from sort tiles
+    # of 256 * sort_items (pieces of unequal length), single and batched
+    print(f"case: scan_items {scan_items} sort_items {sort_items}", flush=True)
+    pts = random_points(150000, 8, 80 + scan_items, dup_frac=0.3, delete_frac=0.1, out_frac=0.01)
+    ref = O.gb_build(8, pts)
+    b = _gpu().build(8, pts, sort_items=sort_items, scan_items=scan_items)
+    assert_same_build(b, ref, 8)
+
+
 def test_scene_dense_and_empty_and_single(hip_device):
     print("case: scene 64^3 / empty / single voxel", flush=True)
     g = _gpu()
