@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="smaller traversal/batch sections")
     ap.add_argument("--only", default="", help="comma list of sections: build,trace,batch,cpu,probe")
     ap.add_argument("--radix-bits", type=int, default=0)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="build with the built-in defaults instead of the setting the autotuner picks for the "
+                         "workload class (tuning runs before the timed region)")
     ap.add_argument("--log2", type=int, default=10, help="edge of the headline grid (10 = 1024^3)")
     return ap.parse_args()
 
@@ -73,6 +76,13 @@ def build_section(args, out):
     log(f"generating {w.name} on the device")
     vox, offs, n = w.generate_device()
     log(f"{n} voxels resident ({n * 16 / 1e9:.2f} GB)")
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = gpu.autotune(args.log2, vox, count=n, repeats=2)
+        tune["seconds"] = time.perf_counter() - t0
+        log(f"autotune: {tune['best']} {tune['best_ms']:.2f} ms (defaults {tune['default_ms']:.2f} ms), "
+            f"{tune['candidates']} settings in {tune['seconds']:.1f} s")
     L = lib()
     t = abi.PyrhipBuildTuning()
     t.radix_bits = args.radix_bits
@@ -116,6 +126,7 @@ def build_section(args, out):
         "phase_ms": {k: float(np.mean([p[k] for p in phases])) for k in
                      ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "sort_pass_ms": pass_ms,
+        "autotune": tune,
     }
     # algorithmic bytes of the whole build: encode 16N read + rec*N write, P
     # passes x 2*rec*N, unique rec*N read + 12U write, brick pass 8U read,
@@ -247,7 +258,7 @@ def cpu_trace_baseline(args, built, cam, W, H):
 
 
 def batch_section(args, out):
-    from pyrit_amd import abi, lib, workloads
+    from pyrit_amd import abi, gpu, lib, workloads
     from pyrit_amd.device import Buffer, Events, synchronize, to_device
 
     count = 1024 if not args.quick else 128
@@ -265,6 +276,12 @@ def batch_section(args, out):
     first = np.zeros(count, np.uint32)
     built = abi.PyrhipBuilt()
     top = int(sizes.max())
+    tune = None
+    if not args.no_autotune:
+        t0 = time.perf_counter()
+        tune = gpu.autotune(top, vox, offsets=offs.astype(np.uint32), chunk_log2=sizes, repeats=2)
+        tune["seconds"] = time.perf_counter() - t0
+        log(f"batch autotune: {tune['best']} {tune['best_ms']:.2f} ms (defaults {tune['default_ms']:.2f} ms)")
 
     def batched():
         abi.check(L, L.pyrhip_build_chunks(top, C.byref(batch), None, None, C.byref(built),
@@ -297,6 +314,7 @@ def batch_section(args, out):
         "per_chunk_gvoxels_per_s": n / (p_ms * 1e-3) / 1e9, "speedup_vs_per_chunk": p_ms / b_ms,
         "per_chunk_latency_ms_p50": float(np.percentile(lat, 50)),
         "per_chunk_latency_ms_p99": float(np.percentile(lat, 99)),
+        "autotune": tune,
     }
     for b in (vox, d_offs, d_sizes):
         b.free()

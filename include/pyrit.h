@@ -193,7 +193,59 @@ typedef struct PyrVoxel {
     uint32_t attribute; /* != 0; 0 deletes in edits */
 } PyrVoxel;
 
+/* ---- host DAGs (reference pyrit.h:401-444) --------------------------------
+ *
+ * Host-side DAGs in the DAG format (nodes, leaves, attributes in DFS rank
+ * order). This implementation builds them on the GPU (the same build as
+ * pyr_geometry_build) and downloads the arrays, so their node order is the
+ * GPU build's; the reference's CPU builder orders nodes differently, and both
+ * describe the same voxels, traverse identically and are equal after
+ * canonicalisation. pyr_dag_* functions are thread-safe and need a HIP device. */
+
+#define PYR_DAG_NO_ATTRIBUTES 0x1u /* geometry only; hits report default_attribute = 1 */
+
+typedef struct PyrDag PyrDag;
+
+/* attribute of voxel (x, y, z), 0 = empty */
+typedef uint32_t (*PyrVoxelFn)(void* user, int32_t x, int32_t y, int32_t z);
+/* optional: is the cube [x, x+size)^3 certainly empty? non-zero skips it */
+typedef int32_t (*PyrRegionEmptyFn)(void* user, int32_t x, int32_t y, int32_t z, uint32_t size);
+
+typedef struct PyrDagInfo {
+    uint32_t log2_size;
+    uint32_t root;
+    uint64_t voxel_count;
+    const uint32_t* nodes;
+    uint64_t node_words;
+    const uint64_t* leaves;
+    uint64_t leaf_count;
+    const uint32_t* attributes; /* NULL with PYR_DAG_NO_ATTRIBUTES */
+    uint64_t attribute_count;
+} PyrDagInfo;
+
+/* voxels[x + N * (y + N * z)], N = 2^log2_size */
+PYR_API PyrResult pyr_dag_build_dense(uint32_t log2_size, const uint32_t* voxels, uint32_t flags, PyrDag** out);
+PYR_API PyrResult pyr_dag_build_points(uint32_t log2_size, const PyrVoxel* voxels, size_t count, uint32_t flags,
+                                       PyrDag** out);
+PYR_API PyrResult pyr_dag_build_fn(uint32_t log2_size, PyrVoxelFn voxel, PyrRegionEmptyFn region_empty, void* user,
+                                   uint32_t flags, PyrDag** out);
+PYR_API void pyr_dag_get_info(const PyrDag* dag, PyrDagInfo* out);
+PYR_API void pyr_dag_destroy(PyrDag* dag);
+/* the reference's binary DAG format ("PYRD", src/io.zig:86-160); buffer == NULL: only *size */
+PYR_API PyrResult pyr_dag_save(const PyrDag* dag, void* buffer, size_t* size);
+PYR_API PyrResult pyr_dag_load(const void* data, size_t size, PyrDag** out);
+
 /* ---- geometry on the GPU (reference pyrit.h:454-480) --------------------- */
+
+/* uploads a host DAG (the DAG may be destroyed right after) */
+PYR_API PyrResult pyr_geometry_create(PyrContext* ctx, const PyrDag* dag, PyrGeometry* out);
+/* coarser LOD of an editable geometry built on the GPU: edge 2^(log2_size -
+ * shift), one voxel per coarse cell (the source voxel with the largest key);
+ * flags: PYR_BUILD_EDITABLE */
+PYR_API PyrResult pyr_geometry_downsample(PyrContext* ctx, PyrGeometry geometry, uint32_t shift, uint32_t flags,
+                                          PyrGeometry* out);
+/* copies a geometry from the GPU into a host DAG (e.g. for pyr_dag_save) */
+PYR_API PyrResult pyr_geometry_download(PyrContext* ctx, PyrGeometry geometry, PyrDag** out);
 
 #define PYR_BUILD_HOST_INPUT 0x1u /* voxels is a host pointer */
 #define PYR_BUILD_EDITABLE 0x2u   /* keep the sorted voxels: pyr_geometry_edit allowed */

@@ -60,14 +60,19 @@ typedef struct PyrhipChunkBatch {
     uint64_t chunk_log2;
 } PyrhipChunkBatch;
 
-/* Tuning of one build; zeros select the defaults. Every setting gives the
- * same output; they trade passes, LDS footprint and occupancy. */
+/* Tuning of one build. Every setting gives the same output; they trade
+ * passes, tile sizes (the work of one 256-thread block), LDS footprint,
+ * occupancy and hash probe lengths. A field left 0 takes the value installed
+ * for the build's workload class by pyrhip_autotune / pyrhip_tuning_set, else
+ * the built-in default. */
 typedef struct PyrhipBuildTuning {
     uint32_t radix_bits;   /* digit width of the LSD passes: 8 (default), 9, 10 or 11 */
     uint32_t sort_items;   /* keys per thread of a sort tile (256 threads): 8, 12, 16, 20 (default), 24 */
-    uint32_t pair_layout;  /* 0: pack key << 32 | attribute when keys fit 32 bits (8-bit digits);
+    uint32_t pair_layout;  /* 2 (default): pack key << 32 | attribute when keys fit 32 bits (8-bit digits);
                               1: always separate key and attribute arrays */
     uint32_t scan_items;   /* records per thread of a unique / brick scan tile: 8 (default), 12, 16, 20 */
+    uint32_t hash_load_pct; /* maximum load of the leaf and node hash tables in percent, 25..95 (default 80);
+                               the output does not depend on it */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
@@ -87,6 +92,42 @@ PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch
                                       uint32_t* roots, uint32_t* first_voxel);
 
 PYR_API void pyrhip_built_free(PyrhipBuilt* built, void* stream);
+
+/* gpu_build.downsample (src/gpu_build.zig:124-131): DAG of edge
+ * 2^(log2_src - shift) from n_src sorted unique voxels (src_keys/src_vals of
+ * an editable build); every source voxel moves to its coarse cell and, per
+ * cell, the voxel with the largest source key wins. shift >= 1 and
+ * log2_src - shift >= 3. */
+PYR_API PyrResult pyrhip_downsample(uint64_t src_keys, uint64_t src_vals, uint32_t n_src, uint32_t log2_src,
+                                    uint32_t shift, const PyrhipBuildTuning* tuning, void* stream, PyrhipBuilt* out);
+
+/* ---- tuning per workload class (no counterpart in the reference, whose
+ * build has fixed launch shapes: gpu_build.zig:186-190) -------------------
+ * A class is (log2_size, bit length of the record count, bit length of the
+ * chunk count; 0 for single builds). */
+
+typedef struct PyrhipTuneReport {
+    PyrhipBuildTuning best; /* every field explicit */
+    float best_ms;          /* median build time of the winner (HIP events on the stream) */
+    float default_ms;       /* the same with the built-in defaults */
+    uint32_t candidates;    /* settings measured */
+    uint32_t rejected;      /* settings that failed or whose output differed from the defaults' (0 expected) */
+} PyrhipTuneReport;
+
+/* Measures settings on this workload (a single build of n PyrVoxel records at
+ * `voxels`, or `batch` when not NULL), each `repeats` times (0: 3) after one
+ * untimed run, by coordinate descent over sort_items, scan_items, radix_bits /
+ * pair_layout and hash_load_pct; a change must be 2 % faster to be taken. The
+ * winner is installed for the workload's class. Synchronises `stream`. */
+PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t n, const PyrhipChunkBatch* batch,
+                                  uint32_t repeats, void* stream, PyrhipTuneReport* out);
+/* installs (tuning != NULL) or removes the setting of a class, e.g. one saved
+ * from an earlier autotune run */
+PYR_API PyrResult pyrhip_tuning_set(uint32_t log2_size, uint32_t n, uint32_t chunk_count,
+                                    const PyrhipBuildTuning* tuning);
+/* the setting a build of this class uses when its tuning fields are 0 */
+PYR_API PyrResult pyrhip_tuning_get(uint32_t log2_size, uint32_t n, uint32_t chunk_count, PyrhipBuildTuning* out);
+PYR_API void pyrhip_tuning_clear(void);
 
 /* Sorted voxel keys of a build (Built.keys/attrs, or a chunk's slice of a
  * batch) back to PyrVoxel records in chunk-local coordinates; log2_size is the

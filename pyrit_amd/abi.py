@@ -22,6 +22,7 @@ PYR_TRACE_ANY_HIT = 0x1
 PYR_TRACE_NO_ATTRIBUTE = 0x2
 PYR_BUILD_HOST_INPUT = 0x1
 PYR_BUILD_EDITABLE = 0x2
+PYR_DAG_NO_ATTRIBUTES = 0x1
 PYR_VERSION = (0 << 16) | 2
 
 
@@ -78,7 +79,13 @@ class PyrhipChunkBatch(C.Structure):
 
 
 class PyrhipBuildTuning(C.Structure):
-    _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("scan_items", u32)]
+    _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("scan_items", u32),
+                ("hash_load_pct", u32)]
+
+
+class PyrhipTuneReport(C.Structure):
+    _fields_ = [("best", PyrhipBuildTuning), ("best_ms", f32), ("default_ms", f32), ("candidates", u32),
+                ("rejected", u32)]
 
 
 class PyrhipBuildTimes(C.Structure):
@@ -163,7 +170,15 @@ class PyrTerrainChunk(C.Structure):
                 ("grid_dim", u32), ("grid_first", u32)]
 
 
+class PyrDagInfo(C.Structure):
+    _fields_ = [("log2_size", u32), ("root", u32), ("voxel_count", u64), ("nodes", C.POINTER(u32)),
+                ("node_words", u64), ("leaves", C.POINTER(u64)), ("leaf_count", u64),
+                ("attributes", C.POINTER(u32)), ("attribute_count", u64)]
+
+
 P = C.POINTER
+VOXEL_FN = C.CFUNCTYPE(u32, vp, i32, i32, i32)
+REGION_EMPTY_FN = C.CFUNCTYPE(i32, vp, i32, i32, i32, u32)
 
 # name -> (restype, argtypes) of every C entry point of libpyrit.so
 FUNCTIONS = {
@@ -176,6 +191,9 @@ FUNCTIONS = {
     "pyr_synchronize": (i32, [vp]),
     "pyr_geometry_build": (i32, [vp, u32, vp, u32, u32, P(vp)]),
     "pyr_geometry_edit": (i32, [vp, vp, vp, u32, u32]),
+    "pyr_geometry_create": (i32, [vp, vp, P(vp)]),
+    "pyr_geometry_downsample": (i32, [vp, vp, u32, u32, P(vp)]),
+    "pyr_geometry_download": (i32, [vp, vp, P(vp)]),
     "pyr_geometry_destroy": (i32, [vp, vp]),
     "pyr_geometry_index": (u32, [vp]),
     "pyr_instance_create": (i32, [vp, vp, P(vp)]),
@@ -196,6 +214,11 @@ FUNCTIONS = {
                                   P(u32)]),
     "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),
     "pyrhip_decode_voxels": (i32, [u64, u64, u32, u32, u64, vp]),
+    "pyrhip_autotune": (i32, [u32, u64, u32, P(PyrhipChunkBatch), u32, vp, P(PyrhipTuneReport)]),
+    "pyrhip_tuning_set": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
+    "pyrhip_tuning_get": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
+    "pyrhip_tuning_clear": (None, []),
+    "pyrhip_downsample": (i32, [u64, u64, u32, u32, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
     "pyrhip_set_timing": (None, [i32]),
     "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),
     "pyrhip_last_build_times": (None, [P(PyrhipBuildTimes)]),
@@ -222,6 +245,13 @@ FUNCTIONS = {
     "pyr_world_edits_save": (i32, [vp, vp, u64]),
     "pyr_world_edits_load": (i32, [vp, vp, vp, u64]),
     "pyr_world_stats": (i32, [vp, P(PyrWorldStats)]),
+    "pyr_dag_build_dense": (i32, [u32, vp, u32, P(vp)]),
+    "pyr_dag_build_points": (i32, [u32, vp, C.c_size_t, u32, P(vp)]),
+    "pyr_dag_build_fn": (i32, [u32, VOXEL_FN, REGION_EMPTY_FN, vp, u32, P(vp)]),
+    "pyr_dag_get_info": (None, [vp, P(PyrDagInfo)]),
+    "pyr_dag_destroy": (None, [vp]),
+    "pyr_dag_save": (i32, [vp, vp, P(C.c_size_t)]),
+    "pyr_dag_load": (i32, [vp, C.c_size_t, P(vp)]),
 }
 
 TERRAIN_FUNCTIONS = {

@@ -1,7 +1,9 @@
 """Build step for the native libraries (run by ``__graft_entry__.build()``).
 
 Compiles, in tree and for gfx950 only:
-  pyrit_amd/libpyrit.so          build.hip, trace.hip, api.cpp (the C ABI of
+  pyrit_amd/libpyrit.so          build.hip, trace.hip, api.cpp, serve.cpp,
+                                 world.cpp, devcache.cpp, dag.cpp, tune.cpp
+                                 (the C ABI of
                                  include/pyrit.h and include/pyrit_hip_build.h)
   pyrit_amd/libpyrit_terrain.so  terrain.hip (seeded workload generator)
 
@@ -33,7 +35,7 @@ COMMON = [
 ]
 
 LIBS = {
-    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp"],
+    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp", "dag.cpp", "tune.cpp"],
     "libpyrit_terrain.so": ["terrain.hip"],
 }
 

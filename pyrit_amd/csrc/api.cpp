@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <new>
 #include <vector>
 
 #include "internal.h"
@@ -487,6 +488,142 @@ PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const
     }
     pool_free(old.keys, ctx->stream);
     pool_free(old.vals, ctx->stream);
+    return PYR_OK;
+}
+
+// geometryCreate (context.zig:799-816): a host DAG uploaded into the pools
+PYR_API PyrResult pyr_geometry_create(PyrContext* ctx, const PyrDag* dag, PyrGeometry* out) {
+    diag_clear();
+    if (!ctx || !dag || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (dag->nodes.empty() || dag->root >= dag->nodes.size()) return fail(PYR_ERROR_INVALID_ARGUMENT, "empty DAG");
+    uint32_t slot = 0;
+    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
+    const uint64_t nw = dag->nodes.size(), nl = dag->leaves.size(), na = dag->has_attrs ? dag->attrs.size() : 0;
+    const int64_t no = ctx->nodes.ra.alloc(nw), lo = ctx->leaves.ra.alloc(nl), ao = ctx->attrs.ra.alloc(na);
+    auto release = [&]() {
+        if (no >= 0) ctx->nodes.ra.release((uint64_t)no, nw);
+        if (lo >= 0) ctx->leaves.ra.release((uint64_t)lo, nl);
+        if (ao >= 0) ctx->attrs.ra.release((uint64_t)ao, na);
+    };
+    if (no < 0 || lo < 0 || ao < 0) {
+        release();
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "geometry pool full (%llu words, %llu leaves, %llu attributes)",
+                    (unsigned long long)nw, (unsigned long long)nl, (unsigned long long)na);
+    }
+    hipStream_t s = ctx->stream;
+    PyrResult r = hip_check(hipMemcpyAsync((char*)ctx->nodes.base + (uint64_t)no * 4, dag->nodes.data(), nw * 4,
+                                           hipMemcpyHostToDevice, s),
+                            "node upload");
+    if (r == PYR_OK && nl)
+        r = hip_check(hipMemcpyAsync((char*)ctx->leaves.base + (uint64_t)lo * 8, dag->leaves.data(), nl * 8,
+                                     hipMemcpyHostToDevice, s),
+                      "leaf upload");
+    if (r == PYR_OK && na)
+        r = hip_check(hipMemcpyAsync((char*)ctx->attrs.base + (uint64_t)ao * 4, dag->attrs.data(), na * 4,
+                                     hipMemcpyHostToDevice, s),
+                      "attribute upload");
+    // the host arrays may be freed when this returns
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "geometry upload");
+    if (r != PYR_OK) {
+        release();
+        return r;
+    }
+    if (slot == ctx->geos.size()) ctx->geos.emplace_back();
+    Geometry& g = ctx->geos[slot];
+    const uint32_t gen = g.gen + 1;
+    g = Geometry();
+    g.gen = gen;
+    g.data.node_offset = (uint32_t)no;
+    g.data.leaf_offset = (uint32_t)lo;
+    g.data.attribute_offset = (uint32_t)ao;
+    g.data.root = dag->root;
+    g.data.log2_size = dag->log2_size;
+    g.data.flags = dag->has_attrs ? PYR_GEOMETRY_HAS_ATTRIBUTES : 0;
+    g.data.default_attribute = 1;
+    g.node_words = nw;
+    g.leaf_count = nl;
+    g.attr_count = na;
+    g.voxel_count = (uint32_t)dag->voxel_count;
+    g.live = true;
+    *out = (PyrGeometry)(uintptr_t)make_handle(slot, g.gen);
+    return PYR_OK;
+}
+
+// geometryDownsample (context.zig:1036-1050): a coarser LOD built on the GPU
+// from the kept sorted voxels of an editable geometry
+PYR_API PyrResult pyr_geometry_downsample(PyrContext* ctx, PyrGeometry geometry, uint32_t shift, uint32_t flags,
+                                          PyrGeometry* out) {
+    diag_clear();
+    if (!ctx || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    Geometry* src = geo_of(ctx, geometry);
+    if (!src) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    if (!src->keys) return fail(PYR_ERROR_INVALID_ARGUMENT, "source is not editable (build it with PYR_BUILD_EDITABLE)");
+    uint32_t slot = 0;
+    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
+    PyrhipBuilt b{};
+    PyrResult r = pyrhip_downsample((uint64_t)(uintptr_t)src->keys, (uint64_t)(uintptr_t)src->vals, src->voxel_count,
+                                    src->data.log2_size, shift, nullptr, ctx->stream, &b);
+    if (r != PYR_OK) return r;
+    if (slot == ctx->geos.size()) ctx->geos.emplace_back();  // may move the vector: src is not used below
+    Geometry& g = ctx->geos[slot];
+    const uint32_t gen = g.gen + 1;
+    g = Geometry();
+    g.gen = gen;
+    r = install(ctx, g, b, (flags & PYR_BUILD_EDITABLE) != 0);
+    if (r != PYR_OK) {
+        pyrhip_built_free(&b, ctx->stream);
+        return r;
+    }
+    g.live = true;
+    *out = (PyrGeometry)(uintptr_t)make_handle(slot, g.gen);
+    return PYR_OK;
+}
+
+// geometryDownload (context.zig:1009-1033): the geometry's pool ranges copied
+// into a host DAG; a chunk of a streamed world brings its batch's node and
+// leaf ranges (the chunk's root refers into them) and its own attributes
+PYR_API PyrResult pyr_geometry_download(PyrContext* ctx, PyrGeometry geometry, PyrDag** out) {
+    diag_clear();
+    if (!ctx || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    Geometry* g = geo_of(ctx, geometry);
+    if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    uint64_t nw = g->node_words, nl = g->leaf_count;
+    if (g->shared >= 0) {
+        nw = ctx->shared[(size_t)g->shared].node_words;
+        nl = ctx->shared[(size_t)g->shared].leaf_count;
+    }
+    const bool has_attrs = (g->data.flags & PYR_GEOMETRY_HAS_ATTRIBUTES) != 0;
+    const uint64_t na = has_attrs ? (g->shared >= 0 ? g->voxel_count : g->attr_count) : 0;
+    PyrDag* d = new (std::nothrow) PyrDag();
+    if (!d) return fail(PYR_ERROR_OUT_OF_MEMORY, "host DAG");
+    d->log2_size = g->data.log2_size;
+    d->root = g->data.root;
+    d->has_attrs = has_attrs;
+    d->nodes.resize(nw);
+    d->leaves.resize(nl);
+    d->attrs.resize(na);
+    hipStream_t s = ctx->stream;
+    PyrResult r = hip_check(hipMemcpyAsync(d->nodes.data(), (char*)ctx->nodes.base + (uint64_t)g->data.node_offset * 4,
+                                           nw * 4, hipMemcpyDeviceToHost, s),
+                            "node download");
+    if (r == PYR_OK && nl)
+        r = hip_check(hipMemcpyAsync(d->leaves.data(), (char*)ctx->leaves.base + (uint64_t)g->data.leaf_offset * 8,
+                                     nl * 8, hipMemcpyDeviceToHost, s),
+                      "leaf download");
+    if (r == PYR_OK && na)
+        r = hip_check(hipMemcpyAsync(d->attrs.data(), (char*)ctx->attrs.base + (uint64_t)g->data.attribute_offset * 4,
+                                     na * 4, hipMemcpyDeviceToHost, s),
+                      "attribute download");
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "geometry download");
+    if (r == PYR_OK && (uint64_t)d->root + 1 >= nw) r = fail(PYR_ERROR_INVALID_ARGUMENT, "geometry without nodes");
+    if (r != PYR_OK) {
+        delete d;
+        return r;
+    }
+    d->voxel_count = d->nodes[d->root + 1];  // the root's voxel count word
+    *out = d;
     return PYR_OK;
 }
 

@@ -66,8 +66,10 @@ constexpr int kScanItems = 8;                     // elements per thread in a sc
 constexpr int kScanTile = kBlock * kScanItems;    // 2048
 constexpr int kNodeStride = 10;                   // words per parent in node_tmp (gbuild.zig:66)
 
-__host__ __device__ inline uint64_t hash_cap_for(uint64_t n) {
-    uint64_t want = n + n / 4, c = 1024;
+// table capacity for n keys at a maximum load of load_pct percent: the next
+// power of two >= n * 100 / load_pct, at least 1024 (80 %: gpu_build.zig:275-277)
+__host__ __device__ inline uint64_t hash_cap_for(uint64_t n, uint32_t load_pct) {
+    uint64_t want = (n * 100 + load_pct - 1) / load_pct, c = 1024;
     while (c < want) c <<= 1;
     return c;
 }
@@ -810,10 +812,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
     }
 }
 
-// clear the first hash_cap_for(count) slots of a table (keys 0, owner ~0)
+// clear the first hash_cap_for(count, load) slots of a table (keys 0, owner ~0)
 __global__ void __launch_bounds__(kBlock) k_hash_clear(const uint32_t* counters, uint32_t count_slot, void* table,
-                                                        uint32_t key_bytes, uint32_t* owner, uint64_t max_cap) {
-    const uint64_t want = hash_cap_for(counters[count_slot]);
+                                                        uint32_t key_bytes, uint32_t* owner, uint64_t max_cap,
+                                                        uint32_t load_pct) {
+    const uint64_t want = hash_cap_for(counters[count_slot], load_pct);
     const uint64_t cap = want < max_cap ? want : max_cap;
     const uint64_t stride = (uint64_t)gridDim.x * kBlock;
     for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < cap; i += stride) {
@@ -837,9 +840,9 @@ __device__ __forceinline__ void owner_min(uint32_t* owner, uint64_t h, uint32_t 
 // before any CAS, so a hot mask costs loads instead of serialised atomics.
 __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, const uint64_t* masks, uint64_t* table,
                                                          uint32_t* owner, uint32_t* slot, uint32_t* ch_count,
-                                                         uint64_t max_cap, uint32_t brick_cap) {
+                                                         uint64_t max_cap, uint32_t brick_cap, uint32_t load_pct) {
     const uint32_t n = min(counters[C_B], brick_cap);
-    const uint64_t want = hash_cap_for(n);
+    const uint64_t want = hash_cap_for(n, load_pct);
     const uint64_t cap = want < max_cap ? want : max_cap, m = cap - 1;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n; b += stride) {
@@ -938,6 +941,7 @@ struct LevelArgs {
     uint32_t ch_cap, par_cap;
     uint64_t table_cap, word_cap;
     uint32_t chunk_count;
+    uint32_t load_pct;        // maximum load of the tuple hash in percent
 };
 
 // node_heads/node_build/node_insert/node_owner (gbuild.zig:475-550) in one
@@ -1009,7 +1013,7 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
 __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, LevelArgs a) {
     const uint32_t np = min(counters[C_NCH + a.lv + 1], a.par_cap);
     const uint32_t nch = min(counters[C_NCH + a.lv], a.ch_cap);
-    const uint64_t want = hash_cap_for(nch);
+    const uint64_t want = hash_cap_for(nch, a.load_pct);
     const uint64_t cap = want < a.table_cap ? want : a.table_cap, hm = cap - 1;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) {
@@ -1142,16 +1146,17 @@ __device__ __forceinline__ uint32_t compact3(uint64_t v) {
 // sorted keys (brick Morton << 6 | bit, chunk bits above lbits dropped) back to
 // PyrVoxel records in chunk-local coordinates
 __global__ void __launch_bounds__(kBlock) k_decode(const uint64_t* keys, const uint32_t* attrs, uint32_t count,
-                                                    uint32_t lbits, int4* out) {
+                                                    uint32_t lbits, uint32_t shift, int4* out) {
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < count; i += stride) {
         const uint64_t key = keys[i] & ((1ull << lbits) - 1);
         const uint64_t brick = key >> 6;
         const uint32_t bit = (uint32_t)(key & 63);
         int4 v;
-        v.x = (int)((compact3(brick) << 2) | (bit & 3));
-        v.y = (int)((compact3(brick >> 1) << 2) | ((bit >> 2) & 3));
-        v.z = (int)((compact3(brick >> 2) << 2) | (bit >> 4));
+        // coordinates, moved to their cells of edge 2^shift (downsample, gbuild.zig:262-274)
+        v.x = (int)(((compact3(brick) << 2) | (bit & 3)) >> shift);
+        v.y = (int)(((compact3(brick >> 1) << 2) | ((bit >> 2) & 3)) >> shift);
+        v.z = (int)(((compact3(brick >> 2) << 2) | (bit >> 4)) >> shift);
         v.w = (int)attrs[i];
         out[i] = v;
     }
@@ -1300,14 +1305,20 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (n64 >= 0xFFFFFFFFull) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many voxels for one build (%llu)",
                                           (unsigned long long)n64);
     const uint32_t n = (uint32_t)n64, nmax = std::max<uint32_t>(n, 1);
-    const uint32_t rb = tuning && tuning->radix_bits ? tuning->radix_bits : 8;
+    // fields the caller leaves 0 come from the tuning installed for this
+    // workload class (autotuner), else from the built-in defaults
+    const PyrhipBuildTuning tun = resolve_tuning(tuning, log2_size, n, enc.chunk_count);
+    const uint32_t rb = tun.radix_bits;
     if (rb < 8 || rb > 11) return fail(PYR_ERROR_INVALID_ARGUMENT, "radix_bits must be in [8, 11]");
-    const uint32_t sort_items = tuning && tuning->sort_items ? tuning->sort_items : 20;
+    if (tun.pair_layout > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "pair_layout must be 0, 1 or 2");
+    const uint32_t sort_items = tun.sort_items;
     if (sort_items != 8 && sort_items != 12 && sort_items != 16 && sort_items != 20 && sort_items != 24)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "sort_items must be 8, 12, 16, 20 or 24");
-    const uint32_t scan_items = tuning && tuning->scan_items ? tuning->scan_items : kScanItems;
+    const uint32_t scan_items = tun.scan_items;
     if (scan_items != 8 && scan_items != 12 && scan_items != 16 && scan_items != 20)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "scan_items must be 8, 12, 16 or 20");
+    const uint32_t load_pct = tun.hash_load_pct;
+    if (load_pct < 25 || load_pct > 95) return fail(PYR_ERROR_INVALID_ARGUMENT, "hash_load_pct must be in [25, 95]");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -1317,7 +1328,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint32_t passes = (key_bits + rb - 1) / rb;
     // keys of at most 32 bits travel packed with their attribute in one u64
     // (16 instead of 24 bytes per record and pass)
-    const bool packed = key_bits <= 32 && rb == 8 && !(tuning && tuning->pair_layout);
+    const bool packed = key_bits <= 32 && rb == 8 && tun.pair_layout != 1;
     const uint32_t radix = 1u << rb;
     const uint64_t k = enc.chunk_count ? enc.chunk_count : 1;
 
@@ -1330,7 +1341,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint64_t np2 = std::min(nb, cells(3));  // parents of bricks
     uint64_t word_bound = 2;
     for (uint32_t lv = 2; lv < log2_size; lv++) word_bound += std::min(nb, cells(lv + 1)) * kNodeStride;
-    const uint64_t cap = hash_cap_for(nb);
+    const uint64_t cap = hash_cap_for(nb, load_pct);
 
     const auto host_t0 = std::chrono::steady_clock::now();
     Alloc A{{}, s};
@@ -1466,9 +1477,9 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // 4. leaves
     const uint32_t brick_grid = grid_for(nb, kScanTile, 2048);
     hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(cap, 1024, 2048)), dim3(kBlock), 0, s, counters, (uint32_t)C_B,
-                       (void*)table, 8u, owner, cap);
+                       (void*)table, 8u, owner, cap, load_pct);
     hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s, counters, masks,
-                       table, owner, slot, set[0].count, cap, (uint32_t)nb);
+                       table, owner, slot, set[0].count, cap, (uint32_t)nb, load_pct);
     {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_leaf_compact, dim3(brick_grid), dim3(kBlock), 0, s, sc, masks, owner, slot,
@@ -1495,10 +1506,11 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         la.table_cap = cap;
         la.word_cap = word_bound;
         la.chunk_count = enc.chunk_count;
+        la.load_pct = load_pct;
         const uint64_t nch = lv == 2 ? nb : std::min(nb, cells(lv));
         const uint32_t g = grid_for(nch, kScanTile, 2048);
-        hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(hash_cap_for(nch), 1024, 2048)), dim3(kBlock), 0, s,
-                           counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, cap);
+        hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(hash_cap_for(nch, load_pct), 1024, 2048)), dim3(kBlock), 0, s,
+                           counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, cap, load_pct);
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_build, dim3(g), dim3(kBlock), 0, s, sc, la);
         hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(nb, cells(lv + 1)), kBlock * 4, 4096)),
@@ -1627,9 +1639,38 @@ PYR_API PyrResult pyrhip_decode_voxels(uint64_t keys, uint64_t attrs, uint32_t c
     if (count)
         hipLaunchKernelGGL(k_decode, dim3(grid_for(count, kBlock * 4, 4096)), dim3(kBlock), 0, (hipStream_t)stream,
                            (const uint64_t*)(uintptr_t)keys, (const uint32_t*)(uintptr_t)attrs, count,
-                           local_bits(log2_size), (int4*)(uintptr_t)out);
+                           local_bits(log2_size), 0u, (int4*)(uintptr_t)out);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "decode: %s", hipGetErrorString(e));
+}
+
+// Source voxels in ascending key order move to their coarse cells; the
+// build's stable sort and last-record-wins unique then keep, per cell, the
+// source voxel with the largest key.
+PYR_API PyrResult pyrhip_downsample(uint64_t src_keys, uint64_t src_vals, uint32_t n_src, uint32_t log2_src,
+                                    uint32_t shift, const PyrhipBuildTuning* tuning, void* stream, PyrhipBuilt* out) {
+    diag_clear();
+    if (!out || (n_src && (!src_keys || !src_vals))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (log2_src < 3 || log2_src > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if (shift == 0 || shift + 3 > log2_src)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "downsampling 2^%u by 2^%u is not possible (result at least 2^3)",
+                    log2_src, shift);
+    hipStream_t s = (hipStream_t)stream;
+    void* vox = nullptr;
+    if (pool_alloc(&vox, std::max<uint64_t>(n_src, 1) * 16, s) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "downsample input of %u voxels", n_src);
+    }
+    if (n_src)
+        hipLaunchKernelGGL(k_decode, dim3(grid_for(n_src, kBlock * 4, 4096)), dim3(kBlock), 0, s,
+                           (const uint64_t*)(uintptr_t)src_keys, (const uint32_t*)(uintptr_t)src_vals, n_src,
+                           local_bits(log2_src), shift, (int4*)vox);
+    hipError_t e = hipGetLastError();
+    PyrResult r = e == hipSuccess ? pyrhip_build(log2_src - shift, 0, 0, 0, (uint64_t)(uintptr_t)vox, n_src, tuning,
+                                                 stream, out)
+                                  : fail(PYR_ERROR_CUDA, "downsample: %s", hipGetErrorString(e));
+    pool_free(vox, s);
+    return r;
 }
 
 PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream) {

@@ -5,8 +5,20 @@
 #include <stdint.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "pyrit.h"
 #include "pyrit_hip_build.h"
+
+// A host DAG (pyr_dag_*): the DAG arrays in host memory.
+struct PyrDag {
+    uint32_t log2_size = 0, root = 0;
+    uint64_t voxel_count = 0;
+    std::vector<uint32_t> nodes;
+    std::vector<uint64_t> leaves;
+    std::vector<uint32_t> attrs;
+    bool has_attrs = true;
+};
 
 namespace pyr {
 
@@ -30,6 +42,12 @@ inline PyrResult fail(PyrResult code, const char* fmt, ...) {
 // stream waits for the freeing stream on the device.
 hipError_t pool_alloc(void** p, uint64_t bytes, hipStream_t s);
 void pool_free(void* p, hipStream_t s);
+
+// Build tuning of one workload: the caller's non-zero fields over the
+// tuning installed for the class (log2_size, bit length of the record count,
+// bit length of the chunk count) over the built-in defaults (tune.cpp).
+PyrhipBuildTuning resolve_tuning(const PyrhipBuildTuning* caller, uint32_t log2_size, uint32_t n,
+                                 uint32_t chunk_count);
 
 // Context internals used by streamed worlds (api.cpp): a chunk batch is
 // installed into the geometry pools once and every chunk gets a geometry

@@ -57,9 +57,13 @@ class Built:
             pass
 
 
-def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0) -> abi.PyrhipBuildTuning:
+def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
+            hash_load_pct: int = 0) -> abi.PyrhipBuildTuning:
+    """0 = the setting installed for the workload class, else the default;
+    pair_layout 1 = separate key / attribute arrays, 2 = packed when keys fit"""
     t = abi.PyrhipBuildTuning()
     t.radix_bits, t.sort_items, t.pair_layout, t.scan_items = radix_bits, sort_items, pair_layout, scan_items
+    t.hash_load_pct = hash_load_pct
     return t
 
 
@@ -72,7 +76,8 @@ def _ptr(x) -> int:
 
 
 def build(log2_size: int, voxels, count: int | None = None, old: Built | None = None, radix_bits: int = 0,
-          stream: int | None = None, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0) -> Built:
+          stream: int | None = None, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
+          hash_load_pct: int = 0) -> Built:
     """Build a DAG from PyrVoxel records (numpy VOXEL array or device Buffer).
     `old` supplies the sorted voxels of an earlier editable build (edits)."""
     keep = None
@@ -85,7 +90,7 @@ def build(log2_size: int, voxels, count: int | None = None, old: Built | None = 
         vptr = _ptr(voxels)
         assert count is not None
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits, sort_items, pair_layout, scan_items)
+    t = _tuning(radix_bits, sort_items, pair_layout, scan_items, hash_load_pct)
     L = lib()
     code = L.pyrhip_build(log2_size, old.raw.keys if old else 0, old.raw.attrs if old else 0,
                           old.voxel_count if old else 0, vptr, count, C.byref(t), stream, C.byref(out))
@@ -93,10 +98,9 @@ def build(log2_size: int, voxels, count: int | None = None, old: Built | None = 
     return Built(out)
 
 
-def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None,
-                 capacity: int = 0, radix_bits: int = 0, stream: int | None = None):
-    """Chunk batch (src/gpu_build.zig:98-122). voxels: VOXEL array or device
-    Buffer; offsets: u32[count+1] host array. Returns (Built, roots, first)."""
+def _batch(voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None, capacity: int):
+    """PyrhipChunkBatch of host or device voxels; the second value keeps the
+    device copies alive"""
     offsets = np.ascontiguousarray(offsets, dtype=np.uint32)
     count = len(offsets) - 1
     keep = []
@@ -116,10 +120,19 @@ def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.nda
     batch = abi.PyrhipChunkBatch()
     batch.count, batch.capacity, batch.voxels, batch.offsets = count, capacity, vptr, ob.ptr
     batch.total, batch.chunk_log2 = int(offsets[-1]), lptr
+    return batch, keep
+
+
+def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None,
+                 capacity: int = 0, radix_bits: int = 0, stream: int | None = None, hash_load_pct: int = 0):
+    """Chunk batch (src/gpu_build.zig:98-122). voxels: VOXEL array or device
+    Buffer; offsets: u32[count+1] host array. Returns (Built, roots, first)."""
+    batch, keep = _batch(voxels, offsets, chunk_log2, capacity)
+    count = batch.count
     roots = np.zeros(count, np.uint32)
     first = np.zeros(count, np.uint32)
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits)
+    t = _tuning(radix_bits, hash_load_pct=hash_load_pct)
     L = lib()
     code = L.pyrhip_build_chunks(log2_size, C.byref(batch), C.byref(t), stream, C.byref(out),
                                  roots.ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -179,3 +192,51 @@ def last_build_times() -> dict:
     t = abi.PyrhipBuildTimes()
     lib().pyrhip_last_build_times(C.byref(t))
     return {k: getattr(t, k) for k, _ in abi.PyrhipBuildTimes._fields_}
+
+
+def _tuning_dict(t: abi.PyrhipBuildTuning) -> dict:
+    return {k: int(getattr(t, k)) for k, _ in abi.PyrhipBuildTuning._fields_}
+
+
+def autotune(log2_size: int, voxels, count: int | None = None, offsets: np.ndarray | None = None,
+             chunk_log2: np.ndarray | None = None, repeats: int = 3, stream: int | None = None) -> dict:
+    """pyrhip_autotune on a single build (offsets None) or a chunk batch;
+    installs the winner for the workload class and returns the report with
+    the class key ({"log2_size", "n", "chunk_count"}) for tuning_set."""
+    keep = []
+    batch = None
+    if offsets is not None:
+        batch, keep = _batch(voxels, offsets, chunk_log2, 0)
+        vptr, n = 0, batch.total
+    elif isinstance(voxels, np.ndarray):
+        vb = to_device(voxels) if len(voxels) else Buffer(16)
+        keep.append(vb)
+        vptr, n = vb.ptr, len(voxels)
+    else:
+        vptr, n = _ptr(voxels), int(count)
+    rep = abi.PyrhipTuneReport()
+    L = lib()
+    abi.check(L, L.pyrhip_autotune(log2_size, vptr, n, C.byref(batch) if batch is not None else None, repeats,
+                                   stream, C.byref(rep)), "pyrhip_autotune")
+    return {"class": {"log2_size": log2_size, "n": n, "chunk_count": batch.count if batch is not None else 0},
+            "best": _tuning_dict(rep.best), "best_ms": rep.best_ms, "default_ms": rep.default_ms,
+            "candidates": rep.candidates, "rejected": rep.rejected}
+
+
+def tuning_get(log2_size: int, n: int, chunk_count: int = 0) -> dict:
+    t = abi.PyrhipBuildTuning()
+    L = lib()
+    abi.check(L, L.pyrhip_tuning_get(log2_size, n, chunk_count, C.byref(t)), "pyrhip_tuning_get")
+    return _tuning_dict(t)
+
+
+def tuning_set(log2_size: int, n: int, chunk_count: int, best: dict | None) -> None:
+    """install a setting (e.g. one saved from an autotune report) or remove it (None)"""
+    L = lib()
+    t = _tuning(**best) if best is not None else None
+    abi.check(L, L.pyrhip_tuning_set(log2_size, n, chunk_count, C.byref(t) if t is not None else None),
+              "pyrhip_tuning_set")
+
+
+def tuning_clear() -> None:
+    lib().pyrhip_tuning_clear()

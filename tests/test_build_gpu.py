@@ -94,7 +94,7 @@ def test_radix_widths_identical(hip_device, radix_bits):
 
 
 @pytest.mark.parametrize("sort_items", [8, 12, 16, 20, 24])
-@pytest.mark.parametrize("pair_layout", [0, 1])
+@pytest.mark.parametrize("pair_layout", [0, 1, 2])
 def test_tile_sizes_and_layouts_identical(hip_device, sort_items, pair_layout):
     # log2 9: 27-bit keys, so layout 0 sorts packed key << 32 | attribute words
     print(f"case: sort_items {sort_items} pair_layout {pair_layout}", flush=True)
@@ -118,6 +118,69 @@ def test_scan_tile_sizes_identical(hip_device, scan_items, sort_items):
     ref = O.gb_build(8, pts)
     b = _gpu().build(8, pts, sort_items=sort_items, scan_items=scan_items)
     assert_same_build(b, ref, 8)
+
+
+@pytest.mark.parametrize("load", [25, 50, 95])
+def test_hash_load_identical(hip_device, load):
+    # the hash capacity changes probe lengths only: owners are the smallest
+    # index with a mask / tuple, so the layout stays the same
+    print(f"case: hash load {load}%", flush=True)
+    pts = random_points(200000, 9, 31, dup_frac=0.2, delete_frac=0.05)
+    ref = O.gb_build(9, pts)
+    assert_same_build(_gpu().build(9, pts, hash_load_pct=load), ref, 9)
+    offs = np.array([0, 70000, 70000, len(pts)], np.uint32)
+    rb_, rr, rf = O.gb_build_chunks(9, pts, offs)
+    bb, roots, first = _gpu().build_chunks(9, pts, offs, hash_load_pct=load)
+    assert_same_build(bb, rb_, 9)
+    assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+
+
+def test_tuning_argument_checks(hip_device):
+    g = _gpu()
+    pts = random_points(1000, 5, 3)
+    for kw in ({"hash_load_pct": 10}, {"hash_load_pct": 99}, {"pair_layout": 3}, {"radix_bits": 12},
+               {"sort_items": 10}, {"scan_items": 24}):
+        with pytest.raises(RuntimeError, match="INVALID_ARGUMENT"):
+            g.build(5, pts, **kw)
+
+
+def test_autotune_installs_an_identical_setting(hip_device):
+    print("case: autotune single build and batch", flush=True)
+    g = _gpu()
+    default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 8, "hash_load_pct": 80}
+    g.tuning_clear()
+    try:
+        pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
+        rep = g.autotune(10, pts, repeats=1)
+        print(rep, flush=True)
+        assert rep["rejected"] == 0 and rep["candidates"] >= 10
+        assert rep["best_ms"] <= rep["default_ms"]
+        assert g.tuning_get(**rep["class"]) == rep["best"]
+        # same class (bit length of the count), other class untouched
+        assert g.tuning_get(10, 300000 - 5000, 0) == rep["best"]
+        assert g.tuning_get(10, 3000, 0) == default
+        # builds that leave the fields 0 use the installed setting: same output
+        assert_same_build(g.build(10, pts), O.gb_build(10, pts), 10)
+        rng = np.random.default_rng(7)
+        chunks = [random_points(int(rng.integers(0, 3000)), 6, int(rng.integers(1 << 30)), dup_frac=0.1)
+                  for _ in range(200)]
+        offs = np.concatenate([[0], np.cumsum([len(c) for c in chunks])]).astype(np.uint32)
+        vox = np.concatenate(chunks)
+        rep2 = g.autotune(6, vox, offsets=offs, repeats=1)
+        print(rep2, flush=True)
+        assert rep2["rejected"] == 0 and rep2["class"]["chunk_count"] == 200
+        assert g.tuning_get(**rep2["class"]) == rep2["best"]
+        ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs)
+        b, roots, first = g.build_chunks(6, vox, offs)
+        assert_same_build(b, ref, 6)
+        assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+        # a saved setting installs again; None removes it
+        g.tuning_set(10, 300000, 0, None)
+        assert g.tuning_get(10, 300000, 0) == default
+        g.tuning_set(10, 300000, 0, rep["best"])
+        assert g.tuning_get(10, 300000, 0) == rep["best"]
+    finally:
+        g.tuning_clear()
 
 
 def test_scene_dense_and_empty_and_single(hip_device):
