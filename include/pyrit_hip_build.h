@@ -129,6 +129,21 @@ PYR_API PyrResult pyrhip_tuning_set(uint32_t log2_size, uint32_t n, uint32_t chu
 PYR_API PyrResult pyrhip_tuning_get(uint32_t log2_size, uint32_t n, uint32_t chunk_count, PyrhipBuildTuning* out);
 PYR_API void pyrhip_tuning_clear(void);
 
+/* Palette packing of a batch's attributes (the reference's
+ * pyr_k_palette_scan / pyr_k_palette_pack, src/device/palette.zig, used by
+ * context.zig:1311-1370 when a world installs a batch). Chunk j owns ranks
+ * [first[j], first[j+1]) of `attrs` (first: host u32[count + 1]). On return
+ * palette_count[j] (host u32[count]) is its number of distinct attributes, or
+ * 0xFFFFFFFF when there are more than 16 (or the value 0xFFFFFFFF occurs), and
+ * word_offset[j] (host u64[count + 1]; the last entry is the total) its start
+ * in the output: 16 palette entries in ascending order (padded with the
+ * first) and then 4-bit indices, eight per word (rank r in bits 4 (r & 7) of
+ * word r >> 3); or its 32-bit attributes. out == 0 computes the layout only;
+ * otherwise out (device, out_words words) receives the packed attributes. */
+PYR_API PyrResult pyrhip_palette_pack(uint64_t attrs, const uint32_t* first, uint32_t count, uint64_t out,
+                                      uint64_t out_words, uint64_t* word_offset, uint32_t* palette_count,
+                                      void* stream);
+
 /* Sorted voxel keys of a build (Built.keys/attrs, or a chunk's slice of a
  * batch) back to PyrVoxel records in chunk-local coordinates; log2_size is the
  * edge the keys were built with (the batch's for a batch slice). The input of
@@ -172,6 +187,7 @@ typedef struct PyrhipDag {
     uint32_t log2_size;
     uint32_t default_attribute;
     uint32_t reserved;
+    uint64_t palette;      /* 0, or 16 attribute values: attrs then holds 4-bit indices, eight per word */
 } PyrhipDag;
 
 /* PyrRay[count] -> PyrHit[count] (device pointers) against one DAG, the

@@ -1269,3 +1269,59 @@ void or_edit_compact_append(const uint32_t* voxels, const uint32_t* counts, uint
             if (o < cap) memcpy(&out_voxels[((uint64_t)c * cap + o) * 4], q, 16);
         }
 }
+
+/* palette.zig:65-152 scan + pack, context.zig:1352-1357 word counts */
+void or_palette_pack(const uint32_t* attrs, const uint32_t* first, uint32_t k, uint32_t* count,
+                     uint64_t* word_offset, uint32_t* out) {
+    uint64_t words = 0;
+    for (uint32_t j = 0; j < k; j++) {
+        const uint64_t lo = first[j], n = (uint64_t)first[j + 1] - lo;
+        uint32_t pal[17];
+        uint32_t m = 0;
+        int full = 0;
+        for (uint64_t i = 0; i < n && !full; i++) {
+            const uint32_t v = attrs[lo + i];
+            if (v == 0xFFFFFFFFu) {
+                full = 1;
+                break;
+            }
+            uint32_t q = 0;
+            while (q < m && pal[q] != v) q++;
+            if (q < m) continue;
+            if (m == 16) {
+                full = 1;
+                break;
+            }
+            /* insertion into ascending order */
+            uint32_t p = m++;
+            while (p > 0 && pal[p - 1] > v) {
+                pal[p] = pal[p - 1];
+                p--;
+            }
+            pal[p] = v;
+        }
+        word_offset[j] = words;
+        count[j] = full ? 0xFFFFFFFFu : m;
+        if (full) {
+            if (out) memcpy(out + words, attrs + lo, n * 4);
+            words += n;
+            continue;
+        }
+        for (uint32_t q = m; q < 16; q++) pal[q] = m ? pal[0] : 0u;
+        if (out) {
+            memcpy(out + words, pal, 64);
+            for (uint64_t w = 0; w < (n + 7) / 8; w++) {
+                uint32_t word = 0;
+                for (uint32_t b = 0; b < 8 && w * 8 + b < n; b++) {
+                    const uint32_t v = attrs[lo + w * 8 + b];
+                    uint32_t idx = 0;
+                    while (pal[idx] != v) idx++;
+                    word |= idx << (4 * b);
+                }
+                out[words + 16 + w] = word;
+            }
+        }
+        words += 16 + (n + 7) / 8;
+    }
+    word_offset[k] = words;
+}

@@ -23,6 +23,9 @@
  *                         grid (tests/common.zig:104-153, cpu_test.zig:42-103)
  *   or_camera_*         : closed forms (centre pixel looks at target; unit dirs)
  *   or_range_*          : the allocation sequence of range_alloc.zig:61-78
+ *   or_palette_pack     : decoding the packed words reproduces every attribute
+ *                         (brute force), the 16-value boundary and the word
+ *                         counts of context.zig:1352-1357
  */
 #ifndef PYRIT_ORACLE_H
 #define PYRIT_ORACLE_H
@@ -142,6 +145,18 @@ void or_edit_apply(uint32_t* voxels /*[k*cap][4]*/, const uint32_t* counts, uint
 void or_edit_compact_append(const uint32_t* voxels, const uint32_t* counts, uint32_t k, uint32_t cap,
                             const uint32_t* edits, const uint32_t* edit_offsets, const uint32_t* edit_used,
                             uint32_t* out_voxels, uint32_t* out_counts);
+
+/* palette packing of a batch's attributes (palette.zig:1-152, layout of
+ * context.zig:1311-1370): chunk j owns ranks [first[j], first[j+1]);
+ * count[j] = distinct values, or 0xFFFFFFFF above 16 (or when 0xFFFFFFFF
+ * occurs); word_offset[j] its start (word_offset[k] = total). A palette chunk
+ * stores 16 entries, then the index of rank r in bits 4 (r & 7) of word
+ * r >> 3; the entries are the distinct values in ascending order padded with
+ * the first (the reference keeps hash-slot order, which concurrent inserts
+ * leave unspecified; any order decodes to the same attributes). out == NULL:
+ * layout only. */
+void or_palette_pack(const uint32_t* attrs, const uint32_t* first, uint32_t k, uint32_t* count,
+                     uint64_t* word_offset, uint32_t* out);
 
 #ifdef __cplusplus
 }

@@ -22,6 +22,8 @@ PYR_TRACE_ANY_HIT = 0x1
 PYR_TRACE_NO_ATTRIBUTE = 0x2
 PYR_BUILD_HOST_INPUT = 0x1
 PYR_BUILD_EDITABLE = 0x2
+PYR_GEOMETRY_HAS_ATTRIBUTES = 0x1
+PYR_GEOMETRY_PALETTE = 0x2
 PYR_DAG_NO_ATTRIBUTES = 0x1
 PYR_VERSION = (0 << 16) | 2
 
@@ -96,7 +98,7 @@ class PyrhipBuildTimes(C.Structure):
 
 class PyrhipDag(C.Structure):
     _fields_ = [("nodes", u64), ("leaves", u64), ("attrs", u64), ("root", u32), ("log2_size", u32),
-                ("default_attribute", u32), ("reserved", u32)]
+                ("default_attribute", u32), ("reserved", u32), ("palette", u64)]
 
 
 class PyrhipServerInfo(C.Structure):
@@ -218,6 +220,7 @@ FUNCTIONS = {
     "pyrhip_tuning_set": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
     "pyrhip_tuning_get": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
     "pyrhip_tuning_clear": (None, []),
+    "pyrhip_palette_pack": (i32, [u64, P(u32), u32, u64, u64, P(u64), P(u32), vp]),
     "pyrhip_downsample": (i32, [u64, u64, u32, u32, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
     "pyrhip_set_timing": (None, [i32]),
     "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),

@@ -2,7 +2,8 @@
 
 Compiles, in tree and for gfx950 only:
   pyrit_amd/libpyrit.so          build.hip, trace.hip, api.cpp, serve.cpp,
-                                 world.cpp, devcache.cpp, dag.cpp, tune.cpp
+                                 world.cpp, devcache.cpp, dag.cpp, tune.cpp,
+                                 palette.hip
                                  (the C ABI of
                                  include/pyrit.h and include/pyrit_hip_build.h)
   pyrit_amd/libpyrit_terrain.so  terrain.hip (seeded workload generator)
@@ -35,7 +36,7 @@ COMMON = [
 ]
 
 LIBS = {
-    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp", "dag.cpp", "tune.cpp"],
+    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp", "dag.cpp", "tune.cpp", "palette.hip"],
     "libpyrit_terrain.so": ["terrain.hip"],
 }
 

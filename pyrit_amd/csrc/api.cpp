@@ -260,25 +260,83 @@ namespace pyr {
 
 hipStream_t ctx_stream(PyrContext* ctx) { return ctx->stream; }
 
-// Copies a finished chunk batch into the pools once (nodes, leaves and all
-// attributes); on success the batch buffers are released and *id names the
-// shared ranges, held by one reference until ctx_release_batch.
-PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* b, int32_t* id) {
-    Geometry tmp;
-    PyrResult r = install(ctx, tmp, *b, false);
-    if (r != PYR_OK) return r;
-    memset(b, 0, sizeof(*b));  // install released the batch buffers
+// Copies a finished chunk batch into the pools once (nodes, leaves and the
+// attributes of every chunk, palette packed where a chunk has at most 16
+// distinct values, context.zig:1311-1370; PYRIT_NO_PALETTE in the environment
+// keeps 32 bits everywhere, as in the reference); on success the batch
+// buffers are released and *id names the shared ranges, held by one
+// reference until ctx_release_batch.
+PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* b, const std::vector<uint32_t>& first, int32_t* id,
+                            std::vector<uint64_t>* attr_word, std::vector<uint8_t>* palette) {
+    hipStream_t s = ctx->stream;
+    const uint32_t k = (uint32_t)first.size() - 1;
+    const uint32_t* attrs = (const uint32_t*)(uintptr_t)b->attrs;
+    std::vector<uint32_t> counts(k, kPaletteOverflow);
+    PaletteScratch ps;
+    PyrResult r = PYR_OK;
+    if (!getenv("PYRIT_NO_PALETTE")) r = palette_scan(attrs, first, s, &counts, &ps);
+    if (r != PYR_OK) {
+        palette_release(&ps, s);
+        return r;
+    }
+    attr_word->assign(k, 0);
+    palette->assign(k, 0);
+    uint64_t words = 0;
+    for (uint32_t j = 0; j < k; j++) {
+        (*attr_word)[j] = words;
+        (*palette)[j] = counts[j] != kPaletteOverflow;
+        words += palette_words(counts[j], first[j + 1] - first[j]);
+    }
+    const int64_t no = ctx->nodes.ra.alloc(b->node_words);
+    const int64_t lo = ctx->leaves.ra.alloc(b->leaf_count);
+    const int64_t ao = ctx->attrs.ra.alloc(words);
+    auto release = [&]() {
+        if (no >= 0) ctx->nodes.ra.release((uint64_t)no, b->node_words);
+        if (lo >= 0) ctx->leaves.ra.release((uint64_t)lo, b->leaf_count);
+        if (ao >= 0) ctx->attrs.ra.release((uint64_t)ao, words);
+    };
+    if (no < 0 || lo < 0 || ao < 0) {
+        release();
+        palette_release(&ps, s);
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "geometry pool full (%u words, %u leaves, %llu attribute words)",
+                    b->node_words, b->leaf_count, (unsigned long long)words);
+    }
+    r = hip_check(hipMemcpyAsync((char*)ctx->nodes.base + (uint64_t)no * 4, (void*)(uintptr_t)b->nodes,
+                                 (uint64_t)b->node_words * 4, hipMemcpyDeviceToDevice, s),
+                  "node copy");
+    if (r == PYR_OK)
+        r = hip_check(hipMemcpyAsync((char*)ctx->leaves.base + (uint64_t)lo * 8, (void*)(uintptr_t)b->leaves,
+                                     (uint64_t)b->leaf_count * 8, hipMemcpyDeviceToDevice, s),
+                      "leaf copy");
+    if (r == PYR_OK) {
+        if (ps.lo) {
+            std::vector<uint64_t> dst(k);
+            for (uint32_t j = 0; j < k; j++) dst[j] = (uint64_t)ao + (*attr_word)[j];
+            r = palette_pack(attrs, first, dst, (uint32_t*)ctx->attrs.base, s, &ps);
+        } else if (words) {
+            r = hip_check(hipMemcpyAsync((char*)ctx->attrs.base + (uint64_t)ao * 4, attrs, words * 4,
+                                         hipMemcpyDeviceToDevice, s),
+                          "attribute copy");
+        }
+    }
+    palette_release(&ps, s);
+    if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "install");
+    if (r != PYR_OK) {
+        release();
+        return r;
+    }
     size_t slot = 0;
     while (slot < ctx->shared.size() && ctx->shared[slot].refs) slot++;
     if (slot == ctx->shared.size()) ctx->shared.emplace_back();
     SharedRange& sr = ctx->shared[slot];
-    sr.node_offset = tmp.data.node_offset;
-    sr.leaf_offset = tmp.data.leaf_offset;
-    sr.attr_offset = tmp.data.attribute_offset;
-    sr.node_words = tmp.node_words;
-    sr.leaf_count = tmp.leaf_count;
-    sr.attr_count = tmp.attr_count;
+    sr.node_offset = (uint64_t)no;
+    sr.leaf_offset = (uint64_t)lo;
+    sr.attr_offset = (uint64_t)ao;
+    sr.node_words = b->node_words;
+    sr.leaf_count = b->leaf_count;
+    sr.attr_count = words;
     sr.refs = 1;
+    pyrhip_built_free(b, s);
     *id = (int32_t)slot;
     return PYR_OK;
 }
@@ -294,8 +352,9 @@ uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t id) {
 
 // A geometry for one chunk of an installed batch: the batch's node and leaf
 // ranges (word refs are relative to the batch start), the chunk's root and
-// its attributes from rank first_attr of the batch on.
-PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, uint32_t first_attr,
+// its attributes at word attr_word of the batch's attribute range: 16
+// palette entries and the index words (palette), or 32-bit values.
+PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, uint64_t attr_word, bool palette,
                                   uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out) {
     uint32_t slot = 0;
     while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
@@ -308,10 +367,15 @@ PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, ui
     g.gen = gen;
     g.data.node_offset = (uint32_t)sr.node_offset;
     g.data.leaf_offset = (uint32_t)sr.leaf_offset;
-    g.data.attribute_offset = (uint32_t)(sr.attr_offset + first_attr);
+    g.data.attribute_offset = (uint32_t)(sr.attr_offset + attr_word);
+    g.data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;
+    if (palette) {  // context.zig:1385-1389
+        g.data.flags |= PYR_GEOMETRY_PALETTE;
+        g.data.palette_offset = g.data.attribute_offset;
+        g.data.attribute_offset += 16;
+    }
     g.data.root = root;
     g.data.log2_size = log2_size;
-    g.data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;
     g.data.default_attribute = 1;
     g.voxel_count = voxel_count;
     g.shared = id;
@@ -612,11 +676,26 @@ PYR_API PyrResult pyr_geometry_download(PyrContext* ctx, PyrGeometry geometry, P
         r = hip_check(hipMemcpyAsync(d->leaves.data(), (char*)ctx->leaves.base + (uint64_t)g->data.leaf_offset * 8,
                                      nl * 8, hipMemcpyDeviceToHost, s),
                       "leaf download");
-    if (r == PYR_OK && na)
+    const bool palette = (g->data.flags & PYR_GEOMETRY_PALETTE) != 0;
+    std::vector<uint32_t> pal(16), packed(palette ? (na + 7) / 8 : 0);
+    if (r == PYR_OK && na && !palette)
         r = hip_check(hipMemcpyAsync(d->attrs.data(), (char*)ctx->attrs.base + (uint64_t)g->data.attribute_offset * 4,
                                      na * 4, hipMemcpyDeviceToHost, s),
                       "attribute download");
+    if (r == PYR_OK && na && palette) {
+        r = hip_check(hipMemcpyAsync(pal.data(), (char*)ctx->attrs.base + (uint64_t)g->data.palette_offset * 4, 64,
+                                     hipMemcpyDeviceToHost, s),
+                      "palette download");
+        if (r == PYR_OK)
+            r = hip_check(hipMemcpyAsync(packed.data(),
+                                         (char*)ctx->attrs.base + (uint64_t)g->data.attribute_offset * 4,
+                                         packed.size() * 4, hipMemcpyDeviceToHost, s),
+                          "attribute download");
+    }
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "geometry download");
+    // palette chunks come back with their 32-bit attributes (attributeAt, dag.zig:64-68)
+    if (r == PYR_OK && palette)
+        for (uint64_t i = 0; i < na; i++) d->attrs[i] = pal[(packed[i >> 3] >> ((i & 7) * 4)) & 15];
     if (r == PYR_OK && (uint64_t)d->root + 1 >= nw) r = fail(PYR_ERROR_INVALID_ARGUMENT, "geometry without nodes");
     if (r != PYR_OK) {
         delete d;

@@ -49,14 +49,40 @@ void pool_free(void* p, hipStream_t s);
 PyrhipBuildTuning resolve_tuning(const PyrhipBuildTuning* caller, uint32_t log2_size, uint32_t n,
                                  uint32_t chunk_count);
 
+// Palette packing of a chunk batch's attributes (palette.hip). lo holds the
+// first attribute rank of each chunk and, last, the total (k + 1 entries).
+// palette_scan reads back per chunk the number of distinct attributes or
+// kPaletteOverflow (more than 16: 32-bit storage); palette_words gives a
+// chunk's words in the attribute pool; palette_pack writes chunk j at word
+// dst[j] of pool: 16 palette entries then 4-bit indices, eight per word, or
+// the 32-bit attributes. The scratch lives from scan to release.
+constexpr uint32_t kPaletteOverflow = 0xFFFFFFFFu;
+struct PaletteScratch {
+    uint32_t* lo = nullptr;
+    uint32_t* palettes = nullptr;
+    uint32_t* table = nullptr;
+    uint32_t* counts = nullptr;  // inside table's block
+};
+uint64_t palette_words(uint32_t count, uint64_t voxels);
+PyrResult palette_scan(const uint32_t* attrs, const std::vector<uint32_t>& lo, hipStream_t s,
+                       std::vector<uint32_t>* counts, PaletteScratch* scratch);
+PyrResult palette_pack(const uint32_t* attrs, const std::vector<uint32_t>& lo, const std::vector<uint64_t>& dst,
+                       uint32_t* pool, hipStream_t s, PaletteScratch* scratch);
+void palette_release(PaletteScratch* scratch, hipStream_t s);
+
 // Context internals used by streamed worlds (api.cpp): a chunk batch is
 // installed into the geometry pools once and every chunk gets a geometry
 // that refers to the batch's shared ranges.
 hipStream_t ctx_stream(PyrContext* ctx);
-PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* built, int32_t* batch_id);
+// Chunk j of the k non-empty chunks of the batch owns attribute ranks
+// [first[j], first[j + 1]) (first[k] = voxel count); on success attr_word[j]
+// is its attribute offset relative to the batch's attribute range and
+// palette[j] says whether it is palette packed.
+PyrResult ctx_install_batch(PyrContext* ctx, PyrhipBuilt* built, const std::vector<uint32_t>& first,
+                            int32_t* batch_id, std::vector<uint64_t>* attr_word, std::vector<uint8_t>* palette);
 void ctx_release_batch(PyrContext* ctx, int32_t batch_id);
 uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t batch_id);
-PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t batch_id, uint32_t root, uint32_t first_attr,
-                                  uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out);
+PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t batch_id, uint32_t root, uint64_t attr_word,
+                                  bool palette, uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out);
 
 }  // namespace pyr

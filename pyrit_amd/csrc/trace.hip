@@ -24,9 +24,18 @@ constexpr uint32_t kMaxSteps = 65536;
 struct DagView {
     const uint32_t* nodes;
     const uint64_t* leaves;
-    const uint32_t* attrs;  // null: default_attribute
+    const uint32_t* attrs;    // null: default_attribute
+    const uint32_t* palette;  // non-null: attrs holds 4-bit indices into these 16 values
     uint32_t root, log2_size, default_attribute, attr_base;
 };
+
+// attributeAt (dag.zig:64-68): rank r's value, or its palette entry through
+// the 4-bit index in bits 4 (r & 7) of word r >> 3
+__device__ __forceinline__ uint32_t attribute_at(const DagView& g, uint32_t rank) {
+    const uint32_t r = g.attr_base + rank;
+    if (g.palette) return g.palette[(g.attrs[r >> 3] >> ((r & 7) * 4)) & 15u];
+    return g.attrs[r];
+}
 
 struct DagHitD {
     float t;
@@ -132,7 +141,7 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                     hit->voxel[a] = (int32_t)v[a];
                 }
                 uint32_t attribute = g.default_attribute;
-                if (want_attr && g.attrs) attribute = g.attrs[g.attr_base + attribute_rank(g, stk, v, brick)];
+                if (want_attr && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
                 const bool mirrored = ((oct >> axis) & 1u) != 0;
                 hit->t = t;
                 hit->face = 2 * axis + (mirrored ? 0u : 1u);
@@ -243,6 +252,8 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
         v.leaves = (const uint64_t*)(uintptr_t)s.leaves + g.leaf_offset;
         v.attrs = (g.flags & PYR_GEOMETRY_HAS_ATTRIBUTES) ? (const uint32_t*)(uintptr_t)s.attributes + g.attribute_offset
                                                          : nullptr;
+        v.palette = (g.flags & PYR_GEOMETRY_PALETTE) ? (const uint32_t*)(uintptr_t)s.attributes + g.palette_offset
+                                                     : nullptr;
         v.root = g.root;
         v.log2_size = g.log2_size;
         v.default_attribute = g.default_attribute;
@@ -383,6 +394,7 @@ DagView view_of(const PyrhipDag& d) {
     g.nodes = (const uint32_t*)(uintptr_t)d.nodes;
     g.leaves = (const uint64_t*)(uintptr_t)d.leaves;
     g.attrs = (const uint32_t*)(uintptr_t)d.attrs;
+    g.palette = d.attrs ? (const uint32_t*)(uintptr_t)d.palette : nullptr;
     g.root = d.root;
     g.log2_size = d.log2_size;
     g.default_attribute = d.default_attribute;

@@ -109,6 +109,7 @@ def lib() -> C.CDLL:
         L.or_edit_apply.argtypes = [C.c_void_p, C.c_void_p, u32, u32, C.c_void_p, C.c_void_p, C.c_void_p]
         L.or_edit_compact_append.argtypes = [C.c_void_p, C.c_void_p, u32, u32, C.c_void_p, C.c_void_p, C.c_void_p,
                                              C.c_void_p, C.c_void_p]
+        L.or_palette_pack.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
 
@@ -339,3 +340,17 @@ def ref_dda(grid: np.ndarray, n: int, o, d, tmin=0.0, tmax=1e30):
     hit = lib().or_ref_dda(_p(grid.ravel()), n, (f64 * 3)(*o), (f64 * 3)(*d), tmin, tmax, C.byref(t), v,
                            C.byref(face))
     return (t.value, tuple(v), face.value) if hit else None
+
+
+def palette_pack(attrs: np.ndarray, first: np.ndarray):
+    """(counts u32[k], word_offset u64[k+1], packed u32[total]) of or_palette_pack"""
+    attrs = np.ascontiguousarray(attrs, np.uint32)
+    first = np.ascontiguousarray(first, np.uint32)
+    k = len(first) - 1
+    counts = np.zeros(k, np.uint32)
+    offs = np.zeros(k + 1, np.uint64)
+    L = lib()
+    L.or_palette_pack(_p(attrs), _p(first), k, _p(counts), _p(offs), None)
+    out = np.zeros(max(int(offs[-1]), 1), np.uint32)
+    L.or_palette_pack(_p(attrs), _p(first), k, _p(counts), _p(offs), _p(out))
+    return counts, offs, out[:int(offs[-1])]
