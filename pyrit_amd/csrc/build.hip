@@ -653,10 +653,10 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t li = w0 + i * kWave + lane;
             const uint32_t idx = base + li;
-            const uint64_t from_lane = __shfl_down(k[i], 1, kWave);
-            const uint64_t from_next = __shfl(k[i + 1 < ITEMS ? i + 1 : i], 0, kWave);
-            const uint64_t prev_lane = __shfl_up(k[i], 1, kWave);
-            const uint64_t prev_item = __shfl(k[i > 0 ? i - 1 : 0], kWave - 1, kWave);
+            const uint64_t from_lane = dpp64<kDppWaveShl1>(k[i]);
+            const uint64_t from_next = lane_read64(k[i + 1 < ITEMS ? i + 1 : i], 0);
+            const uint64_t prev_lane = dpp64<kDppWaveShr1>(k[i]);
+            const uint64_t prev_item = lane_read64(k[i > 0 ? i - 1 : 0], kWave - 1);
             f[i] = 0;
             bool rs = false;
             if (li < len) {
@@ -701,7 +701,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
         // carry into the piece: the run reaching into element 0 from before
         if (wave == 0) {
             bool carry = false;
-            const uint64_t k0 = __shfl(k[0], 0, kWave);
+            const uint64_t k0 = lane_read64(k[0], 0);
             // (out-of-range records never head a brick: no look through them)
             if (len > 0 && (k0 & invalid) == 0 && base > seg_start && (key_at(base - 1) >> 6) == (k0 >> 6)) {
                 // records base-1, base-2, ... of the same brick, 64 at a time;
@@ -723,9 +723,9 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
                             vj = vals[j];
                         }
                     }
-                    // key of record j + 1 (the shuffle runs in every lane: a
-                    // shuffle from an inactive lane yields no defined value)
-                    const uint64_t up = __shfl_up(kj, 1, kWave);
+                    // key of record j + 1 (the lane move runs in every lane:
+                    // one from an inactive lane yields no defined value)
+                    const uint64_t up = dpp64<kDppWaveShr1>(kj);
                     const uint64_t nk = lane == 0 ? after : up;
                     const bool same = in && (kj >> 6) == (k0 >> 6);
                     const bool kept = same && nk != kj && (kj & invalid) == 0 && vj != 0;
@@ -735,7 +735,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
                     }
                     const uint64_t sm = __ballot(same);
                     if (sm != ~0ull) break;  // the run starts inside this window
-                    after = __shfl(kj, kWave - 1, kWave);
+                    after = lane_read64(kj, kWave - 1);
                     top -= kWave;
                 }
             }
@@ -791,16 +791,28 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
                     if (first_voxel && e[i] == 0) atomicMin(&first_voxel[td.x], (uint32_t)pos);
                 }
             }
-            // OR of the row's mask bits per brick, merged at each brick's last kept lane
+            // OR of the row's mask bits per brick, merged at each brick's last
+            // kept lane: a prefix OR over lanes of the same brick (bricks are
+            // contiguous in lane order; a lane of another brick, or a record
+            // that is not kept, contributes nothing), steps inside rows of 16
+            // and then across rows. Brick indices are below 2^32 - 1.
             uint64_t m = ok ? 1ull << (k[i] & 63) : 0;
-            const uint64_t bk = ok ? bi : ~0ull;
-#pragma unroll
-            for (int o = 1; o < kWave; o <<= 1) {
-                const uint64_t y = __shfl_up(m, o, kWave);
-                const uint64_t yb = __shfl_up(bk, o, kWave);
-                if (lane >= (uint32_t)o && yb == bk) m |= y;
-            }
-            const uint64_t nbk = __shfl_down(bk, 1, kWave);
+            const uint32_t bk = ok ? (uint32_t)bi : 0xFFFFFFFFu;
+            // a lane without a source reads ~bk: never equal, nothing merged
+#define PYR_SEG_OR_STEP(CTRL, ROWS)                                  \
+    {                                                                \
+        const uint64_t y = dpp64<CTRL, ROWS>(m);                     \
+        const uint32_t yb = dpp32<CTRL, ROWS>(bk, ~bk);              \
+        if (yb == bk) m |= y;                                        \
+    }
+            PYR_SEG_OR_STEP(kDppRowShr1, 0xf)
+            PYR_SEG_OR_STEP(kDppRowShr2, 0xf)
+            PYR_SEG_OR_STEP(kDppRowShr4, 0xf)
+            PYR_SEG_OR_STEP(kDppRowShr8, 0xf)
+            PYR_SEG_OR_STEP(kDppRowBcast15, 0xa)
+            PYR_SEG_OR_STEP(kDppRowBcast31, 0xc)
+#undef PYR_SEG_OR_STEP
+            const uint32_t nbk = dpp32<kDppWaveShl1>(bk, ~bk);
             const bool run_end = lane == kWave - 1 || nbk != bk;
             if (ok && run_end) atomicOr((unsigned long long*)&masks[bi], (unsigned long long)m);
         }
@@ -865,8 +877,7 @@ __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, cons
             }
             owner_min(owner, h, b);
         }
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1;
-        const uint64_t h0 = __shfl(h, leader, kWave);
+        const uint64_t h0 = first_lane64(h);  // the slot of the wave's first active lane
         if (follower) h = h0;  // b0 < b already competes for the owner
         slot[b] = (uint32_t)h;
         ch_count[b] = (uint32_t)__popcll(mask);
@@ -1055,8 +1066,7 @@ __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, Lev
             }
             owner_min(a.owner, h, p);
         }
-        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1;
-        const uint64_t h0 = __shfl(h, leader, kWave);
+        const uint64_t h0 = first_lane64(h);  // the slot of the wave's first active lane
         if (follower) h = h0;
         a.slot[p] = (uint32_t)h;
     }

@@ -65,20 +65,60 @@ __device__ __forceinline__ uint64_t first_lane64(uint64_t v) {
     return ((uint64_t)first_lane32((uint32_t)(v >> 32)) << 32) | (uint64_t)first_lane32((uint32_t)v);
 }
 
+// ---------------------------------------------------------------------------
+// Lane moves with DPP (the data-parallel-primitive operand modifier of gfx9:
+// the VALU reads another lane's register, no LDS instruction and no LDS
+// latency; shuffles compile to ds_bpermute, which goes through the LDS).
+// Every lane of the wave must be active. A lane whose source lane does not
+// exist, or whose row the row mask leaves out, gets `old`.
+// ---------------------------------------------------------------------------
+constexpr int kDppRowShr1 = 0x111;     // lane i <- lane i - 1 inside its row of 16
+constexpr int kDppRowShr2 = 0x112;
+constexpr int kDppRowShr4 = 0x114;
+constexpr int kDppRowShr8 = 0x118;
+constexpr int kDppWaveShl1 = 0x130;    // lane i <- lane i + 1
+constexpr int kDppWaveShr1 = 0x138;    // lane i <- lane i - 1
+constexpr int kDppRowBcast15 = 0x142;  // with row mask 0xa: rows 1 and 3 <- lanes 15 and 47
+constexpr int kDppRowBcast31 = 0x143;  // with row mask 0xc: rows 2 and 3 <- lane 31
+
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint32_t dpp32(uint32_t v, uint32_t old = 0) {
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWS, 0xf, false);
+}
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint64_t dpp64(uint64_t v, uint64_t old = 0) {
+    const uint32_t lo = dpp32<CTRL, ROWS>((uint32_t)v, (uint32_t)old);
+    const uint32_t hi = dpp32<CTRL, ROWS>((uint32_t)(v >> 32), (uint32_t)(old >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+// value of lane `lane` (a constant or wave-uniform index) in every lane
+__device__ __forceinline__ uint32_t lane_read32(uint32_t v, int lane) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, lane);
+}
+__device__ __forceinline__ uint64_t lane_read64(uint64_t v, int lane) {
+    return ((uint64_t)lane_read32((uint32_t)(v >> 32), lane) << 32) | lane_read32((uint32_t)v, lane);
+}
+
+// inclusive prefix sum over the 64 lanes: four steps inside rows of 16,
+// then the row totals
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
-    const uint32_t lane = lane_id();
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) {
-        uint32_t y = __shfl_up(x, o, kWave);
-        if (lane >= (uint32_t)o) x += y;
-    }
+    x += dpp32<kDppRowShr1>(x);
+    x += dpp32<kDppRowShr2>(x);
+    x += dpp32<kDppRowShr4>(x);
+    x += dpp32<kDppRowShr8>(x);
+    x += dpp32<kDppRowBcast15, 0xa>(x);
+    x += dpp32<kDppRowBcast31, 0xc>(x);
     return x;
 }
 
 __device__ __forceinline__ uint64_t wave_sum64(uint64_t x) {
-#pragma unroll
-    for (int o = 1; o < kWave; o <<= 1) x += __shfl_xor(x, o, kWave);
-    return x;
+    x += dpp64<kDppRowShr1>(x);
+    x += dpp64<kDppRowShr2>(x);
+    x += dpp64<kDppRowShr4>(x);
+    x += dpp64<kDppRowShr8>(x);
+    x += dpp64<kDppRowBcast15, 0xa>(x);
+    x += dpp64<kDppRowBcast31, 0xc>(x);
+    return lane_read64(x, kWave - 1);
 }
 
 // ---------------------------------------------------------------------------
@@ -184,7 +224,7 @@ __device__ __forceinline__ uint32_t tile_scan(const uint32_t (&v)[ITEMS], uint32
     for (int i = 0; i < ITEMS; i++) {
         uint32_t inc = wave_inclusive_scan(v[i]);
         excl[i] = carry + inc - v[i];
-        carry += __shfl(inc, kWave - 1, kWave);
+        carry += lane_read32(inc, kWave - 1);
     }
     if (lane == 0) s_wave[wave] = carry;
     __syncthreads();
