@@ -70,7 +70,7 @@ typedef struct PyrhipBuildTuning {
     uint32_t sort_items;   /* keys per thread of a sort tile (256 threads): 8, 12, 16, 20 (default), 24 */
     uint32_t pair_layout;  /* 2 (default): pack key << 32 | attribute when keys fit 32 bits (8-bit digits);
                               1: always separate key and attribute arrays */
-    uint32_t scan_items;   /* records per thread of a unique / brick scan tile: 8 (default), 12, 16, 20 */
+    uint32_t scan_items;   /* records per thread of a unique / brick scan tile: 8, 12 (default), 16, 20 */
     uint32_t hash_load_pct; /* maximum load of the leaf and node hash tables in percent, 25..95 (default 80);
                                the output does not depend on it */
 } PyrhipBuildTuning;

@@ -32,6 +32,7 @@
 #include <atomic>
 #include <chrono>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.h"
@@ -597,25 +598,28 @@ __device__ __forceinline__ uint64_t scan_index(uint32_t tile, int i) {
 // same time, give every kept record its unique-voxel position and its brick
 // index; bricks' masks are OR-reduced per row and merged with one atomicOr
 // per (brick, row).
-template <int ITEMS>
+template <int ITEMS, bool PACKED>
 __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_t* status_heads, Segs g, uint32_t subs,
-                                                           RecordBufs b, uint32_t lbits, bool packed,
+                                                           RecordBufs b, uint32_t lbits,
                                                            uint64_t* brick_key, uint64_t* masks,
                                                            uint32_t* first_voxel, uint64_t brick_cap) {
     constexpr int W = kBlock / kWave;
+    // packed records carry keys of at most 32 bits: compare and move them as such
+    using K = typename std::conditional<PACKED, uint32_t, uint64_t>::type;
+    constexpr K kNone = (K)~0ull;
     __shared__ uint32_t s_tile, s_wave[W], s_wave2[W];
     __shared__ uint64_t s_pre[2];
     __shared__ uint32_t s_any_start[W], s_any_kept[W], s_out[W], s_carry0;
-    const uint64_t invalid = 1ull << lbits;
+    const K invalid = (K)(1ull << lbits);
     const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
     const uint64_t lt = (1ull << lane) - 1;
     const uint32_t fin = su.counters[C_FINAL];
     const uint64_t* keys = b.keys[fin];
     const uint32_t* vals = b.vals[fin];
     uint64_t* ukey = b.keys[fin ^ 1];
-    uint32_t* uval = packed ? b.vals[0] : b.vals[fin ^ 1];
+    uint32_t* uval = PACKED ? b.vals[0] : b.vals[fin ^ 1];
     const uint32_t ntiles = tile_count(g) * subs;
-    auto key_at = [&](uint32_t idx) -> uint64_t { return packed ? keys[idx] >> 32 : keys[idx]; };
+    auto key_at = [&](uint32_t idx) -> K { return PACKED ? (K)(keys[idx] >> 32) : (K)keys[idx]; };
     for (;;) {
         const uint32_t tile = next_tile(su, &s_tile);
         if (tile >= ntiles) break;
@@ -627,17 +631,17 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
         const uint32_t end = seg_end(g, td.x);
         const uint64_t chunk = (uint64_t)td.x << lbits;
         const uint32_t w0 = wave * kWave * ITEMS;  // this wave's first element in the piece
-        uint64_t k[ITEMS];
+        K k[ITEMS];
         uint32_t v[ITEMS], f[ITEMS], e[ITEMS], h[ITEMS], eh[ITEMS];
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t li = w0 + i * kWave + lane;
-            k[i] = ~0ull;
+            k[i] = kNone;
             v[i] = 0;
             if (li < len) {
-                if (packed) {
+                if constexpr (PACKED) {
                     const uint64_t w = keys[base + li];
-                    k[i] = w >> 32;
+                    k[i] = (K)(w >> 32);
                     v[i] = (uint32_t)w;
                 } else {
                     k[i] = keys[base + li];
@@ -645,32 +649,47 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
                 }
             }
         }
-        // the key before this wave's first element (memory) and before the piece
-        const uint64_t wave_prev = w0 < len && base + w0 > seg_start ? key_at(base + w0 - 1) : ~0ull;
+        // keys just outside this wave's elements, loaded together with them:
+        // the one before its first element, the one after its last (the next
+        // wave's first, or the first after the piece), and for wave 0 the 64
+        // records before the piece (a brick run reaching into it)
+        const uint32_t wend = min(len, w0 + (uint32_t)(kWave * ITEMS));
+        const K wave_prev = w0 < len && base + w0 > seg_start ? key_at(base + w0 - 1) : kNone;
+        const K wave_next = w0 < len && base + wend < end ? key_at(base + wend) : kNone;
+        K back_k = kNone;
+        uint32_t back_v = 0;
+        if (wave == 0 && len > 0 && (int64_t)base - 1 - (int64_t)lane >= (int64_t)seg_start) {
+            const uint32_t j = base - 1 - lane;
+            if constexpr (PACKED) {
+                const uint64_t w = keys[j];
+                back_k = (K)(w >> 32);
+                back_v = (uint32_t)w;
+            } else {
+                back_k = keys[j];
+                back_v = vals[j];
+            }
+        }
         // kept flags (next key from the next lane, lane 63: lane 0 of the next item)
         uint64_t run_start_mask[ITEMS], kept_mask[ITEMS];
 #pragma unroll
         for (int i = 0; i < ITEMS; i++) {
             const uint32_t li = w0 + i * kWave + lane;
             const uint32_t idx = base + li;
-            const uint64_t from_lane = dpp64<kDppWaveShl1>(k[i]);
-            const uint64_t from_next = lane_read64(k[i + 1 < ITEMS ? i + 1 : i], 0);
-            const uint64_t prev_lane = dpp64<kDppWaveShr1>(k[i]);
-            const uint64_t prev_item = lane_read64(k[i > 0 ? i - 1 : 0], kWave - 1);
+            const K from_lane = dppx<kDppWaveShl1>(k[i]);
+            const K from_next = lane_read(k[i + 1 < ITEMS ? i + 1 : i], 0);
+            const K prev_lane = dppx<kDppWaveShr1>(k[i]);
+            const K prev_item = lane_read(k[i > 0 ? i - 1 : 0], kWave - 1);
             f[i] = 0;
             bool rs = false;
             if (li < len) {
                 bool last = idx + 1 == end;
                 if (!last) {
-                    uint64_t nk;
-                    if (li + 1 == len || (lane == kWave - 1 && i == ITEMS - 1))
-                        nk = key_at(idx + 1);
-                    else
-                        nk = lane < kWave - 1 ? from_lane : from_next;
+                    // the last element of the wave's share takes the key after it
+                    const K nk = li + 1 == wend ? wave_next : lane < kWave - 1 ? from_lane : from_next;
                     last = nk != k[i];
                 }
                 f[i] = last && (k[i] & invalid) == 0 && v[i] != 0;
-                const uint64_t pk = lane > 0 ? prev_lane : (i > 0 ? prev_item : wave_prev);
+                const K pk = lane > 0 ? prev_lane : (i > 0 ? prev_item : wave_prev);
                 // a new brick run starts here (or the segment starts here)
                 rs = idx == seg_start || (pk >> 6) != (k[i] >> 6);
             }
@@ -701,32 +720,24 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
         // carry into the piece: the run reaching into element 0 from before
         if (wave == 0) {
             bool carry = false;
-            const uint64_t k0 = lane_read64(k[0], 0);
+            const K k0 = lane_read(k[0], 0);
             // (out-of-range records never head a brick: no look through them)
-            if (len > 0 && (k0 & invalid) == 0 && base > seg_start && (key_at(base - 1) >> 6) == (k0 >> 6)) {
-                // records base-1, base-2, ... of the same brick, 64 at a time;
-                // the record after each one decides whether it is the last of its key
+            // (wave 0's wave_prev is the record before the piece)
+            if (len > 0 && (k0 & invalid) == 0 && base > seg_start && (wave_prev >> 6) == (k0 >> 6)) {
+                // records base-1, base-2, ... of the same brick, 64 at a time
+                // (the first window came with the piece); the record after
+                // each one decides whether it is the last of its key
                 int64_t top = (int64_t)base - 1;
-                uint64_t after = k0;  // key of the record just after the window
+                K after = k0;  // key of the record just after the window
+                K kj = back_k;
+                uint32_t vj = back_v;
                 for (;;) {
                     const int64_t j = top - (int64_t)lane;
                     const bool in = j >= (int64_t)seg_start;
-                    uint64_t kj = ~0ull;
-                    uint32_t vj = 0;
-                    if (in) {
-                        if (packed) {
-                            const uint64_t w = keys[j];
-                            kj = w >> 32;
-                            vj = (uint32_t)w;
-                        } else {
-                            kj = keys[j];
-                            vj = vals[j];
-                        }
-                    }
                     // key of record j + 1 (the lane move runs in every lane:
                     // one from an inactive lane yields no defined value)
-                    const uint64_t up = dpp64<kDppWaveShr1>(kj);
-                    const uint64_t nk = lane == 0 ? after : up;
+                    const K up = dppx<kDppWaveShr1>(kj);
+                    const K nk = lane == 0 ? after : up;
                     const bool same = in && (kj >> 6) == (k0 >> 6);
                     const bool kept = same && nk != kj && (kj & invalid) == 0 && vj != 0;
                     if (__ballot(kept)) {
@@ -735,8 +746,21 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
                     }
                     const uint64_t sm = __ballot(same);
                     if (sm != ~0ull) break;  // the run starts inside this window
-                    after = lane_read64(kj, kWave - 1);
+                    after = lane_read(kj, kWave - 1);
                     top -= kWave;
+                    const int64_t jn = top - (int64_t)lane;
+                    kj = kNone;
+                    vj = 0;
+                    if (jn >= (int64_t)seg_start) {
+                        if constexpr (PACKED) {
+                            const uint64_t w = keys[jn];
+                            kj = (K)(w >> 32);
+                            vj = (uint32_t)w;
+                        } else {
+                            kj = keys[jn];
+                            vj = vals[jn];
+                        }
+                    }
                 }
             }
             if (lane == 0) s_carry0 = carry;
@@ -1287,9 +1311,13 @@ void launch_unique_bricks_t(hipStream_t s, uint32_t sort_tile, uint64_t sort_til
     const uint32_t subs = (sort_tile + tile - 1) / tile;
     // brick-head look-back statuses follow the kept-record ones
     uint64_t* status_heads = sc.status + sort_tiles * subs;
-    hipLaunchKernelGGL(k_unique_bricks<ITEMS>, dim3((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048)),
-                       dim3(kBlock), 0, s, sc, status_heads, seg, subs, bufs, lbits, packed, brick_key, masks,
-                       first_dev, nb);
+    const dim3 grid((uint32_t)std::min<uint64_t>(sort_tiles * subs, 2048));
+    if (packed)
+        hipLaunchKernelGGL((k_unique_bricks<ITEMS, true>), grid, dim3(kBlock), 0, s, sc, status_heads, seg, subs, bufs,
+                           lbits, brick_key, masks, first_dev, nb);
+    else
+        hipLaunchKernelGGL((k_unique_bricks<ITEMS, false>), grid, dim3(kBlock), 0, s, sc, status_heads, seg, subs,
+                           bufs, lbits, brick_key, masks, first_dev, nb);
 }
 
 // unique + bricks with scan tiles of 256 * items records

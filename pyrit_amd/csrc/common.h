@@ -99,6 +99,14 @@ __device__ __forceinline__ uint64_t lane_read64(uint64_t v, int lane) {
     return ((uint64_t)lane_read32((uint32_t)(v >> 32), lane) << 32) | lane_read32((uint32_t)v, lane);
 }
 
+// the same for keys of either width
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint32_t dppx(uint32_t v, uint32_t old = 0) { return dpp32<CTRL, ROWS>(v, old); }
+template <int CTRL, int ROWS = 0xf>
+__device__ __forceinline__ uint64_t dppx(uint64_t v, uint64_t old = 0) { return dpp64<CTRL, ROWS>(v, old); }
+__device__ __forceinline__ uint32_t lane_read(uint32_t v, int lane) { return lane_read32(v, lane); }
+__device__ __forceinline__ uint64_t lane_read(uint64_t v, int lane) { return lane_read64(v, lane); }
+
 // inclusive prefix sum over the 64 lanes: four steps inside rows of 16,
 // then the row totals
 __device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {

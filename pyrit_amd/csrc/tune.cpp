@@ -23,7 +23,7 @@
 namespace pyr {
 namespace {
 
-constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 8, 80};
+constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80};
 
 struct Table {
     std::mutex m;

@@ -147,7 +147,7 @@ def test_tuning_argument_checks(hip_device):
 def test_autotune_installs_an_identical_setting(hip_device):
     print("case: autotune single build and batch", flush=True)
     g = _gpu()
-    default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 8, "hash_load_pct": 80}
+    default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80}
     g.tuning_clear()
     try:
         pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
