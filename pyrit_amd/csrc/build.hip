@@ -389,8 +389,16 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     uint64_t* keys_out = a.b.keys[src ^ 1];
     uint32_t* vals_out = a.b.vals[src ^ 1];
     const uint32_t tid = threadIdx.x, lane = lane_id(), wave = tid / kWave;
+    // per-wave lane masks of each digit while ranking one row (zero between
+    // rows); the staging area is free until the ranking is done
+    // (wider digits whose masks do not fit match the digit with RB ballots)
+    constexpr bool kLdsMatch = W * R <= kSortTile;
+    uint64_t* s_match = s_keys;
     if (tid == 0) s_tile = atomicAdd(a.tile_ctr, 1u);
-    for (uint32_t i = tid; i < W * R; i += kBlock) (&s_wave_cnt[0][0])[i] = 0;
+    for (uint32_t i = tid; i < W * R; i += kBlock) {
+        (&s_wave_cnt[0][0])[i] = 0;
+        if constexpr (kLdsMatch) s_match[i] = 0;
+    }
     __syncthreads();
     const uint32_t tile = s_tile;
     if (tile >= tile_count(a.g)) return;  // grids are sized by an upper bound of the tile count
@@ -409,25 +417,41 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         k[i] = ok ? keys_in[td.y + li] : 0;
         if constexpr (!PACKED) v[i] = ok ? vals_in[td.y + li] : 0;
     }
-    // stable ranking inside the wave: lanes with equal digits found by RB
-    // ballots; the lowest peer bumps the wave's counter for that digit
-    volatile uint32_t* cnt = s_wave_cnt[wave];
+    // stable ranking inside the wave: the lanes holding a digit OR their
+    // lane bit into the wave's mask of that digit (one LDS atomic; lanes of
+    // one digit conflict, others do not), read it back and clear it; the
+    // lowest peer bumps the wave's counter for that digit. (Matching the
+    // digit with RB ballots costs about nine VALU operations per bit.)
+    // LDS accesses are relaxed workgroup-scope atomics: the wave's lanes
+    // execute each LDS instruction together and in program order, and
+    // same-address atomics keep that order through the compiler (volatile
+    // would turn them into flat accesses that wait on the vector memory
+    // counter)
     const uint64_t lt = (1ull << lane) - 1;
 #pragma unroll
     for (int i = 0; i < kSortItems; i++) {
         bool ok = wbase + i * kWave + lane < count;
         uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
-        uint64_t peers = __ballot(ok);
+        uint64_t peers;
+        if constexpr (kLdsMatch) {
+            uint64_t* mp = &s_match[wave * R + d];
+            if (ok) __hip_atomic_fetch_or(mp, 1ull << lane, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            peers = ok ? __hip_atomic_load(mp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP) : 0ull;
+            if (ok) __hip_atomic_store(mp, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        } else {
+            peers = __ballot(ok);
 #pragma unroll
-        for (int b = 0; b < RB; b++) {
-            uint64_t m = __ballot((d >> b) & 1);
-            peers &= ((d >> b) & 1) ? m : ~m;
+            for (int b = 0; b < RB; b++) {
+                uint64_t m = __ballot((d >> b) & 1);
+                peers &= ((d >> b) & 1) ? m : ~m;
+            }
         }
-        uint32_t c = cnt[d];
+        uint32_t* cp = &s_wave_cnt[wave][d];
+        const uint32_t c = __hip_atomic_load(cp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         uint32_t below = (uint32_t)__popcll(peers & lt);
         rank[i] = c + below;
-        if (ok && below == 0) cnt[d] = c + (uint32_t)__popcll(peers);
-        __builtin_amdgcn_wave_barrier();
+        if (ok && below == 0)
+            __hip_atomic_store(cp, c + (uint32_t)__popcll(peers), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
     __syncthreads();
     // per digit: wave-exclusive prefixes and the tile total
