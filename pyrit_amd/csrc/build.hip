@@ -353,7 +353,8 @@ struct SortArgs {
     Segs g;
     uint32_t shift;
     const uint32_t* digit_base;  // [segments][radix] first position of each digit in each segment, this pass
-    uint64_t* status;            // [tiles][radix]
+    uint64_t* status;            // [tiles][radix], wide words (epoch-tagged)
+    uint32_t* status32;          // [tiles][radix], narrow words (cleared before the pass)
     uint32_t epoch;
     uint32_t* tile_ctr;
     uint32_t* err;
@@ -366,7 +367,11 @@ constexpr int kPoll = 16;
 
 // PACKED: one u64 word per record, key << 32 | attribute (keys of at most 32
 // bits); the digit is taken at a.shift >= 32 and there is no value stream.
-template <int RB, int ITEMS, bool PACKED>
+// NARROW: look-back status words of 32 bits, [31:30] flag | [29:0] value,
+// for segments of fewer than 2^30 records; the host clears them before the
+// pass (no epoch tag). The look-back reads of a tile are most of what a
+// pass reads beyond its records, so halving the word halves that traffic.
+template <int RB, int ITEMS, bool PACKED, bool NARROW>
 __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     // tile of kBlock * ITEMS keys (the names shadow the defaults above)
     constexpr int kSortItems = ITEMS;
@@ -496,17 +501,32 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     // decoupled look-back per digit over the tiles of the segment: publish
     // the tile count, then the prefix
     const int64_t first = td.w;
+    using SW = typename std::conditional<NARROW, uint32_t, uint64_t>::type;
+    // status word of flag 1 (tile count) or 2 (inclusive prefix), its flag
+    // (0 = not published in this pass) and its value
+    auto make = [&](uint32_t f, uint64_t v) -> SW {
+        if constexpr (NARROW) return (SW)((f << 30) | (uint32_t)v);
+        else return status_word((uint64_t)f << 62, a.epoch, v);
+    };
+    auto flag_of = [&](SW w) -> uint32_t {
+        if constexpr (NARROW) return (uint32_t)(w >> 30);
+        else return status_flag(w, a.epoch);
+    };
+    auto value_of = [&](SW w) -> uint64_t {
+        if constexpr (NARROW) return w & 0x3FFFFFFFu;
+        else return w & kValueMask;
+    };
 #pragma unroll
     for (uint32_t j = 0; j < DPT; j++) {
         const uint32_t d = tid * DPT + j;
-        uint64_t* st = a.status + d;
+        SW* st;
+        if constexpr (NARROW) st = a.status32 + d;
+        else st = a.status + d;
         uint64_t excl = 0;
         if (tile == td.w) {  // first tile of its segment
-            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, tot[j]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st[(uint64_t)tile * R], make(2, tot[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagAgg, a.epoch, tot[j]), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st[(uint64_t)tile * R], make(1, tot[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             // predecessors are polled newest to oldest and summed until an
             // inclusive prefix: kPollFirst in the first round (the previous
             // tile's prefix is often there already), kPoll (independent loads
@@ -515,24 +535,24 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             Wait wt;
             int width = kPollFirst;
             while (t >= first) {
-                uint64_t w[kPoll];
+                SW w[kPoll];
 #pragma unroll
                 for (int q = 0; q < kPoll; q++)
                     w[q] = q < width && t - q >= first
                                ? __hip_atomic_load(&st[(uint64_t)(t - q) * R], __ATOMIC_RELAXED,
                                                    __HIP_MEMORY_SCOPE_AGENT)
-                               : status_word(kFlagPre, a.epoch, 0);  // before the segment: prefix 0
+                               : make(2, 0);  // before the segment: prefix 0
                 int used = 0;
                 bool done = false, wait = false;
 #pragma unroll
                 for (int q = 0; q < kPoll; q++) {
                     if (done || wait || q >= width) continue;
-                    const uint32_t f = status_flag(w[q], a.epoch);
+                    const uint32_t f = flag_of(w[q]);
                     if (f == 0) {
                         wait = true;
                         continue;
                     }
-                    excl += w[q] & kValueMask;
+                    excl += value_of(w[q]);
                     used++;
                     done = f == 2;
                 }
@@ -541,8 +561,8 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
                 width = kPoll;
                 if (wait && !wait_more(wt, a.err, 2u)) break;
             }
-            __hip_atomic_store(&st[(uint64_t)tile * R], status_word(kFlagPre, a.epoch, excl + tot[j]),
-                               __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(&st[(uint64_t)tile * R], make(2, excl + tot[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
         }
         s_dst[d] = (uint64_t)seg_base[d] + excl;
     }
@@ -1302,28 +1322,30 @@ struct Timer {
     }
 };
 
-template <int RB, bool PACKED>
+template <int RB, bool PACKED, bool NARROW>
 void launch_sort_rb(hipStream_t s, uint32_t items, uint32_t tiles, const SortArgs& a) {
     switch (items) {
-        case 8: hipLaunchKernelGGL((k_sort_pass<RB, 8, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
-        case 12: hipLaunchKernelGGL((k_sort_pass<RB, 12, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
-        case 20: hipLaunchKernelGGL((k_sort_pass<RB, 20, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
-        case 24: hipLaunchKernelGGL((k_sort_pass<RB, 24, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
-        default: hipLaunchKernelGGL((k_sort_pass<RB, 16, PACKED>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 8: hipLaunchKernelGGL((k_sort_pass<RB, 8, PACKED, NARROW>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 12: hipLaunchKernelGGL((k_sort_pass<RB, 12, PACKED, NARROW>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 20: hipLaunchKernelGGL((k_sort_pass<RB, 20, PACKED, NARROW>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        case 24: hipLaunchKernelGGL((k_sort_pass<RB, 24, PACKED, NARROW>), dim3(tiles), dim3(kBlock), 0, s, a); break;
+        default: hipLaunchKernelGGL((k_sort_pass<RB, 16, PACKED, NARROW>), dim3(tiles), dim3(kBlock), 0, s, a); break;
     }
 }
 
-// packed records are sorted with 8-bit digits only (the fastest width)
+// packed records are sorted with 8-bit digits only (the fastest width), with
+// narrow look-back words when a.status32 is set
 void launch_sort(hipStream_t s, uint32_t rb, uint32_t items, bool packed, uint32_t tiles, const SortArgs& a) {
     if (packed) {
-        launch_sort_rb<8, true>(s, items, tiles, a);
+        if (a.status32) launch_sort_rb<8, true, true>(s, items, tiles, a);
+        else launch_sort_rb<8, true, false>(s, items, tiles, a);
         return;
     }
     switch (rb) {
-        case 8: launch_sort_rb<8, false>(s, items, tiles, a); break;
-        case 9: launch_sort_rb<9, false>(s, items, tiles, a); break;
-        case 10: launch_sort_rb<10, false>(s, items, tiles, a); break;
-        default: launch_sort_rb<11, false>(s, items, tiles, a); break;
+        case 8: launch_sort_rb<8, false, false>(s, items, tiles, a); break;
+        case 9: launch_sort_rb<9, false, false>(s, items, tiles, a); break;
+        case 10: launch_sort_rb<10, false, false>(s, items, tiles, a); break;
+        default: launch_sort_rb<11, false, false>(s, items, tiles, a); break;
     }
 }
 
@@ -1430,10 +1452,15 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint64_t scan_tiles = (nmax + kScanTile - 1) / kScanTile;
     // sort passes: one word per (tile, digit); unique: one per kScanTile piece of a tile
     // (sort passes: one word per tile and digit; unique/bricks: two per scan piece)
+    // packed records in segments of fewer than 2^30 records: narrow sort
+    // look-back words in an array of their own, cleared before every pass
+    const bool narrow = packed && n < (1u << 30);
     const uint64_t status_words =
-        std::max<uint64_t>(sort_tiles * std::max<uint32_t>(radix, 2 * ((sort_tile + kScanTile - 1) / kScanTile)),
+        std::max<uint64_t>(sort_tiles * std::max<uint32_t>(narrow ? 0u : radix,
+                                                           2 * ((sort_tile + kScanTile - 1) / kScanTile)),
                            scan_tiles);
     uint64_t* status = A.get<uint64_t>(status_words);
+    uint32_t* status32 = narrow ? A.get<uint32_t>(sort_tiles * radix) : nullptr;
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
     uint64_t* table = A.get<uint64_t>(cap);
@@ -1523,8 +1550,9 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // (the device skips passes that move nothing and picks each pass's
     // source buffer, so the buffers are named by index here)
     for (uint32_t p = 0; p < passes && n; p++) {
+        if (narrow) fill32(s, status32, 0, sort_tiles * radix);
         SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + p * rb, digit_base + (uint64_t)p * segs * radix,
-                    status, ++epoch, tile_ctr(), counters + C_ERR};
+                    status, status32, ++epoch, tile_ctr(), counters + C_ERR};
         launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
     }
     tm.mark();
