@@ -134,7 +134,8 @@ def build_section(args, out):
     alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 12.0 * U + 8.0 * U + 4.0 * W + 8.0 * Lc
     out["build"]["algorithmic_bytes"] = alg
     out["build"]["algorithmic_GBps"] = alg / (ms * 1e-3) / 1e9
-    kname = "k_sort_pass<%d,%d,%s>" % (rb, items, "true" if rec == 8 else "false")
+    narrow = phases[-1].get("status_bytes", 8) == 4
+    kname = "k_sort_pass<%d,%d,%s,%s>" % (rb, items, "true" if rec == 8 else "false", "true" if narrow else "false")
     out["roofline"] = {"kernel": kname, "bound": "hbm", "achieved": achieved,
                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                        "traffic": measured_traffic(kname, n), "bytes_per_launch": pass_bytes,

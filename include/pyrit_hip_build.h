@@ -73,6 +73,8 @@ typedef struct PyrhipBuildTuning {
     uint32_t scan_items;   /* records per thread of a unique / brick scan tile: 8, 12 (default), 16, 20 */
     uint32_t hash_load_pct; /* maximum load of the leaf and node hash tables in percent, 25..95 (default 80);
                                the output does not depend on it */
+    uint32_t status_words;  /* sort look-back words: 2 (default) 32 bits when a build of packed records holds
+                               fewer than 2^30 of them, 1 always 64 bits */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
@@ -117,7 +119,7 @@ typedef struct PyrhipTuneReport {
 /* Measures settings on this workload (a single build of n PyrVoxel records at
  * `voxels`, or `batch` when not NULL), each `repeats` times (0: 3) after one
  * untimed run, by coordinate descent over sort_items, scan_items, radix_bits /
- * pair_layout and hash_load_pct; a change must be 2 % faster to be taken. The
+ * pair_layout, hash_load_pct and status_words; a change must be 2 % faster to be taken. The
  * winner is installed for the workload's class. Synchronises `stream`. */
 PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t n, const PyrhipChunkBatch* batch,
                                   uint32_t repeats, void* stream, PyrhipTuneReport* out);
@@ -162,6 +164,8 @@ typedef struct PyrhipBuildTimes {
     /* host milliseconds: allocating device memory, enqueueing the kernels,
      * waiting for the final read-back */
     float host_alloc_ms, host_enqueue_ms, host_wait_ms;
+    uint32_t status_bytes; /* bytes per sort look-back word: 4 or 8 */
+    uint32_t reserved;
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 8): the

@@ -1403,6 +1403,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         return fail(PYR_ERROR_INVALID_ARGUMENT, "scan_items must be 8, 12, 16 or 20");
     const uint32_t load_pct = tun.hash_load_pct;
     if (load_pct < 25 || load_pct > 95) return fail(PYR_ERROR_INVALID_ARGUMENT, "hash_load_pct must be in [25, 95]");
+    if (tun.status_words < 1 || tun.status_words > 2)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "status_words must be 1 or 2");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -1454,7 +1456,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // (sort passes: one word per tile and digit; unique/bricks: two per scan piece)
     // packed records in segments of fewer than 2^30 records: narrow sort
     // look-back words in an array of their own, cleared before every pass
-    const bool narrow = packed && n < (1u << 30);
+    const bool narrow = packed && n < (1u << 30) && tun.status_words == 2;
     const uint64_t status_words =
         std::max<uint64_t>(sort_tiles * std::max<uint32_t>(narrow ? 0u : radix,
                                                            2 * ((sort_tile + kScanTile - 1) / kScanTile)),
@@ -1639,6 +1641,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         g_times.radix_bits = rb;
         g_times.record_bytes = packed ? 8 : 12;
         g_times.sort_items = sort_items;
+        g_times.status_bytes = narrow ? 4 : 8;
         using ms = std::chrono::duration<float, std::milli>;
         g_times.host_alloc_ms = ms(host_t1 - host_t0).count();
         g_times.host_enqueue_ms = ms(host_t2 - host_t1).count();

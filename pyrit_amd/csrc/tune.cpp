@@ -23,7 +23,7 @@
 namespace pyr {
 namespace {
 
-constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80};
+constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2};
 
 struct Table {
     std::mutex m;
@@ -54,11 +54,12 @@ void overlay(PyrhipBuildTuning& t, const PyrhipBuildTuning& o) {
     if (o.pair_layout) t.pair_layout = o.pair_layout;
     if (o.scan_items) t.scan_items = o.scan_items;
     if (o.hash_load_pct) t.hash_load_pct = o.hash_load_pct;
+    if (o.status_words) t.status_words = o.status_words;
 }
 
 bool same_tuning(const PyrhipBuildTuning& a, const PyrhipBuildTuning& b) {
     return a.radix_bits == b.radix_bits && a.sort_items == b.sort_items && a.pair_layout == b.pair_layout &&
-           a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct;
+           a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct && a.status_words == b.status_words;
 }
 
 // what a build produced, enough to tell two settings' outputs apart
@@ -216,14 +217,17 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // coordinate descent: one knob at a time, the others at the best so far;
         // a change must win by 2 % to count as more than noise
         const uint32_t sorts[] = {12, 16, 20, 24}, scans[] = {8, 12, 16, 20}, loads[] = {50, 65, 80, 90};
-        const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {11, 2}};
-        for (int dim = 0; dim < 4; dim++) {
-            for (int v = 0; v < 4; v++) {
+        // (11-bit digits are left out: 2048 look-back words per tile made a
+        // pass ten times slower than 8-bit digits on every workload measured)
+        const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
+        for (int dim = 0; dim < 5; dim++) {
+            for (int v = 0; v < (dim == 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];
                 if (dim == 1) c.scan_items = scans[v];
                 if (dim == 2) c.radix_bits = layouts[v][0], c.pair_layout = layouts[v][1];
                 if (dim == 3) c.hash_load_pct = loads[v];
+                if (dim == 4) c.status_words = 2 - v;
                 if (same_tuning(c, best)) continue;
                 float ms = 0;
                 Signature sig;

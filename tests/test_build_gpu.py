@@ -135,11 +135,24 @@ def test_hash_load_identical(hip_device, load):
     assert np.array_equal(roots, rr) and np.array_equal(first, rf)
 
 
+@pytest.mark.parametrize("status_words", [1, 2])
+def test_lookback_word_widths_identical(hip_device, status_words):
+    # 64-bit epoch-tagged and 32-bit cleared look-back words, single and batched
+    print(f"case: look-back words {status_words}", flush=True)
+    pts = random_points(250000, 9, 33, dup_frac=0.2, delete_frac=0.05, out_frac=0.01)
+    assert_same_build(_gpu().build(9, pts, status_words=status_words), O.gb_build(9, pts), 9)
+    offs = np.array([0, 1000, 90000, 90000, len(pts)], np.uint32)
+    rb_, rr, rf = O.gb_build_chunks(9, pts, offs)
+    bb, roots, first = _gpu().build_chunks(9, pts, offs, status_words=status_words)
+    assert_same_build(bb, rb_, 9)
+    assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+
+
 def test_tuning_argument_checks(hip_device):
     g = _gpu()
     pts = random_points(1000, 5, 3)
     for kw in ({"hash_load_pct": 10}, {"hash_load_pct": 99}, {"pair_layout": 3}, {"radix_bits": 12},
-               {"sort_items": 10}, {"scan_items": 24}):
+               {"sort_items": 10}, {"scan_items": 24}, {"status_words": 3}):
         with pytest.raises(RuntimeError, match="INVALID_ARGUMENT"):
             g.build(5, pts, **kw)
 
@@ -147,7 +160,8 @@ def test_tuning_argument_checks(hip_device):
 def test_autotune_installs_an_identical_setting(hip_device):
     print("case: autotune single build and batch", flush=True)
     g = _gpu()
-    default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80}
+    default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80,
+               "status_words": 2}
     g.tuning_clear()
     try:
         pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
