@@ -194,6 +194,21 @@ typedef struct PyrhipDag {
     uint64_t palette;      /* 0, or 16 attribute values: attrs then holds 4-bit indices, eight per word */
 } PyrhipDag;
 
+/* Path edit (the reference's pyr_k_path_edit, src/device/pathedit.zig): sets
+ * or removes voxels (x, y, z, attribute; 0 removes) of a finished DAG without
+ * a rebuild. Only the paths from the root to the edited bricks are written
+ * again: a new brick goes to leaves[leaf_count ..] and the new nodes to
+ * nodes[node_words ..], up to the given capacities (device arrays; the old
+ * words stay valid). attrs_out (device, voxel_count + count words) receives
+ * the new attribute array in DFS rank order; dag->attrs is read only. Edits
+ * apply in the order given. out_flags: 0 = applied (out_root, out_count are
+ * the new root and voxel count), 2 = the DAG became empty, 4 = the capacities
+ * were too small (nothing usable was written). */
+PYR_API PyrResult pyrhip_path_edit(const PyrhipDag* dag, uint32_t voxel_count, uint32_t node_words,
+                                   uint32_t node_capacity, uint32_t leaf_count, uint32_t leaf_capacity,
+                                   const uint32_t* edits, uint32_t count, uint64_t attrs_out, void* stream,
+                                   uint32_t* out_root, uint32_t* out_count, uint32_t* out_flags);
+
 /* PyrRay[count] -> PyrHit[count] (device pointers) against one DAG, the
  * traversal of src/device/dag.zig:160-351 plus toHit (src/device/trace.zig:
  * 115-118) with instance 0. flags: PYR_TRACE_ANY_HIT / PYR_TRACE_NO_ATTRIBUTE. */

@@ -673,8 +673,9 @@ typedef struct {
     VecU32 out_nodes; VecU64 out_leaves; NodeTable nt;
     /* memo: (level, input offset) -> output offset */
     uint64_t* mkeys; uint32_t* mvals; uint64_t mcap, mused;
-    /* input leaf -> output leaf */
+    /* leaf mask -> output leaf (the full mask ~0 is the map's empty key: kept apart) */
     uint64_t* lkeys; uint32_t* lvals; uint64_t lcap, lused;
+    int64_t full_leaf;
 } Canon;
 
 static int map_get(uint64_t* keys, uint32_t* vals, uint64_t cap, uint64_t key, uint32_t* val) {
@@ -724,11 +725,21 @@ static int64_t canon_node(Canon* c, uint32_t off, uint32_t s) {
     for (uint32_t k = 0; k < n; k++) {
         uint32_t ref = c->nodes[off + 2 + k];
         if (s == 2) {
-            uint32_t li;
-            if (!(c->lcap && map_get(c->lkeys, c->lvals, c->lcap, ref, &li))) {
+            /* equal masks are one leaf, whatever input slots hold them */
+            const uint64_t m = c->leaves[ref];
+            uint32_t li = 0;
+            int found;
+            if (m == ~0ull) {
+                found = c->full_leaf >= 0;
+                if (found) li = (uint32_t)c->full_leaf;
+            } else {
+                found = c->lcap && map_get(c->lkeys, c->lvals, c->lcap, m, &li);
+            }
+            if (!found) {
                 li = (uint32_t)c->out_leaves.n;
-                if (vu64_push(&c->out_leaves, c->leaves[ref])) return -1;
-                if (map_put(&c->lkeys, &c->lvals, &c->lcap, &c->lused, ref, li)) return -1;
+                if (vu64_push(&c->out_leaves, m)) return -1;
+                if (m == ~0ull) c->full_leaf = li;
+                else if (map_put(&c->lkeys, &c->lvals, &c->lcap, &c->lused, m, li)) return -1;
             }
             words[2 + k] = li;
         } else {
@@ -748,6 +759,7 @@ int or_canon(const uint32_t* nodes, const uint64_t* leaves, uint32_t root, uint3
     memset(&c, 0, sizeof(c));
     memset(out, 0, sizeof(*out));
     c.nodes = nodes; c.leaves = leaves; c.log2_size = log2_size;
+    c.full_leaf = -1;
     int64_t r = canon_node(&c, root, log2_size - 1);
     free(c.nt.slots); free(c.mkeys); free(c.mvals); free(c.lkeys); free(c.lvals);
     if (r < 0) { free(c.out_nodes.p); free(c.out_leaves.p); return -1; }

@@ -81,7 +81,9 @@ int  or_gb_build_chunks(uint32_t log2_size, const uint8_t* chunk_log2, uint32_t 
 void or_built_free(OrBuilt* b);
 
 /* Canonical form: DFS from root, post-order interning of remapped tuples in one
- * cross-level table, leaves renumbered by first visit (reproduces dag_builder). */
+ * cross-level table, leaves merged by mask and renumbered by first visit
+ * (reproduces dag_builder; a DAG with repeated leaves or tuples, as path edits
+ * leave behind, maps to the minimal DAG of the same voxels). */
 int  or_canon(const uint32_t* nodes, const uint64_t* leaves, uint32_t root, uint32_t log2_size, OrDag* out);
 
 /* --- traversal (device/dag.zig, device/trace.zig, device/render.zig) ------- */

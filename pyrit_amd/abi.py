@@ -221,6 +221,7 @@ FUNCTIONS = {
     "pyrhip_tuning_set": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
     "pyrhip_tuning_get": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),
     "pyrhip_tuning_clear": (None, []),
+    "pyrhip_path_edit": (i32, [P(PyrhipDag), u32, u32, u32, u32, u32, P(u32), u32, u64, vp, P(u32), P(u32), P(u32)]),
     "pyrhip_palette_pack": (i32, [u64, P(u32), u32, u64, u64, P(u64), P(u32), vp]),
     "pyrhip_downsample": (i32, [u64, u64, u32, u32, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
     "pyrhip_set_timing": (None, [i32]),

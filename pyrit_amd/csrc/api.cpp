@@ -8,6 +8,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -35,6 +36,9 @@ struct Geometry {
     uint32_t voxel_count = 0;
     uint32_t users = 0;  // instances referring to it
     int32_t shared = -1; // >= 0: pool ranges of a chunk batch shared with other geometries
+    // after a path edit: nodes also in the path-edit arena, attributes in a range of its own
+    bool uses_arena = false;
+    uint64_t own_attr_offset = 0, own_attr_count = 0;
 };
 
 // pool ranges of one installed chunk batch; every chunk's geometry refers to
@@ -55,6 +59,9 @@ struct Pool {
     void* base = nullptr;
     uint64_t elem = 0;  // bytes per element
     RangeAlloc ra{0};
+    // path-edit arena: the top eighth of the node and leaf pools, above every
+    // range the allocator hands out, bump-allocated (context.zig:435-443)
+    uint64_t arena_start = 0, arena_next = 0, arena_end = 0;
 };
 
 }  // namespace
@@ -126,6 +133,7 @@ struct PyrContext {
     float* w2v_dev = nullptr;
     uint32_t history = 0;
     uint32_t views = 0;
+    uint32_t arena_users = 0;  // geometries with nodes in the path-edit arena
 };
 
 namespace {
@@ -157,9 +165,12 @@ PyrResult hip_check(hipError_t e, const char* what) {
     return fail(PYR_ERROR_CUDA, "%s: %s", what, hipGetErrorString(e));
 }
 
-PyrResult pool_init(Pool& p, uint64_t bytes, uint64_t elem) {
+PyrResult pool_init(Pool& p, uint64_t bytes, uint64_t elem, bool arena) {
     p.elem = elem;
-    p.ra = RangeAlloc(bytes / elem);
+    const uint64_t total = bytes / elem;
+    p.arena_end = total;
+    p.arena_start = p.arena_next = arena ? total - total / 8 : total;
+    p.ra = RangeAlloc(p.arena_start);
     return hip_check(hipMalloc(&p.base, bytes), "pool allocation");
 }
 
@@ -173,6 +184,14 @@ void release_shared(PyrContext* ctx, int32_t id) {
 }
 
 void release_geometry_ranges(PyrContext* ctx, Geometry& g) {
+    if (g.own_attr_count) {
+        ctx->attrs.ra.release(g.own_attr_offset, g.own_attr_count);
+        g.own_attr_offset = g.own_attr_count = 0;
+    }
+    if (g.uses_arena) {
+        g.uses_arena = false;
+        ctx->arena_users--;
+    }
     if (g.shared >= 0) {
         release_shared(ctx, g.shared);
         g.shared = -1;
@@ -385,6 +404,112 @@ PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, ui
     return PYR_OK;
 }
 
+PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& reqs, const std::vector<uint32_t>& edits,
+                        std::vector<uint8_t>* applied, std::vector<uint32_t>* voxels) {
+    applied->assign(reqs.size(), 0);
+    voxels->assign(reqs.size(), 0);
+    hipStream_t s = ctx->stream;
+    // nothing refers to the arena: start it over (after the work in flight)
+    if (ctx->arena_users == 0 &&
+        (ctx->nodes.arena_next != ctx->nodes.arena_start || ctx->leaves.arena_next != ctx->leaves.arena_start)) {
+        PyrResult r = hip_check(hipStreamSynchronize(s), "path edit");
+        if (r != PYR_OK) return r;
+        ctx->nodes.arena_next = ctx->nodes.arena_start;
+        ctx->leaves.arena_next = ctx->leaves.arena_start;
+    }
+    std::vector<PathEditJob> jobs;
+    std::vector<size_t> job_req;
+    std::vector<uint64_t> caps;
+    std::vector<uint32_t> sorted;
+    std::vector<uint32_t> order;
+    for (size_t i = 0; i < reqs.size(); i++) {
+        const PathEditRequest& q = reqs[i];
+        Geometry* g = geo_of(ctx, q.geometry);
+        // only chunks of installed batches: their node and leaf ranges lie below the arena
+        if (!g || g->shared < 0 || q.count == 0 || q.count > kPathEditMax) continue;
+        if (!(g->data.flags & PYR_GEOMETRY_HAS_ATTRIBUTES) || g->data.log2_size < 3) continue;
+        const uint32_t n = 1u << g->data.log2_size;
+        bool in_range = (uint64_t)q.first + q.count <= edits.size() / 4;
+        for (uint32_t e = 0; in_range && e < q.count; e++) {
+            const uint32_t* v = &edits[((size_t)q.first + e) * 4];
+            in_range = v[0] < n && v[1] < n && v[2] < n;
+        }
+        if (!in_range) continue;
+        const uint64_t levels = g->data.log2_size - 2;
+        const uint64_t node_cap = (uint64_t)q.count * (levels * 10 + 4) + 8, leaf_cap = q.count;
+        if (ctx->nodes.arena_next + node_cap > ctx->nodes.arena_end ||
+            ctx->leaves.arena_next + leaf_cap > ctx->leaves.arena_end)
+            continue;
+        const uint64_t cap = (uint64_t)g->voxel_count + q.count;
+        const int64_t a = ctx->attrs.ra.alloc(cap);
+        if (a < 0) continue;
+        PathEditJob j;
+        j.node_base = g->data.node_offset;
+        j.leaf_base = g->data.leaf_offset;
+        j.root = g->data.root;
+        j.log2_size = g->data.log2_size;
+        j.attr_src = g->data.attribute_offset;
+        j.attr_count = g->voxel_count;
+        j.attr_dst = (uint32_t)a;
+        j.palette = (g->data.flags & PYR_GEOMETRY_PALETTE) ? 1u : 0u;
+        j.pal_src = g->data.palette_offset;
+        j.node_arena = (uint32_t)ctx->nodes.arena_next;
+        j.node_cap = (uint32_t)node_cap;
+        j.leaf_arena = (uint32_t)ctx->leaves.arena_next;
+        j.leaf_cap = (uint32_t)leaf_cap;
+        j.edit_first = (uint32_t)(sorted.size() / 4);
+        j.edit_count = q.count;
+        ctx->nodes.arena_next += node_cap;
+        ctx->leaves.arena_next += leaf_cap;
+        // the kernel takes a job's edits grouped by brick, in order inside a brick
+        order.resize(q.count);
+        for (uint32_t e = 0; e < q.count; e++) order[e] = e;
+        auto brick = [&](uint32_t e) {
+            const uint32_t* v = &edits[((size_t)q.first + e) * 4];
+            return ((uint64_t)(v[0] >> 2) << 42) | ((uint64_t)(v[1] >> 2) << 21) | (v[2] >> 2);
+        };
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return brick(x) < brick(y); });
+        for (uint32_t e : order)
+            sorted.insert(sorted.end(), &edits[((size_t)q.first + e) * 4], &edits[((size_t)q.first + e) * 4] + 4);
+        jobs.push_back(j);
+        job_req.push_back(i);
+        caps.push_back(cap);
+    }
+    if (jobs.empty()) return PYR_OK;
+    uint32_t* attrs = (uint32_t*)ctx->attrs.base;
+    PyrResult r = run_path_edits(jobs, sorted, (uint32_t*)ctx->nodes.base, (uint64_t*)ctx->leaves.base, attrs, attrs, s);
+    for (size_t k = 0; k < jobs.size(); k++) {
+        const PathEditJob& j = jobs[k];
+        Geometry* g = geo_of(ctx, reqs[job_req[k]].geometry);
+        if (r != PYR_OK || j.out_flags || !g) {
+            ctx->attrs.ra.release(j.attr_dst, caps[k]);
+            continue;
+        }
+        if (g->own_attr_count) ctx->attrs.ra.release(g->own_attr_offset, g->own_attr_count);
+        g->own_attr_offset = j.attr_dst;
+        g->own_attr_count = caps[k];
+        g->data.root = j.out_root;
+        g->data.attribute_offset = j.attr_dst;
+        g->data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;  // 32-bit attributes from here on
+        g->data.palette_offset = 0;
+        g->voxel_count = j.out_count;
+        if (!g->uses_arena) {
+            g->uses_arena = true;
+            ctx->arena_users++;
+        }
+        (*applied)[job_req[k]] = 1;
+        (*voxels)[job_req[k]] = j.out_count;
+    }
+    return r;
+}
+
+bool ctx_arena_pressure(PyrContext* ctx) {
+    const Pool& n = ctx->nodes;
+    const Pool& l = ctx->leaves;
+    return (n.arena_next - n.arena_start) * 4 > (n.arena_end - n.arena_start) * 3 ||
+           (l.arena_next - l.arena_start) * 4 > (l.arena_end - l.arena_start) * 3;
+}
+
 }  // namespace pyr
 
 extern "C" {
@@ -433,9 +558,10 @@ PYR_API PyrResult pyr_create(const PyrCreateInfo* info, PyrContext** out) {
     ctx->max_instances = info->max_instances ? info->max_instances : 65536;
     ctx->max_geometries = info->max_geometries ? info->max_geometries : 4096;
     const uint64_t mib = 1ull << 20;
-    if (r == PYR_OK) r = pool_init(ctx->nodes, info->node_pool_bytes ? info->node_pool_bytes : 256 * mib, 4);
-    if (r == PYR_OK) r = pool_init(ctx->leaves, info->leaf_pool_bytes ? info->leaf_pool_bytes : 256 * mib, 8);
-    if (r == PYR_OK) r = pool_init(ctx->attrs, info->attribute_pool_bytes ? info->attribute_pool_bytes : 256 * mib, 4);
+    if (r == PYR_OK) r = pool_init(ctx->nodes, info->node_pool_bytes ? info->node_pool_bytes : 256 * mib, 4, true);
+    if (r == PYR_OK) r = pool_init(ctx->leaves, info->leaf_pool_bytes ? info->leaf_pool_bytes : 256 * mib, 8, true);
+    if (r == PYR_OK)
+        r = pool_init(ctx->attrs, info->attribute_pool_bytes ? info->attribute_pool_bytes : 256 * mib, 4, false);
     if (r == PYR_OK)
         r = hip_check(hipMalloc(&ctx->geo_dev, (uint64_t)ctx->max_geometries * sizeof(PyrGeometryData)), "tables");
     for (int q = 0; q < 2 && r == PYR_OK; q++)

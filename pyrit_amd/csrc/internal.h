@@ -70,6 +70,24 @@ PyrResult palette_pack(const uint32_t* attrs, const std::vector<uint32_t>& lo, c
                        uint32_t* pool, hipStream_t s, PaletteScratch* scratch);
 void palette_release(PaletteScratch* scratch, hipStream_t s);
 
+// Path edits (pathedit.hip): voxel edits applied to a finished DAG by
+// rewriting the root-to-brick paths into an arena. Offsets are in words of
+// the pools passed to run_path_edits; node and leaf references inside the
+// DAG are relative to node_base / leaf_base. A job's edits (x, y, z,
+// attribute; chunk-local) are edits[edit_first ..][4], grouped by brick with
+// their order kept inside a brick; the array is overwritten.
+constexpr uint32_t kPathEmpty = 2;     // the chunk became empty
+constexpr uint32_t kPathOverflow = 4;  // the job's arena share was too small
+struct PathEditJob {
+    uint32_t node_base = 0, leaf_base = 0, root = 0, log2_size = 0;
+    uint32_t attr_src = 0, attr_count = 0, attr_dst = 0, palette = 0;  // palette: source is 4-bit indices
+    uint32_t pal_src = 0, node_arena = 0, node_cap = 0, leaf_arena = 0;
+    uint32_t leaf_cap = 0, edit_first = 0, edit_count = 0, reserved = 0;
+    uint32_t out_root = 0, out_count = 0, out_flags = 0, reserved2 = 0;
+};
+PyrResult run_path_edits(std::vector<PathEditJob>& jobs, const std::vector<uint32_t>& edits, uint32_t* nodes_pool,
+                         uint64_t* leaves_pool, const uint32_t* src_pool, uint32_t* dst_pool, hipStream_t s);
+
 // Context internals used by streamed worlds (api.cpp): a chunk batch is
 // installed into the geometry pools once and every chunk gets a geometry
 // that refers to the batch's shared ranges.
@@ -84,5 +102,21 @@ void ctx_release_batch(PyrContext* ctx, int32_t batch_id);
 uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t batch_id);
 PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t batch_id, uint32_t root, uint64_t attr_word,
                                   bool palette, uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out);
+
+// Path edits of batch chunks (context.zig:1093+): request i applies
+// edits[first .. first + count)[4] (x, y, z, attribute; chunk-local, in the
+// order given) to its geometry. applied[i] = 1 and voxels[i] = the new voxel
+// count on success; a request that does not fit (arena, attribute pool, too
+// many edits) or that empties its chunk is not applied and its chunk must be
+// rebuilt. The arena restarts once no geometry uses it.
+struct PathEditRequest {
+    PyrGeometry geometry;
+    uint32_t first, count;
+};
+constexpr uint32_t kPathEditMax = 4096;  // more edits in one chunk: a rebuild is cheaper (types.zig:602)
+PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& reqs, const std::vector<uint32_t>& edits,
+                        std::vector<uint8_t>* applied, std::vector<uint32_t>* voxels);
+// three quarters of the node or leaf arena used (context.zig:1063-1068)
+bool ctx_arena_pressure(PyrContext* ctx);
 
 }  // namespace pyr

@@ -88,6 +88,35 @@ def test_canon_is_identity_on_builder_output():
     assert np.array_equal(c.nodes, d.nodes) and np.array_equal(c.leaves, d.leaves) and c.root == d.root
 
 
+def test_canon_merges_repeated_leaves_and_tuples():
+    """A DAG that repeats a leaf and a node tuple in fresh slots (what path
+    edits leave behind) canonicalises to the minimal DAG of the same voxels."""
+    d = O.build_dense(6, scene_grid(64))
+    nodes, leaves = d.nodes.copy(), d.leaves.copy()
+    # find a level-2 node (children are leaves): walk down the first children
+    root = d.root
+    node, s = root, 5
+    path = []
+    while s > 2:
+        path.append(node)
+        node = int(nodes[node + 2])
+        s -= 1
+    # copy its first leaf to a new slot and the node to new words pointing at the copy
+    leaf_copy = len(leaves)
+    leaves = np.append(leaves, leaves[nodes[node + 2]])
+    n = bin(int(nodes[node]) & 0xFF).count("1")
+    copy = len(nodes)
+    words = nodes[node:node + 2 + n].copy()
+    words[2] = leaf_copy
+    nodes = np.concatenate([nodes, words])
+    # the parent refers to the copy; equal voxels, one more leaf and tuple
+    parent = path[-1]
+    nodes[parent + 2] = copy
+    assert len(leaves) == len(d.leaves) + 1
+    c = O.canon(nodes, leaves, root, 6)
+    assert c.same(d) and np.array_equal(c.leaves, d.leaves)
+
+
 @pytest.mark.parametrize("log2,seed", [(3, 1), (5, 2), (6, 3), (7, 4)])
 def test_serial_gpu_build_matches_cpu_builder(log2, seed):
     # cpu_test.zig:758-807: gpu build (serial) == cpu builder by canonical
