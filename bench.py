@@ -299,10 +299,19 @@ def batch_section(args, out):
         batched()
     b_ms = ev.stop() / reps
     # palette packing of the batch's attributes, as a world installs a batch
-    # (16 entries + 4-bit indices per chunk with at most 16 distinct values)
+    # (16 entries + 4-bit indices per chunk with at most 16 distinct values);
+    # this build also reports the phases for the batch's algorithmic bytes
+    gpu.set_timing(True)
     abi.check(L, L.pyrhip_build_chunks(top, C.byref(batch), None, None, C.byref(built),
                                        roots.ctypes.data_as(C.POINTER(C.c_uint32)),
                                        first.ctypes.data_as(C.POINTER(C.c_uint32))), "batch")
+    bt = gpu.last_build_times()
+    gpu.set_timing(False)
+    # algorithmic bytes as for the single build: encode 16N read + rN write,
+    # P moving passes x 2rN, unique rN read + 12U write, bricks 8U, outputs
+    rec, passes = bt["record_bytes"], max(1, bt["sort_passes"])
+    U, W, Lc = built.voxel_count, built.node_words, built.leaf_count
+    b_alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 20.0 * U + 4.0 * W + 8.0 * Lc
     nonempty = roots != 0xFFFFFFFF
     pfirst = np.ascontiguousarray(np.append(first[nonempty], built.voxel_count), np.uint32)
     k = len(pfirst) - 1
@@ -346,6 +355,9 @@ def batch_section(args, out):
         "workload": w.name, "chunks": count, "voxels": n, "batched_ms": b_ms,
         "batched_gvoxels_per_s": n / (b_ms * 1e-3) / 1e9, "per_chunk_ms": p_ms,
         "per_chunk_gvoxels_per_s": n / (p_ms * 1e-3) / 1e9, "speedup_vs_per_chunk": p_ms / b_ms,
+        "sort_passes": passes, "record_bytes": rec, "unique": U, "algorithmic_bytes": b_alg,
+        "algorithmic_GBps": b_alg / (b_ms * 1e-3) / 1e9, "hbm_frac": b_alg / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
+        "phase_ms": {k: bt[k] for k in ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "per_chunk_latency_ms_p50": float(np.percentile(lat, 50)),
         "per_chunk_latency_ms_p99": float(np.percentile(lat, 99)),
         "autotune": tune,
