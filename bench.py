@@ -72,7 +72,7 @@ def build_section(args, out):
     from pyrit_amd import abi, gpu, lib, workloads
     from pyrit_amd.device import Events, synchronize
 
-    w = workloads.config2(log2_size=args.log2)
+    w = workloads.heightfield_grid(log2_size=args.log2)
     log(f"generating {w.name} on the device")
     vox, offs, n = w.generate_device()
     log(f"{n} voxels resident ({n * 16 / 1e9:.2f} GB)")
@@ -173,13 +173,13 @@ def trace_section(args, out):
     from pyrit_amd.device import Buffer, Events, synchronize
 
     log2 = 12 if not args.quick else 10
-    w = workloads.config4(log2_size=log2, caves=20000 if log2 == 12 else 2000)
+    w = workloads.skin_grid(log2_size=log2, caves=20000 if log2 == 12 else 2000)
     log(f"generating {w.name}")
     vox, offs, n = w.generate_device()
     L = lib()
     built = abi.PyrhipBuilt()
     t0 = time.time()
-    abi.check(L, L.pyrhip_build(log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build config4")
+    abi.check(L, L.pyrhip_build(log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build skin DAG")
     log(f"built {n} skin voxels in {time.time() - t0:.2f}s: {built.node_words} words, {built.leaf_count} leaves")
     vox.free()
     dag = abi.PyrhipDag()
@@ -263,7 +263,7 @@ def batch_section(args, out):
     from pyrit_amd.device import Buffer, Events, synchronize, to_device
 
     count = 1024 if not args.quick else 128
-    w = workloads.config3(count=count)
+    w = workloads.uneven_chunks(count=count)
     log(f"generating {w.name}")
     vox, offs, n = w.generate_device()
     sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
@@ -374,7 +374,7 @@ def serve_section(args, out):
     from pyrit_amd import serving, workloads
 
     count = 1024 if not args.quick else 128
-    w = workloads.config3(count=count, seed=31)
+    w = workloads.uneven_chunks(count=count, seed=31)
     log(f"serving: generating {w.name} inputs")
     vox, offs, n = w.generate_device()
     inputs = [(w.chunks[c].log2_size, vox.ptr + int(offs[c]) * 16, int(offs[c + 1] - offs[c])) for c in range(count)]
@@ -415,7 +415,7 @@ def cpu_section(args, out):
     import oracle_py as O
     from pyrit_amd import workloads
 
-    w = workloads.config2(log2_size=8, caves=2000 // 64)
+    w = workloads.heightfield_grid(log2_size=8, caves=2000 // 64)
     vox, _ = w.generate_host()
     reps, t_total, nvox = 0, 0.0, 0
     t_end = time.time() + (12.0 if not args.quick else 3.0)
@@ -461,7 +461,7 @@ def probe_section(args, out):
 def main() -> int:
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1",
+        print(json.dumps({"error": "bench.py measures one GPU only (the build and traversal kernels of one device); run with --gpus 1",
                           "n_gpus": args.gpus}), flush=True)
         return 2
     rank = int(os.environ.get("RANK", "0"))

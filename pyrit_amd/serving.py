@@ -103,7 +103,7 @@ def random_edits(rng: np.random.Generator, log2_size: int, count: int) -> np.nda
 
 def poisson_schedule(inputs, rate: float, duration: float, seed: int = 5, edit_frac: float = 0.2):
     """Arrival offsets (s) and requests of a Poisson stream, generated before
-    the run so that the driver only submits: ("build", j) rebuilds chunk j
+    the run so that the load generator only submits: ("build", j) rebuilds chunk j
     (uniform over inputs) from inputs[j]; ("edit", j, records) edits a chunk
     an earlier request of the stream built."""
     rng = np.random.default_rng(seed)

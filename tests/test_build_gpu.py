@@ -27,11 +27,11 @@ def assert_same_build(b, ref, log2):
     assert np.array_equal(b.keys(), ref["keys"])
 
 
-def test_config1_bit_exact(hip_device):
+def test_terrain_chunk_64_bit_exact(hip_device):
     from pyrit_amd import workloads
 
-    print("case: config1 64^3 build", flush=True)
-    vox, _ = workloads.config1().generate_host()
+    print("case: terrain chunk 64^3 build", flush=True)
+    vox, _ = workloads.terrain_chunk64().generate_host()
     ref = O.gb_build(6, vox)
     b = _gpu().build(6, vox)
     assert_same_build(b, ref, 6)
@@ -295,7 +295,7 @@ def test_mixed_size_batch(hip_device):
     print("case: mixed-size batch", flush=True)
     from pyrit_amd import workloads
 
-    w = workloads.config3(seed=5, count=24, edges=(8, 16, 32, 64))
+    w = workloads.uneven_chunks(seed=5, count=24, edges=(8, 16, 32, 64))
     vox, offs64 = w.generate_host()
     offs = offs64.astype(np.uint32)
     sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
@@ -320,8 +320,8 @@ def test_device_generator_matches_host(hip_device):
     print("case: terrain generator host == device", flush=True)
     from pyrit_amd import workloads
 
-    for w in (workloads.config1(), workloads.config3(seed=8, count=16, edges=(8, 16, 32, 64)),
-              workloads.config2(log2_size=6, caves=40), workloads.config4(log2_size=7, caves=30)):
+    for w in (workloads.terrain_chunk64(), workloads.uneven_chunks(seed=8, count=16, edges=(8, 16, 32, 64)),
+              workloads.heightfield_grid(log2_size=6, caves=40), workloads.skin_grid(log2_size=7, caves=30)):
         hv, ho = w.generate_host()
         dv, do, total = w.generate_device()
         assert total == len(hv) and np.array_equal(do, ho)

@@ -105,7 +105,7 @@ def test_resident_chunk_traces_like_the_oracle(hip_device):
     from pyrit_amd.device import to_device
     from pyrit_amd.serving import Server
 
-    w = workloads.config3(seed=12, count=6, edges=(16, 32, 64), probs=(0.4, 0.3, 0.3))
+    w = workloads.uneven_chunks(seed=12, count=6, edges=(16, 32, 64), probs=(0.4, 0.3, 0.3))
     vox, offs = w.generate_host()
     sv = Server(workers=2)
     bufs = []

@@ -32,11 +32,11 @@ def product_camera(eye, target, up, fov, w, h):
     return c
 
 
-def test_config1_primary_rays(hip_device):
-    print("case: config1 256x256 primary rays", flush=True)
+def test_terrain_chunk_64_primary_rays(hip_device):
+    print("case: terrain chunk 64^3, 256x256 primary rays", flush=True)
     from pyrit_amd import gpu, workloads
 
-    vox, _ = workloads.config1().generate_host()
+    vox, _ = workloads.terrain_chunk64().generate_host()
     b = gpu.build(6, vox)
     eye, tgt, up = (-32, 96, -32), (32, 32, 32), (0, 1, 0)
     pc = product_camera(eye, tgt, up, np.pi / 3, 256, 256)
@@ -99,7 +99,7 @@ def test_shadow_rays(hip_device):
     print("case: shadow rays", flush=True)
     from pyrit_amd import gpu, workloads
 
-    vox, _ = workloads.config1().generate_host()
+    vox, _ = workloads.terrain_chunk64().generate_host()
     b = gpu.build(6, vox)
     c = O.camera((-32, 96, -32), (32, 32, 32), (0, 1, 0), np.pi / 3, 128, 128)
     rays = O.camera_rays(c)
