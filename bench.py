@@ -340,6 +340,7 @@ def batch_section(args, out):
                     "ms": pal_ms, "note": "scan + pack incl. the count read-back; algorithmic bytes 2 x 4U read + "
                                           "4 x words write",
                     "GBps": (8.0 * U + 4.0 * words) / (pal_ms * 1e-3) / 1e9}
+    graph_info = graph_measure(top, vox, offs, sizes)
     # one build call per chunk on the same GPU (same inputs)
     lat = []
     ev.start()
@@ -362,9 +363,86 @@ def batch_section(args, out):
         "per_chunk_latency_ms_p99": float(np.percentile(lat, 99)),
         "autotune": tune,
         "palette": palette_info,
+        "graph": graph_info,
     }
     for b in (vox, d_offs, d_sizes):
         b.free()
+
+
+def graph_measure(top, vox, offs, sizes):
+    """The batch build replayed from a HIP graph (captured once, buffers
+    allocated once, inputs already in the graph's input buffer) against the
+    eager batch build with its inputs on the device: the whole batch, its
+    first 16 chunks, and 16 edit-sized chunks where launch overhead counts."""
+    from pyrit_amd import abi, gpu, lib
+    from pyrit_amd.device import hip, synchronize, to_device
+
+    L = lib()
+    # 16 chunks of 32^3 holding 4096 scattered voxels each
+    rng = np.random.default_rng(12)
+    small = np.zeros(16 * 4096, abi.VOXEL)
+    for f in ("x", "y", "z"):
+        small[f] = rng.integers(0, 32, len(small))
+    small["attribute"] = rng.integers(1, 200, len(small))
+    small_dev = to_device(small)
+    small_offs = (np.arange(17) * 4096).astype(np.uint32)
+    res = {}
+    cases = (("full", top, vox, offs, sizes, len(sizes)), ("16_chunks", top, vox, offs, sizes, 16),
+             ("16_chunks_of_4096_voxels", 5, small_dev, small_offs, None, 16))
+    for label, l2, src, src_offs, src_sizes, k in cases:
+        o = np.ascontiguousarray(src_offs[:k + 1], np.uint32)
+        total = int(o[-1])
+        sz = np.ascontiguousarray(src_sizes[:k], np.uint8) if src_sizes is not None else None
+        d_o = to_device(o)
+        d_sz = to_device(sz) if sz is not None else None
+        batch = abi.PyrhipChunkBatch()
+        batch.count, batch.capacity, batch.voxels, batch.offsets = k, 0, src.ptr, d_o.ptr
+        batch.total, batch.chunk_log2 = total, d_sz.ptr if d_sz else 0
+        roots_e = np.zeros(k, np.uint32)
+        first_e = np.zeros(k, np.uint32)
+        built = abi.PyrhipBuilt()
+
+        def eager():
+            abi.check(L, L.pyrhip_build_chunks(l2, C.byref(batch), None, None, C.byref(built),
+                                               roots_e.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                               first_e.ctypes.data_as(C.POINTER(C.c_uint32))), "batch")
+
+        g = gpu.BuildGraph(l2, k, total, mixed_sizes=sz is not None)
+        try:
+            # inputs resident in the graph's buffer, as a loop producing them in place
+            if hip().hipMemcpy(C.c_void_p(g.voxels_ptr()), C.c_void_p(src.ptr), C.c_size_t(total * 16), 3) != 0:
+                raise RuntimeError("hipMemcpy")
+            g.launch(None, o, sz)
+            reps = 3 if k > 16 else 50
+            synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                out_g, roots_g, first_g = g.launch(None, o, sz)
+            g_ms = (time.perf_counter() - t0) * 1e3 / reps
+            eager()
+            same = (built.root == out_g.root and built.node_words == out_g.node_words
+                    and built.leaf_count == out_g.leaf_count and np.array_equal(roots_e, roots_g)
+                    and np.array_equal(first_e, first_g))
+            L.pyrhip_built_free(C.byref(built), None)
+            synchronize()
+            # both end in the counter read-back the host waits for
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                eager()
+                L.pyrhip_built_free(C.byref(built), None)
+            synchronize()
+            e_ms = (time.perf_counter() - t0) * 1e3 / reps
+        finally:
+            g.destroy()
+            d_o.free()
+            if d_sz:
+                d_sz.free()
+        res[label] = {"chunks": k, "voxels": total, "graph_ms": g_ms, "eager_ms": e_ms,
+                      "graph_gvoxels_per_s": total / (g_ms * 1e-3) / 1e9, "speedup": e_ms / g_ms,
+                      "same_output": bool(same)}
+        log(f"graph {label}: {g_ms:.3f} ms vs eager {e_ms:.3f} ms ({k} chunks, {total} voxels), same={same}")
+    small_dev.free()
+    return res
 
 
 def serve_section(args, out):

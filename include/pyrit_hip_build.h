@@ -95,6 +95,32 @@ PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch
 
 PYR_API void pyrhip_built_free(PyrhipBuilt* built, void* stream);
 
+/* ---- chunk-batch builds captured into a HIP graph (no counterpart in the
+ * reference, which launches every kernel of gpu_build.zig:181-336 one by
+ * one) ---------------------------------------------------------------------
+ * The whole build of a batch of `count` chunks holding up to `total_max`
+ * voxels (capacity layout when capacity > 0, packed otherwise; per-chunk
+ * edges given at launch when mixed_sizes != 0) is captured once, with every
+ * buffer allocated once; a launch uploads the offsets, copies the voxels
+ * into the graph's input buffer (skipped when they are already there, see
+ * pyrhip_graph_voxels), replays the graph and reads the counters back. The
+ * kernels take their sizes from the device offsets, so any batch of at most
+ * `count` chunks (trailing chunks empty) and `total_max` voxels replays the
+ * same graph. The outputs live in the graph's buffers: valid until the next
+ * launch or pyrhip_graph_destroy, never passed to pyrhip_built_free. One
+ * launch at a time per graph. */
+typedef struct PyrhipGraph PyrhipGraph;
+PYR_API PyrResult pyrhip_graph_create(uint32_t log2_size, uint32_t count, uint32_t capacity, uint32_t total_max,
+                                      int32_t mixed_sizes, const PyrhipBuildTuning* tuning, PyrhipGraph** out);
+/* device address of the graph's voxel input buffer */
+PYR_API uint64_t pyrhip_graph_voxels(PyrhipGraph* graph);
+/* offsets: host u32[count + 1]; chunk_log2: host u8[count] for mixed sizes, else NULL; voxels: device
+ * (0 = already in the input buffer) */
+PYR_API PyrResult pyrhip_graph_launch(PyrhipGraph* graph, uint64_t voxels, const uint32_t* offsets,
+                                      const uint8_t* chunk_log2, void* stream, PyrhipBuilt* out, uint32_t* roots,
+                                      uint32_t* first_voxel);
+PYR_API void pyrhip_graph_destroy(PyrhipGraph* graph);
+
 /* gpu_build.downsample (src/gpu_build.zig:124-131): DAG of edge
  * 2^(log2_src - shift) from n_src sorted unique voxels (src_keys/src_vals of
  * an editable build); every source voxel moves to its coarse cell and, per

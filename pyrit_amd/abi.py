@@ -15,6 +15,15 @@ RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), 
 HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
 
 PYR_OK = 0
+PYR_ERROR_INVALID_ARGUMENT = -1
+PYR_ERROR_INVALID_HANDLE = -2
+PYR_ERROR_OUT_OF_MEMORY = -3
+PYR_ERROR_CAPACITY = -4
+PYR_ERROR_IN_USE = -5
+PYR_ERROR_CUDA = -6
+PYR_ERROR_COMPILE = -7
+PYR_ERROR_NOT_FOUND = -8
+PYR_ERROR_VERSION = -9
 PYR_FLT_MAX = np.float32(3.402823466e38)
 PYR_NO_HIT = 0xFFFFFFFF
 PYR_HIT_INSIDE = 0x20
@@ -216,6 +225,10 @@ FUNCTIONS = {
     "pyrhip_build_chunks": (i32, [u32, P(PyrhipChunkBatch), P(PyrhipBuildTuning), vp, P(PyrhipBuilt), P(u32),
                                   P(u32)]),
     "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),
+    "pyrhip_graph_create": (i32, [u32, u32, u32, u32, i32, P(PyrhipBuildTuning), P(vp)]),
+    "pyrhip_graph_voxels": (u64, [vp]),
+    "pyrhip_graph_launch": (i32, [vp, u64, P(u32), P(u8), vp, P(PyrhipBuilt), P(u32), P(u32)]),
+    "pyrhip_graph_destroy": (None, [vp]),
     "pyrhip_decode_voxels": (i32, [u64, u64, u32, u32, u64, vp]),
     "pyrhip_autotune": (i32, [u32, u64, u32, P(PyrhipChunkBatch), u32, vp, P(PyrhipTuneReport)]),
     "pyrhip_tuning_set": (i32, [u32, u32, u32, P(PyrhipBuildTuning)]),

@@ -32,6 +32,7 @@
 #include <atomic>
 #include <chrono>
 #include <mutex>
+#include <new>
 #include <type_traits>
 #include <vector>
 
@@ -113,14 +114,15 @@ __device__ __forceinline__ uint32_t seg_end(const Segs& g, uint32_t seg) { retur
 
 // tile_first[s] = tiles of segments before s (one block; segments are
 // scanned 256 at a time), tile_first[count] = *total = the tile count.
-// Offsets that are not a partition of [0, n) raise error bit 32 and leave no
-// tiles, so no later kernel touches a record.
+// Offsets that are not a partition of [0, n) (of [0, m) with m <= n when
+// `at_most`: a graph sized for n records replays smaller batches) raise error
+// bit 32 and leave no tiles, so no later kernel touches a record.
 __global__ void __launch_bounds__(kBlock) k_seg_tiles(const uint32_t* off, uint32_t count, uint32_t n,
-                                                       uint32_t tile, uint32_t* tile_first, uint32_t* total,
-                                                       uint32_t* err) {
+                                                       uint32_t at_most, uint32_t tile, uint32_t* tile_first,
+                                                       uint32_t* total, uint32_t* err) {
     __shared__ uint32_t s_wave[kBlock / kWave];
     __shared__ uint32_t s_bad;
-    if (threadIdx.x == 0) s_bad = off[0] != 0 || off[count] != n;
+    if (threadIdx.x == 0) s_bad = off[0] != 0 || (at_most ? off[count] > n : off[count] != n);
     __syncthreads();
     uint32_t carry = 0;
     for (uint32_t b = 0; b < count; b += kBlock) {
@@ -1306,15 +1308,16 @@ struct Timer {
     hipEvent_t ev[8];
     int n = 0;
     hipStream_t s;
+    bool on = g_timing;
     void mark() {
-        if (!g_timing) return;
+        if (!on) return;
         hipEventCreate(&ev[n]);
         hipEventRecord(ev[n], s);
         n++;
     }
     float between(int a, int b) {
         float ms = 0;
-        if (g_timing && b < n) hipEventElapsedTime(&ms, ev[a], ev[b]);
+        if (on && b < n) hipEventElapsedTime(&ms, ev[a], ev[b]);
         return ms;
     }
     ~Timer() {
@@ -1380,9 +1383,27 @@ void launch_unique_bricks(hipStream_t s, uint32_t items, uint32_t sort_tile, uin
 
 }  // namespace
 
-// One build: single geometry (enc.chunk_count == 0) or chunk batch.
+// A chunk-batch build captured into a HIP graph (pyrhip_graph_*): every
+// buffer is allocated once for the largest batch, the kernels take their
+// sizes from the device arrays (offsets, counters), and a replay ends with
+// the counters, roots and first voxels copied into pinned host memory.
+struct GraphBuild {
+    std::vector<void*> scratch;
+    uint64_t* kb[2] = {nullptr, nullptr};
+    uint32_t* vb[2] = {nullptr, nullptr};
+    uint64_t* leaves = nullptr;
+    uint32_t* nodes = nullptr;
+    uint32_t* host = nullptr;  // pinned: counters [C_TILE], roots [count], first [count]
+    bool packed = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+};
+
+// One build: single geometry (enc.chunk_count == 0) or chunk batch. With
+// `gb` (batches only) the launches are captured into a graph on `s` instead
+// of run, and the buffers move to gb.
 PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBuildTuning* tuning, hipStream_t s,
-                    PyrhipBuilt* out, uint32_t* roots_host, uint32_t* first_host) {
+                    PyrhipBuilt* out, uint32_t* roots_host, uint32_t* first_host, GraphBuild* gb = nullptr) {
     memset(out, 0, sizeof(*out));
     EncodeArgs enc = enc_in;
     const uint64_t n64 = (uint64_t)enc.n_old + enc.n_new;
@@ -1510,8 +1531,17 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     }
 
     const auto host_t1 = std::chrono::steady_clock::now();  // allocations done
+    // (a graph replays on other streams: the waits that reused cached blocks
+    // put on `s` must be over before capture)
+    if (gb && (hipStreamSynchronize(s) != hipSuccess || hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess)) {
+        (void)hipGetLastError();
+        A.release();
+        free_outputs();
+        return fail(PYR_ERROR_CUDA, "graph capture could not start");
+    }
     Timer tm;
     tm.s = s;
+    if (gb) tm.on = false;
     tm.mark();
     // zero state
     fill32(s, counters, 0, C_COUNT);
@@ -1532,8 +1562,8 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     enc.packed = packed;
     if (n) {
         if (enc.chunk_count) {
-            hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(kBlock), 0, s, enc.offsets, enc.chunk_count, n, sort_tile,
-                               tile_first, counters + C_TILES, counters + C_ERR);
+            hipLaunchKernelGGL(k_seg_tiles, dim3(1), dim3(kBlock), 0, s, enc.offsets, enc.chunk_count, n,
+                               gb ? 1u : 0u, sort_tile, tile_first, counters + C_TILES, counters + C_ERR);
             hipLaunchKernelGGL(k_seg_desc, dim3(grid_for(enc.chunk_count, kBlock / kWave, 1024)), dim3(kBlock), 0, s,
                                enc.offsets, tile_first, enc.chunk_count, sort_tile, counters + C_TILES,
                                (uint4*)seg.desc);
@@ -1616,13 +1646,43 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                        nodes_out, (uint32_t)std::min(set_cap[ch], np2), enc.chunk_count, word_bound);
     tm.mark();
 
-    // single read-back (gpu_build.zig:338-344)
+    // single read-back (gpu_build.zig:338-344); a graph copies into its pinned buffer
     uint32_t c[C_TILE];
-    hipError_t e = hipMemcpyAsync(c, counters, sizeof(c), hipMemcpyDeviceToHost, s);
+    uint32_t* const hc = gb ? gb->host : c;
+    if (gb) {
+        roots_host = gb->host + C_TILE;
+        first_host = roots_host + enc.chunk_count;
+    }
+    hipError_t e = hipMemcpyAsync(hc, counters, sizeof(c), hipMemcpyDeviceToHost, s);
     if (enc.chunk_count && e == hipSuccess) {
         e = hipMemcpyAsync(roots_host, roots_dev, enc.chunk_count * 4, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess)
             e = hipMemcpyAsync(first_host, first_dev, enc.chunk_count * 4, hipMemcpyDeviceToHost, s);
+    }
+    if (gb) {
+        hipGraph_t graph = nullptr;
+        const hipError_t ce = hipStreamEndCapture(s, &graph);
+        if (e == hipSuccess) e = ce;
+        hipGraphExec_t exec = nullptr;
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            if (graph) (void)hipGraphDestroy(graph);
+            A.release();
+            free_outputs();
+            return fail(PYR_ERROR_CUDA, "graph capture: %s", hipGetErrorString(e));
+        }
+        gb->graph = graph;
+        gb->exec = exec;
+        gb->scratch.swap(A.ptrs);
+        for (int q = 0; q < 2; q++) {
+            gb->kb[q] = kb[q];
+            gb->vb[q] = vb[q];
+        }
+        gb->leaves = leaves_out;
+        gb->nodes = nodes_out;
+        gb->packed = packed;
+        return PYR_OK;
     }
     A.release();
     const auto host_t2 = std::chrono::steady_clock::now();  // everything enqueued
@@ -1712,6 +1772,136 @@ PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch
     while ((1u << bits) < batch->count) bits++;
     enc.chunk_bits = bits;
     return run_build(enc, log2_size, tuning, (hipStream_t)stream, out, roots, first_voxel);
+}
+
+struct PyrhipGraph {
+    pyr::GraphBuild gb;
+    uint32_t log2_size = 0, count = 0, capacity = 0, total_max = 0;
+    bool mixed = false;
+    hipStream_t stream = nullptr;  // the capture stream
+    void* voxels = nullptr;        // inputs the graph reads
+    uint32_t* offsets = nullptr;
+    uint8_t* chunk_log2 = nullptr;
+};
+
+PYR_API void pyrhip_graph_destroy(PyrhipGraph* g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    if (g->gb.exec) (void)hipGraphExecDestroy(g->gb.exec);
+    if (g->gb.graph) (void)hipGraphDestroy(g->gb.graph);
+    for (void* p : g->gb.scratch) pool_free(p, g->stream);
+    for (int q = 0; q < 2; q++) {
+        pool_free(g->gb.kb[q], g->stream);
+        pool_free(g->gb.vb[q], g->stream);
+    }
+    pool_free(g->gb.leaves, g->stream);
+    pool_free(g->gb.nodes, g->stream);
+    if (g->gb.host) (void)hipHostFree(g->gb.host);
+    (void)hipFree(g->voxels);
+    (void)hipFree(g->offsets);
+    (void)hipFree(g->chunk_log2);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    (void)hipGetLastError();
+    delete g;
+}
+
+PYR_API PyrResult pyrhip_graph_create(uint32_t log2_size, uint32_t count, uint32_t capacity, uint32_t total_max,
+                                      int32_t mixed_sizes, const PyrhipBuildTuning* tuning, PyrhipGraph** out) {
+    diag_clear();
+    if (!out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    *out = nullptr;
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if (count == 0 || total_max == 0) return fail(PYR_ERROR_INVALID_ARGUMENT, "empty graph batch");
+    if (capacity && (uint64_t)count * capacity < total_max)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "total_max exceeds count * capacity");
+    const uint64_t records = capacity ? (uint64_t)count * capacity : total_max;
+    PyrhipGraph* g = new (std::nothrow) PyrhipGraph();
+    if (!g) return fail(PYR_ERROR_OUT_OF_MEMORY, "graph");
+    g->log2_size = log2_size;
+    g->count = count;
+    g->capacity = capacity;
+    g->total_max = total_max;
+    g->mixed = mixed_sizes != 0;
+    bool ok = hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipMalloc(&g->voxels, records * sizeof(PyrVoxel)) == hipSuccess;
+    ok = ok && hipMalloc((void**)&g->offsets, ((uint64_t)count + 1) * 4) == hipSuccess;
+    ok = ok && (!g->mixed || hipMalloc((void**)&g->chunk_log2, count) == hipSuccess);
+    ok = ok && hipHostMalloc((void**)&g->gb.host, ((uint64_t)C_TILE + 2ull * count) * 4, 0) == hipSuccess;
+    if (!ok) {
+        (void)hipGetLastError();
+        pyrhip_graph_destroy(g);
+        return fail(PYR_ERROR_OUT_OF_MEMORY, "graph buffers for %u chunks and %u voxels", count, total_max);
+    }
+    EncodeArgs enc{};
+    enc.voxels = (const int4*)g->voxels;
+    enc.n_new = total_max;  // sizes for the buffers; the kernels read the offsets
+    enc.log2_size = log2_size;
+    enc.chunk_count = count;
+    enc.capacity = capacity;
+    enc.offsets = g->offsets;
+    enc.chunk_log2 = g->chunk_log2;
+    uint32_t bits = 0;
+    while ((1u << bits) < count) bits++;
+    enc.chunk_bits = bits;
+    PyrhipBuilt unused{};
+    const PyrResult r = run_build(enc, log2_size, tuning, g->stream, &unused, nullptr, nullptr, &g->gb);
+    if (r != PYR_OK) {
+        pyrhip_graph_destroy(g);
+        return r;
+    }
+    *out = g;
+    return PYR_OK;
+}
+
+PYR_API uint64_t pyrhip_graph_voxels(PyrhipGraph* g) { return g ? (uint64_t)(uintptr_t)g->voxels : 0; }
+
+PYR_API PyrResult pyrhip_graph_launch(PyrhipGraph* g, uint64_t voxels, const uint32_t* offsets,
+                                      const uint8_t* chunk_log2, void* stream, PyrhipBuilt* out, uint32_t* roots,
+                                      uint32_t* first_voxel) {
+    diag_clear();
+    if (!g || !offsets || !out || !roots || !first_voxel) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (g->mixed != (chunk_log2 != nullptr))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "chunk_log2 must be given exactly for graphs of mixed sizes");
+    // the kernels trust the offsets: check them here
+    if (offsets[0] != 0) return fail(PYR_ERROR_INVALID_ARGUMENT, "offsets[0] must be 0");
+    for (uint32_t c = 0; c < g->count; c++) {
+        if (offsets[c + 1] < offsets[c]) return fail(PYR_ERROR_INVALID_ARGUMENT, "offsets decrease at chunk %u", c);
+        if (g->capacity && offsets[c + 1] - offsets[c] > g->capacity)
+            return fail(PYR_ERROR_INVALID_ARGUMENT, "chunk %u exceeds the capacity", c);
+        if (chunk_log2 && (chunk_log2[c] < 3 || chunk_log2[c] > g->log2_size))
+            return fail(PYR_ERROR_INVALID_ARGUMENT, "chunk_log2[%u] outside [3, log2_size]", c);
+    }
+    const uint32_t total = offsets[g->count];
+    if (total > g->total_max) return fail(PYR_ERROR_INVALID_ARGUMENT, "%u voxels, the graph holds %u", total, g->total_max);
+    hipStream_t s = (hipStream_t)stream;
+    hipError_t e = hipMemcpyAsync(g->offsets, offsets, ((uint64_t)g->count + 1) * 4, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && chunk_log2)
+        e = hipMemcpyAsync(g->chunk_log2, chunk_log2, g->count, hipMemcpyHostToDevice, s);
+    const uint64_t bytes = (g->capacity ? (uint64_t)g->count * g->capacity : total) * sizeof(PyrVoxel);
+    if (e == hipSuccess && voxels && voxels != (uint64_t)(uintptr_t)g->voxels && bytes)
+        e = hipMemcpyAsync(g->voxels, (const void*)(uintptr_t)voxels, bytes, hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipGraphLaunch(g->gb.exec, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(PYR_ERROR_CUDA, "graph launch: %s", hipGetErrorString(e));
+    }
+    const uint32_t* c = g->gb.host;
+    if (c[C_ERR] != 0) return fail(PYR_ERROR_CUDA, "build: device error flags 0x%x", c[C_ERR]);
+    memset(out, 0, sizeof(*out));
+    out->log2_size = g->log2_size;
+    out->root = c[C_ROOT];
+    out->nodes = (uint64_t)(uintptr_t)g->gb.nodes;
+    out->node_words = c[C_WORDS];
+    out->leaves = (uint64_t)(uintptr_t)g->gb.leaves;
+    out->leaf_count = c[C_LEAVES];
+    const uint32_t fin = c[C_FINAL] & 1;
+    out->keys = (uint64_t)(uintptr_t)g->gb.kb[fin ^ 1];
+    out->attrs = (uint64_t)(uintptr_t)(g->gb.packed ? g->gb.vb[0] : g->gb.vb[fin ^ 1]);
+    out->voxel_count = c[C_U];
+    memcpy(roots, c + C_TILE, (uint64_t)g->count * 4);
+    memcpy(first_voxel, c + C_TILE + g->count, (uint64_t)g->count * 4);
+    return PYR_OK;
 }
 
 PYR_API void pyrhip_built_free(PyrhipBuilt* b, void* stream) {

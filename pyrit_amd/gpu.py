@@ -242,3 +242,66 @@ def tuning_set(log2_size: int, n: int, chunk_count: int, best: dict | None) -> N
 
 def tuning_clear() -> None:
     lib().pyrhip_tuning_clear()
+
+
+class GraphOutput(Built):
+    """A graph launch's output: arrays in the graph's buffers, valid until its
+    next launch; never freed here."""
+
+    def free(self) -> None:
+        pass
+
+
+class BuildGraph:
+    """A chunk-batch build captured once into a HIP graph (pyrhip_graph_*),
+    replayed for any batch of at most `count` chunks and `total_max` voxels."""
+
+    def __init__(self, log2_size: int, count: int, total_max: int, capacity: int = 0, mixed_sizes: bool = False):
+        self.count = count
+        self.handle = C.c_void_p()
+        L = lib()
+        abi.check(L, L.pyrhip_graph_create(log2_size, count, capacity, total_max, int(mixed_sizes), None,
+                                           C.byref(self.handle)), "pyrhip_graph_create")
+
+    def voxels_ptr(self) -> int:
+        return lib().pyrhip_graph_voxels(self.handle)
+
+    def launch(self, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None, stream: int | None = None):
+        """-> (GraphOutput, roots, first); offsets: u32[k + 1] for k <= count
+        chunks (the rest are empty); voxels: VOXEL array, device Buffer /
+        address, or None when they are already in voxels_ptr()"""
+        offs = np.zeros(self.count + 1, np.uint32)
+        o = np.ascontiguousarray(offsets, np.uint32)
+        offs[:len(o)] = o
+        offs[len(o):] = o[-1]  # trailing chunks empty
+        keep = None
+        if isinstance(voxels, np.ndarray):
+            assert voxels.dtype == abi.VOXEL
+            keep = to_device(voxels) if len(voxels) else None
+            vptr = keep.ptr if keep else 0
+        else:
+            vptr = _ptr(voxels)
+        l2 = None
+        if chunk_log2 is not None:
+            l2 = np.full(self.count, 3, np.uint8)
+            l2[:len(chunk_log2)] = chunk_log2
+        out = abi.PyrhipBuilt()
+        roots = np.zeros(self.count, np.uint32)
+        first = np.zeros(self.count, np.uint32)
+        L = lib()
+        abi.check(L, L.pyrhip_graph_launch(self.handle, vptr, offs.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           l2.ctypes.data_as(C.POINTER(C.c_uint8)) if l2 is not None else None,
+                                           stream, C.byref(out), roots.ctypes.data_as(C.POINTER(C.c_uint32)),
+                                           first.ctypes.data_as(C.POINTER(C.c_uint32))), "pyrhip_graph_launch")
+        return GraphOutput(out), roots, first
+
+    def destroy(self) -> None:
+        if self.handle:
+            lib().pyrhip_graph_destroy(self.handle)
+            self.handle = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
