@@ -122,7 +122,7 @@ def build_section(args, out):
     out["build"] = {
         "workload": w.name, "voxels": n, "unique": U, "node_words": W, "leaves": Lc, "ms": ms,
         "gvoxels_per_s": n / (ms * 1e-3) / 1e9, "sort_passes": passes, "radix_bits": rb,
-        "record_bytes": rec, "sort_items": items,
+        "record_bytes": rec, "sort_items": items, "finish": phases[-1]["finish"],
         "phase_ms": {k: float(np.mean([p[k] for p in phases])) for k in
                      ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "sort_pass_ms": pass_ms,
@@ -557,7 +557,8 @@ def main() -> int:
         log(f"copy probe {out['copy_probe']['GBps']:.0f} GB/s")
     if "build" in only:
         n, ms = build_section(args, out)
-        log(f"build {out['build']['gvoxels_per_s']:.2f} Gvox/s, sort pass {out['roofline']['achieved']:.0f} GB/s")
+        log(f"build {out['build']['gvoxels_per_s']:.2f} Gvox/s, sort pass {out['roofline']['achieved']:.0f} GB/s, "
+            f"finish {out['build']['finish']}, phases {out['build']['phase_ms']}")
     if "batch" in only:
         batch_section(args, out)
         log(f"batched {out['batched']['batched_gvoxels_per_s']:.2f} Gvox/s, "

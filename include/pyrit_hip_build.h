@@ -75,6 +75,9 @@ typedef struct PyrhipBuildTuning {
                                the output does not depend on it */
     uint32_t status_words;  /* sort look-back words: 2 (default) 32 bits when a build of packed records holds
                                fewer than 2^30 of them, 1 always 64 bits */
+    uint32_t finish;        /* single builds: 2 sort only the key bits above the low 14 and take the unique
+                               voxels and bricks from a table per 2^14-key bucket; 1 full sort and unique scan;
+                               0 (default) 2 when there are at least 2048 records per bucket */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
@@ -191,7 +194,7 @@ typedef struct PyrhipBuildTimes {
      * waiting for the final read-back */
     float host_alloc_ms, host_enqueue_ms, host_wait_ms;
     uint32_t status_bytes; /* bytes per sort look-back word: 4 or 8 */
-    uint32_t reserved;
+    uint32_t finish;       /* 2: partial sort + bucket tables, 1: full sort + unique scan */
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
 /* Plain device-to-device copy kernel of `bytes` (multiple of 8): the

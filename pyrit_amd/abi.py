@@ -91,7 +91,7 @@ class PyrhipChunkBatch(C.Structure):
 
 class PyrhipBuildTuning(C.Structure):
     _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("scan_items", u32),
-                ("hash_load_pct", u32), ("status_words", u32)]
+                ("hash_load_pct", u32), ("status_words", u32), ("finish", u32)]
 
 
 class PyrhipTuneReport(C.Structure):
@@ -103,7 +103,7 @@ class PyrhipBuildTimes(C.Structure):
     _fields_ = [("encode_ms", f32), ("sort_ms", f32), ("unique_ms", f32), ("leaves_ms", f32), ("levels_ms", f32),
                 ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32), ("record_bytes", u32),
                 ("sort_items", u32), ("host_alloc_ms", f32), ("host_enqueue_ms", f32), ("host_wait_ms", f32),
-                ("status_bytes", u32), ("reserved", u32)]
+                ("status_bytes", u32), ("finish", u32)]
 
 
 class PyrhipDag(C.Structure):

@@ -181,6 +181,7 @@ struct EncodeArgs {
     uint32_t* vals;
     uint32_t* hist;           // [passes][segments][radix]
     uint32_t passes, radix_bits;
+    uint32_t shift0;          // first key bit of the lowest digit (bucket finish: bits below are not sorted)
     uint32_t packed;          // keys[i] = key << 32 | attribute, vals unused
 };
 
@@ -235,7 +236,7 @@ __device__ __forceinline__ void emit_record(const EncodeArgs& a, uint32_t* s_his
     }
     const uint64_t active = __ballot(1);
     for (uint32_t p = 0; p < a.passes; p++) {
-        const uint32_t dg = (uint32_t)((key >> (p * a.radix_bits)) & (radix - 1));
+        const uint32_t dg = (uint32_t)((key >> (a.shift0 + p * a.radix_bits)) & (radix - 1));
         const uint32_t d0 = __builtin_amdgcn_readfirstlane(dg);
         const uint64_t same = __ballot(dg == d0);
         if (same == active) {
@@ -894,6 +895,179 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
     }
 }
 
+// ---------------------------------------------------------------------------
+// bucket finish of single builds: partial sort + one key table per bucket
+// ---------------------------------------------------------------------------
+// The radix passes sort only the key bits at and above `lb` (the bucket of a
+// key is key >> lb); inside a bucket the records keep their input order. The
+// last record of a key is then the one with the highest index in its bucket,
+// which a table of 2^lb slots in LDS finds with one atomic max per record:
+// the same unique voxels (last record wins, attribute 0 deletes, out-of-range
+// keys dropped) and bricks as the full sort and the unique scan
+// (gbuild.zig:366-428), in the same order, without the low passes.
+constexpr uint32_t kBucketBits = 14;
+constexpr uint32_t kBucketSlots = 1u << kBucketBits;
+constexpr uint32_t kBucketBlock = 512;
+constexpr uint32_t kBucketWaves = kBucketBlock / kWave;
+constexpr int kBucketLoads = 8;   // record loads per thread in flight
+constexpr int kBucketGroup = 16;  // bricks per wave whose winners are fetched together
+
+// start[b] = first record whose key is at least b << lb, b in [0, buckets]
+// (start[buckets] = first out-of-range record, whose keys sort last)
+template <bool PACKED>
+__global__ void __launch_bounds__(kBlock) k_bucket_bounds(const uint32_t* counters, RecordBufs b, uint32_t n,
+                                                           uint32_t lb, uint32_t buckets, uint32_t* start) {
+    const uint64_t* keys = b.keys[counters[C_FINAL]];
+    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i <= buckets; i += gridDim.x * kBlock) {
+        const uint64_t target = (uint64_t)i << lb;
+        uint32_t lo = 0, hi = n;
+        while (lo < hi) {
+            const uint32_t mid = lo + (hi - lo) / 2;
+            const uint64_t k = PACKED ? keys[mid] >> 32 : keys[mid];
+            if (k < target) lo = mid + 1;
+            else hi = mid;
+        }
+        start[i] = lo;
+    }
+}
+
+// One bucket per tile, tiles handed out in order: clear the table, record
+// per key the highest index + 1 of its records, fetch each winner's attribute
+// (kept when non-zero), count kept voxels and non-empty bricks per brick of
+// 64 slots, two decoupled look-backs for the bucket's first unique position
+// and brick index (waves 0 and 1), then write keys, attributes, brick keys
+// and masks. A brick lies inside one bucket, so its mask is written whole.
+template <bool PACKED>
+__global__ void __launch_bounds__(kBucketBlock, 2)
+    k_bucket_unique(ScanCtx su, uint64_t* status_heads, const uint32_t* start, uint32_t buckets, RecordBufs b,
+                    uint32_t lb, uint64_t* brick_key, uint64_t* masks, uint64_t brick_cap, uint32_t n_cap) {
+    constexpr uint32_t kBricksMax = kBucketSlots / 64;
+    __shared__ uint32_t tab[kBucketSlots];
+    __shared__ uint64_t s_mask[kBricksMax];
+    __shared__ uint32_t s_bu[kBricksMax], s_bh[kBricksMax];
+    __shared__ uint32_t s_tile, s_wu[kBucketWaves], s_wh[kBucketWaves];
+    __shared__ uint64_t s_pre[2];
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint64_t lt = (1ull << lane) - 1;
+    const uint32_t slots = 1u << lb, nbr = slots >> 6;
+    const uint32_t fin = su.counters[C_FINAL];
+    const uint64_t* keys = b.keys[fin];
+    const uint32_t* vals = b.vals[fin];
+    uint64_t* ukey = b.keys[fin ^ 1];
+    uint32_t* uval = PACKED ? b.vals[0] : b.vals[fin ^ 1];
+    for (;;) {
+        const uint32_t bk = next_tile(su, &s_tile);
+        if (bk >= buckets) break;
+        const uint32_t s = start[bk], e = start[bk + 1];
+        uint32_t tot_u = 0, tot_b = 0;
+        if (s < e) {  // (uniform in the block)
+            for (uint32_t i = threadIdx.x; i < slots / 4; i += kBucketBlock)
+                reinterpret_cast<uint4*>(tab)[i] = make_uint4(0u, 0u, 0u, 0u);
+            __syncthreads();
+            for (uint32_t j0 = s; j0 < e; j0 += kBucketBlock * kBucketLoads) {
+                uint64_t kk[kBucketLoads];
+#pragma unroll
+                for (int u = 0; u < kBucketLoads; u++) {
+                    const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
+                    kk[u] = j < e ? keys[j] : 0;
+                }
+#pragma unroll
+                for (int u = 0; u < kBucketLoads; u++) {
+                    const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
+                    if (j < e) {
+                        const uint64_t key = PACKED ? kk[u] >> 32 : kk[u];
+                        // (records of another bucket cannot be here; out-of-range
+                        // keys lie past start[buckets])
+                        __hip_atomic_fetch_max(&tab[(uint32_t)key & (slots - 1)], j - s + 1, __ATOMIC_RELAXED,
+                                               __HIP_MEMORY_SCOPE_WORKGROUP);
+                    }
+                }
+            }
+            __syncthreads();
+            // winners' attributes into the table, brick masks of kept voxels
+            for (uint32_t g0 = wave * kBucketGroup; g0 < nbr; g0 += kBucketWaves * kBucketGroup) {
+                uint32_t idx[kBucketGroup], v[kBucketGroup];
+#pragma unroll
+                for (int q = 0; q < kBucketGroup; q++) idx[q] = g0 + q < nbr ? tab[(g0 + q) * 64 + lane] : 0u;
+#pragma unroll
+                for (int q = 0; q < kBucketGroup; q++) {
+                    v[q] = 0;
+                    if (idx[q]) v[q] = PACKED ? (uint32_t)keys[s + idx[q] - 1] : vals[s + idx[q] - 1];
+                }
+#pragma unroll
+                for (int q = 0; q < kBucketGroup; q++) {
+                    if (g0 + q < nbr) {
+                        tab[(g0 + q) * 64 + lane] = v[q];
+                        const uint64_t m = __ballot(v[q] != 0);
+                        if (lane == 0) s_mask[g0 + q] = m;
+                    }
+                }
+            }
+            __syncthreads();
+            // exclusive prefixes of kept voxels and non-empty bricks over the bricks
+            uint32_t cu = 0, ch = 0;
+            if (threadIdx.x < nbr) {
+                const uint64_t m = s_mask[threadIdx.x];
+                cu = (uint32_t)__popcll(m);
+                ch = m != 0;
+            }
+            const uint32_t iu = wave_inclusive_scan(cu), ih = wave_inclusive_scan(ch);
+            if (lane == kWave - 1) {
+                s_wu[wave] = iu;
+                s_wh[wave] = ih;
+            }
+            __syncthreads();
+            uint32_t ou = 0, oh = 0;
+            for (uint32_t w = 0; w < kBucketWaves; w++) {
+                if (w < wave) {
+                    ou += s_wu[w];
+                    oh += s_wh[w];
+                }
+                tot_u += s_wu[w];
+                tot_b += s_wh[w];
+            }
+            if (threadIdx.x < nbr) {
+                s_bu[threadIdx.x] = ou + iu - cu;
+                s_bh[threadIdx.x] = oh + ih - ch;
+            }
+        }
+        if (wave == 0) {
+            const uint64_t p = lookback_wave(su.status, bk, tot_u, su.epoch, &su.counters[C_ERR]);
+            if (lane == 0) s_pre[0] = p;
+        } else if (wave == 1) {
+            const uint64_t p = lookback_wave(status_heads, bk, tot_b, su.epoch, &su.counters[C_ERR]);
+            if (lane == 0) s_pre[1] = p;
+        }
+        __syncthreads();
+        const uint64_t pre_u = s_pre[0], pre_b = s_pre[1];
+        if (tot_u) {
+            for (uint32_t br = wave; br < nbr; br += kBucketWaves) {
+                const uint64_t m = s_mask[br];
+                if (!m) continue;  // (uniform in the wave)
+                const uint64_t bi = pre_b + s_bh[br], pos = pre_u + s_bu[br] + __popcll(m & lt);
+                const uint64_t key0 = ((uint64_t)bk << lb) | ((uint64_t)br << 6);
+                if (bi >= brick_cap || pre_u + s_bu[br] + __popcll(m) > n_cap) {  // only after a failed look-back
+                    if (lane == 0) atomicOr(&su.counters[C_ERR], 16u);
+                    continue;
+                }
+                if (lane == 0) {
+                    brick_key[bi] = key0 >> 6;
+                    masks[bi] = m;
+                }
+                if ((m >> lane) & 1) {
+                    ukey[pos] = key0 | lane;
+                    uval[pos] = tab[br * 64 + lane];
+                }
+            }
+        }
+        if (bk == buckets - 1 && threadIdx.x == 0) {
+            su.counters[C_U] = (uint32_t)(pre_u + tot_u);
+            su.counters[C_B] = (uint32_t)(pre_b + tot_b);
+            su.counters[C_NCH + 2] = (uint32_t)(pre_b + tot_b);
+        }
+    }
+}
+
 // clear the first hash_cap_for(count, load) slots of a table (keys 0, owner ~0)
 __global__ void __launch_bounds__(kBlock) k_hash_clear(const uint32_t* counters, uint32_t count_slot, void* table,
                                                         uint32_t key_bytes, uint32_t* owner, uint64_t max_cap,
@@ -1432,7 +1606,19 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // the keys at the unique step (chunk << lbits | local, which must fit)
     if (lbits + enc.chunk_bits > 63) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
     const uint32_t key_bits = lbits + 1;
-    const uint32_t passes = (key_bits + rb - 1) / rb;
+    // bucket finish (single builds): sort the bits from lb up only, then one
+    // key table per bucket of 2^lb keys; by default when the buckets hold
+    // 2048 records or more on average
+    if (tun.finish > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "finish must be 0, 1 or 2");
+    const uint32_t lb_try = std::min(kBucketBits, lbits);
+    const bool bucket_ok = enc.chunk_count == 0 && lbits - lb_try <= 20;
+    if (tun.finish == 2 && !bucket_ok)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "the bucket finish needs a single build of at most 2^20 buckets");
+    const bool bucketed =
+        bucket_ok && (tun.finish == 2 || (tun.finish == 0 && ((uint64_t)n >> (lbits - lb_try)) >= 2048));
+    const uint32_t lb = bucketed ? lb_try : 0;
+    const uint32_t buckets = bucketed ? 1u << (lbits - lb) : 0;
+    const uint32_t passes = (key_bits - lb + rb - 1) / rb;
     // keys of at most 32 bits travel packed with their attribute in one u64
     // (16 instead of 24 bytes per record and pass)
     const bool packed = key_bits <= 32 && rb == 8 && tun.pair_layout != 1;
@@ -1478,11 +1664,14 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // packed records in segments of fewer than 2^30 records: narrow sort
     // look-back words in an array of their own, cleared before every pass
     const bool narrow = packed && n < (1u << 30) && tun.status_words == 2;
-    const uint64_t status_words =
+    // (bucket finish: two words per bucket)
+    const uint64_t status_words = std::max<uint64_t>(
         std::max<uint64_t>(sort_tiles * std::max<uint32_t>(narrow ? 0u : radix,
-                                                           2 * ((sort_tile + kScanTile - 1) / kScanTile)),
-                           scan_tiles);
+                                                            2 * ((sort_tile + kScanTile - 1) / kScanTile)),
+                           scan_tiles),
+        2ull * buckets);
     uint64_t* status = A.get<uint64_t>(status_words);
+    uint32_t* bucket_start = bucketed ? A.get<uint32_t>((uint64_t)buckets + 1) : nullptr;
     uint32_t* status32 = narrow ? A.get<uint32_t>(sort_tiles * radix) : nullptr;
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
@@ -1547,7 +1736,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     fill32(s, counters, 0, C_COUNT);
     fill32(s, hist, 0, hist_words);
     fill64(s, status, 0, status_words);
-    fill64(s, masks, 0, nb);
+    if (!bucketed) fill64(s, masks, 0, nb);  // (the bucket finish writes whole masks)
     if (enc.chunk_count) fill32(s, roots_dev, 0xFFFFFFFFu, enc.chunk_count * 2);
 
     uint32_t launch = 0, epoch = 0;
@@ -1559,6 +1748,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     enc.hist = hist;
     enc.passes = passes;
     enc.radix_bits = rb;
+    enc.shift0 = lb;
     enc.packed = packed;
     if (n) {
         if (enc.chunk_count) {
@@ -1583,14 +1773,29 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // source buffer, so the buffers are named by index here)
     for (uint32_t p = 0; p < passes && n; p++) {
         if (narrow) fill32(s, status32, 0, sort_tiles * radix);
-        SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + p * rb, digit_base + (uint64_t)p * segs * radix,
+        SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + lb + p * rb, digit_base + (uint64_t)p * segs * radix,
                     status, status32, ++epoch, tile_ctr(), counters + C_ERR};
         launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
     }
     tm.mark();
     // 3. unique into the buffer pair that does not hold the sorted records;
     // it becomes Built.keys/attrs
-    if (n) {
+    if (n && bucketed) {
+        ScanCtx sc{status, ++epoch, tile_ctr(), counters};
+        const dim3 bgrid(grid_for((uint64_t)buckets + 1, kBlock, 1024));
+        const dim3 ugrid(std::min<uint32_t>(buckets, 1024));
+        if (packed) {
+            hipLaunchKernelGGL(k_bucket_bounds<true>, bgrid, dim3(kBlock), 0, s, counters, bufs, n, lb, buckets,
+                               bucket_start);
+            hipLaunchKernelGGL(k_bucket_unique<true>, ugrid, dim3(kBucketBlock), 0, s, sc, status + buckets,
+                               bucket_start, buckets, bufs, lb, brick_key, masks, nb, n);
+        } else {
+            hipLaunchKernelGGL(k_bucket_bounds<false>, bgrid, dim3(kBlock), 0, s, counters, bufs, n, lb, buckets,
+                               bucket_start);
+            hipLaunchKernelGGL(k_bucket_unique<false>, ugrid, dim3(kBucketBlock), 0, s, sc, status + buckets,
+                               bucket_start, buckets, bufs, lb, brick_key, masks, nb, n);
+        }
+    } else if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         launch_unique_bricks(s, scan_items, sort_tile, sort_tiles, sc, seg, bufs, lbits, packed, brick_key, masks,
                              first_dev, nb);
@@ -1702,6 +1907,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         g_times.record_bytes = packed ? 8 : 12;
         g_times.sort_items = sort_items;
         g_times.status_bytes = narrow ? 4 : 8;
+        g_times.finish = bucketed ? 2 : 1;
         using ms = std::chrono::duration<float, std::milli>;
         g_times.host_alloc_ms = ms(host_t1 - host_t0).count();
         g_times.host_enqueue_ms = ms(host_t2 - host_t1).count();

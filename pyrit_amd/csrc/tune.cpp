@@ -23,7 +23,7 @@
 namespace pyr {
 namespace {
 
-constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2};
+constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2, 0};
 
 struct Table {
     std::mutex m;
@@ -55,11 +55,13 @@ void overlay(PyrhipBuildTuning& t, const PyrhipBuildTuning& o) {
     if (o.scan_items) t.scan_items = o.scan_items;
     if (o.hash_load_pct) t.hash_load_pct = o.hash_load_pct;
     if (o.status_words) t.status_words = o.status_words;
+    if (o.finish) t.finish = o.finish;
 }
 
 bool same_tuning(const PyrhipBuildTuning& a, const PyrhipBuildTuning& b) {
     return a.radix_bits == b.radix_bits && a.sort_items == b.sort_items && a.pair_layout == b.pair_layout &&
-           a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct && a.status_words == b.status_words;
+           a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct && a.status_words == b.status_words &&
+           a.finish == b.finish;
 }
 
 // what a build produced, enough to tell two settings' outputs apart
@@ -220,14 +222,17 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // (11-bit digits are left out: 2048 look-back words per tile made a
         // pass ten times slower than 8-bit digits on every workload measured)
         const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
-        for (int dim = 0; dim < 5; dim++) {
-            for (int v = 0; v < (dim == 4 ? 2 : 4); v++) {
+        // (the finish only differs for single builds)
+        const int dims = batch ? 5 : 6;
+        for (int dim = 0; dim < dims; dim++) {
+            for (int v = 0; v < (dim >= 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];
                 if (dim == 1) c.scan_items = scans[v];
                 if (dim == 2) c.radix_bits = layouts[v][0], c.pair_layout = layouts[v][1];
                 if (dim == 3) c.hash_load_pct = loads[v];
                 if (dim == 4) c.status_words = 2 - v;
+                if (dim == 5) c.finish = 2 - v;
                 if (same_tuning(c, best)) continue;
                 float ms = 0;
                 Signature sig;

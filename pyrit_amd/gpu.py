@@ -58,13 +58,14 @@ class Built:
 
 
 def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
-            hash_load_pct: int = 0, status_words: int = 0) -> abi.PyrhipBuildTuning:
+            hash_load_pct: int = 0, status_words: int = 0, finish: int = 0) -> abi.PyrhipBuildTuning:
     """0 = the setting installed for the workload class, else the default;
     pair_layout 1 = separate key / attribute arrays, 2 = packed when keys fit"""
     t = abi.PyrhipBuildTuning()
     t.radix_bits, t.sort_items, t.pair_layout, t.scan_items = radix_bits, sort_items, pair_layout, scan_items
     t.hash_load_pct = hash_load_pct
     t.status_words = status_words
+    t.finish = finish
     return t
 
 
@@ -78,7 +79,7 @@ def _ptr(x) -> int:
 
 def build(log2_size: int, voxels, count: int | None = None, old: Built | None = None, radix_bits: int = 0,
           stream: int | None = None, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
-          hash_load_pct: int = 0, status_words: int = 0) -> Built:
+          hash_load_pct: int = 0, status_words: int = 0, finish: int = 0) -> Built:
     """Build a DAG from PyrVoxel records (numpy VOXEL array or device Buffer).
     `old` supplies the sorted voxels of an earlier editable build (edits)."""
     keep = None
@@ -91,7 +92,7 @@ def build(log2_size: int, voxels, count: int | None = None, old: Built | None = 
         vptr = _ptr(voxels)
         assert count is not None
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits, sort_items, pair_layout, scan_items, hash_load_pct, status_words)
+    t = _tuning(radix_bits, sort_items, pair_layout, scan_items, hash_load_pct, status_words, finish)
     L = lib()
     code = L.pyrhip_build(log2_size, old.raw.keys if old else 0, old.raw.attrs if old else 0,
                           old.voxel_count if old else 0, vptr, count, C.byref(t), stream, C.byref(out))

@@ -148,20 +148,75 @@ def test_lookback_word_widths_identical(hip_device, status_words):
     assert np.array_equal(roots, rr) and np.array_equal(first, rf)
 
 
+@pytest.mark.parametrize("log2,n,seed,layout", [(3, 50, 1, 0), (4, 300, 2, 0), (5, 3000, 3, 0), (6, 20000, 4, 0),
+                                                (8, 200000, 6, 0), (9, 90000, 7, 1), (10, 100000, 8, 0),
+                                                (11, 40000, 9, 0)])
+def test_bucket_finish_bit_exact(hip_device, log2, n, seed, layout):
+    """Partial sort (key bits from 14 up) + one key table per bucket: the
+    same arrays as the full sort and unique scan, for packed and separate
+    records (log2 9 with separate arrays, log2 11 with 34-bit keys)."""
+    print(f"case: bucket finish log2={log2} n={n} seed={seed} layout={layout}", flush=True)
+    pts = random_points(n, log2, seed, dup_frac=0.2, delete_frac=0.05, out_frac=0.02)
+    ref = O.gb_build(log2, pts)
+    g = _gpu()
+    for finish in (1, 2):
+        b = g.build(log2, pts, finish=finish, pair_layout=layout)
+        assert_same_build(b, ref, log2)
+    g.set_timing(True)
+    g.build(log2, pts, finish=2, pair_layout=layout)
+    assert g.last_build_times()["finish"] == 2
+    g.set_timing(False)
+    # an edit build through the bucket finish
+    ed = random_points(max(n // 10, 5), log2, seed + 100, dup_frac=0.2, delete_frac=0.3)
+    b2 = g.build(log2, ed, old=b, finish=2, pair_layout=layout)
+    assert_same_build(b2, O.gb_build(log2, ed, ref["keys"], ref["attrs"]), log2)
+
+
+@pytest.mark.parametrize("case", ["solid 64", "one key repeated", "low corner 1024", "terrain chunk 64"])
+def test_bucket_finish_edge_cases(hip_device, case):
+    """Buckets whose 16384 slots are all set, a bucket of far more records
+    than slots (one key 100000 times, the last wins), every record in bucket
+    0 of a 1024^3 grid with every pass skipped, and a real terrain chunk."""
+    print(f"case: bucket finish, {case}", flush=True)
+    rng = np.random.default_rng(21)
+    if case == "solid 64":
+        log2 = 6
+        pts = grid_to_points(np.ones((64, 64, 64), np.uint32) * 7)
+        pts = pts[rng.permutation(len(pts))]
+    elif case == "one key repeated":
+        log2 = 7
+        pts = np.zeros(100000, O.VOXEL)
+        pts["x"], pts["y"], pts["z"] = 100, 3, 77
+        pts["attribute"] = rng.integers(1, 1000, len(pts))
+        pts = np.concatenate([pts, random_points(5000, 7, 22, dup_frac=0.2, delete_frac=0.1)])
+    elif case == "low corner 1024":
+        log2, pts = 10, random_points(4000, 3, 12, dup_frac=0.3, delete_frac=0.1)
+    else:
+        from pyrit_amd import workloads
+
+        log2 = 6
+        pts, _ = workloads.terrain_chunk64().generate_host()
+    ref = O.gb_build(log2, pts)
+    assert_same_build(_gpu().build(log2, pts, finish=2), ref, log2)
+
+
 def test_tuning_argument_checks(hip_device):
     g = _gpu()
     pts = random_points(1000, 5, 3)
     for kw in ({"hash_load_pct": 10}, {"hash_load_pct": 99}, {"pair_layout": 3}, {"radix_bits": 12},
-               {"sort_items": 10}, {"scan_items": 24}, {"status_words": 3}):
+               {"sort_items": 10}, {"scan_items": 24}, {"status_words": 3}, {"finish": 3}):
         with pytest.raises(RuntimeError, match="INVALID_ARGUMENT"):
             g.build(5, pts, **kw)
+    # the bucket finish: single builds of at most 2^20 buckets (log2 <= 11)
+    with pytest.raises(RuntimeError, match="INVALID_ARGUMENT"):
+        g.build(12, random_points(1000, 12, 3), finish=2)
 
 
 def test_autotune_installs_an_identical_setting(hip_device):
     print("case: autotune single build and batch", flush=True)
     g = _gpu()
     default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80,
-               "status_words": 2}
+               "status_words": 2, "finish": 0}
     g.tuning_clear()
     try:
         pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
