@@ -931,6 +931,7 @@ constexpr uint32_t kBucketBlock = 512;
 constexpr uint32_t kBucketWaves = kBucketBlock / kWave;
 constexpr int kBucketLoads = 8;   // record loads per thread in flight
 constexpr int kBucketGroup = 16;  // bricks per wave whose winners are fetched together
+constexpr int kBucketRegs = kBucketSlots / kBucketBlock;  // records per thread of a bucket held in registers
 
 // start[b] = first record whose key is at least b << lb, b in [0, buckets]
 // (start[buckets] = first out-of-range record, whose keys sort last)
@@ -952,13 +953,15 @@ __global__ void __launch_bounds__(kBlock) k_bucket_bounds(const uint32_t* counte
 }
 
 // One bucket per tile, tiles handed out in order: clear the table, record
-// per key the highest index + 1 of its records, fetch each winner's attribute
-// (kept when non-zero), count kept voxels and non-empty bricks per brick of
+// per key the highest index + 1 of its records, put each winner's attribute
+// in its slot (kept when non-zero; a bucket of at most 2^14 packed records
+// stays in registers, so the winners write their attribute after a compare,
+// larger ones fetch it again), count kept voxels and non-empty bricks per brick of
 // 64 slots, two decoupled look-backs for the bucket's first unique position
 // and brick index (waves 0 and 1), then write keys, attributes, brick keys
 // and masks. A brick lies inside one bucket, so its mask is written whole.
 template <bool PACKED>
-__global__ void __launch_bounds__(kBucketBlock, 2)
+__global__ void __launch_bounds__(kBucketBlock, 4)
     k_bucket_unique(ScanCtx su, uint64_t* status_heads, const uint32_t* start, uint32_t buckets, RecordBufs b,
                     uint32_t lb, uint64_t* brick_key, uint64_t* masks, uint64_t brick_cap, uint32_t n_cap) {
     constexpr uint32_t kBricksMax = kBucketSlots / 64;
@@ -984,44 +987,72 @@ __global__ void __launch_bounds__(kBucketBlock, 2)
             for (uint32_t i = threadIdx.x; i < slots / 4; i += kBucketBlock)
                 reinterpret_cast<uint4*>(tab)[i] = make_uint4(0u, 0u, 0u, 0u);
             __syncthreads();
-            for (uint32_t j0 = s; j0 < e; j0 += kBucketBlock * kBucketLoads) {
-                uint64_t kk[kBucketLoads];
+            if (PACKED && e - s <= kBucketSlots) {
+                // the whole bucket in registers: winners take their attribute
+                // straight from them (no second read of the records)
+                uint32_t sl[kBucketRegs], at[kBucketRegs];  // slot (~0: no record) and attribute
 #pragma unroll
-                for (int u = 0; u < kBucketLoads; u++) {
-                    const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
-                    kk[u] = j < e ? keys[j] : 0;
+                for (int u = 0; u < kBucketRegs; u++) {
+                    const uint32_t j = s + u * kBucketBlock + threadIdx.x;
+                    const uint64_t w = j < e ? keys[j] : ~0ull;
+                    sl[u] = j < e ? (uint32_t)(w >> 32) & (slots - 1) : ~0u;
+                    at[u] = (uint32_t)w;
                 }
 #pragma unroll
-                for (int u = 0; u < kBucketLoads; u++) {
-                    const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
-                    if (j < e) {
-                        const uint64_t key = PACKED ? kk[u] >> 32 : kk[u];
-                        // (records of another bucket cannot be here; out-of-range
-                        // keys lie past start[buckets])
-                        __hip_atomic_fetch_max(&tab[(uint32_t)key & (slots - 1)], j - s + 1, __ATOMIC_RELAXED,
+                for (int u = 0; u < kBucketRegs; u++)
+                    if (sl[u] != ~0u)
+                        __hip_atomic_fetch_max(&tab[sl[u]], u * kBucketBlock + threadIdx.x + 1, __ATOMIC_RELAXED,
                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+                __syncthreads();
+                uint32_t win = 0;
+#pragma unroll
+                for (int u = 0; u < kBucketRegs; u++)
+                    if (sl[u] != ~0u && tab[sl[u]] == u * kBucketBlock + threadIdx.x + 1) win |= 1u << u;
+                __syncthreads();  // every comparison is done before a winner overwrites its slot
+#pragma unroll
+                for (int u = 0; u < kBucketRegs; u++)
+                    if ((win >> u) & 1) tab[sl[u]] = at[u];
+            } else {
+                for (uint32_t j0 = s; j0 < e; j0 += kBucketBlock * kBucketLoads) {
+                    uint64_t kk[kBucketLoads];
+#pragma unroll
+                    for (int u = 0; u < kBucketLoads; u++) {
+                        const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
+                        kk[u] = j < e ? keys[j] : 0;
                     }
+#pragma unroll
+                    for (int u = 0; u < kBucketLoads; u++) {
+                        const uint32_t j = j0 + u * kBucketBlock + threadIdx.x;
+                        if (j < e) {
+                            const uint64_t key = PACKED ? kk[u] >> 32 : kk[u];
+                            // (records of another bucket cannot be here; out-of-range
+                            // keys lie past start[buckets])
+                            __hip_atomic_fetch_max(&tab[(uint32_t)key & (slots - 1)], j - s + 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+                        }
+                    }
+                }
+                __syncthreads();
+                // winners' attributes into the table
+                for (uint32_t g0 = wave * kBucketGroup; g0 < nbr; g0 += kBucketWaves * kBucketGroup) {
+                    uint32_t idx[kBucketGroup], v[kBucketGroup];
+#pragma unroll
+                    for (int q = 0; q < kBucketGroup; q++) idx[q] = g0 + q < nbr ? tab[(g0 + q) * 64 + lane] : 0u;
+#pragma unroll
+                    for (int q = 0; q < kBucketGroup; q++) {
+                        v[q] = 0;
+                        if (idx[q]) v[q] = PACKED ? (uint32_t)keys[s + idx[q] - 1] : vals[s + idx[q] - 1];
+                    }
+#pragma unroll
+                    for (int q = 0; q < kBucketGroup; q++)
+                        if (g0 + q < nbr) tab[(g0 + q) * 64 + lane] = v[q];
                 }
             }
             __syncthreads();
-            // winners' attributes into the table, brick masks of kept voxels
-            for (uint32_t g0 = wave * kBucketGroup; g0 < nbr; g0 += kBucketWaves * kBucketGroup) {
-                uint32_t idx[kBucketGroup], v[kBucketGroup];
-#pragma unroll
-                for (int q = 0; q < kBucketGroup; q++) idx[q] = g0 + q < nbr ? tab[(g0 + q) * 64 + lane] : 0u;
-#pragma unroll
-                for (int q = 0; q < kBucketGroup; q++) {
-                    v[q] = 0;
-                    if (idx[q]) v[q] = PACKED ? (uint32_t)keys[s + idx[q] - 1] : vals[s + idx[q] - 1];
-                }
-#pragma unroll
-                for (int q = 0; q < kBucketGroup; q++) {
-                    if (g0 + q < nbr) {
-                        tab[(g0 + q) * 64 + lane] = v[q];
-                        const uint64_t m = __ballot(v[q] != 0);
-                        if (lane == 0) s_mask[g0 + q] = m;
-                    }
-                }
+            // brick masks of kept voxels (attribute non-zero)
+            for (uint32_t br = wave; br < nbr; br += kBucketWaves) {
+                const uint64_t m = __ballot(tab[br * 64 + lane] != 0);
+                if (lane == 0) s_mask[br] = m;
             }
             __syncthreads();
             // exclusive prefixes of kept voxels and non-empty bricks over the bricks
