@@ -75,9 +75,10 @@ typedef struct PyrhipBuildTuning {
                                the output does not depend on it */
     uint32_t status_words;  /* sort look-back words: 2 (default) 32 bits when a build of packed records holds
                                fewer than 2^30 of them, 1 always 64 bits */
-    uint32_t finish;        /* single builds: 2 sort only the key bits above the low 14 and take the unique
-                               voxels and bricks from a table per 2^14-key bucket; 1 full sort and unique scan;
-                               0 (default) 2 when there are at least 2048 records per bucket */
+    uint32_t finish;        /* 2: sort only the key bits above the low 14 and take the unique voxels and
+                               bricks from a table per bucket of 2^14 keys of a chunk; 1: full sort and unique
+                               scan; 0 (default): 2 when a single build has 2048 records per bucket on average
+                               (a batch 512 per bucket of chunks of its largest size) */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique

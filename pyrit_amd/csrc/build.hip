@@ -57,6 +57,7 @@ enum : uint32_t {
     C_PASS = 64,   // bit p: radix pass p moves records (some segment has two digits)
     C_PSRC = 65,   // bit p: record buffer pass p reads (it writes the other)
     C_FINAL = 66,  // record buffer that holds the sorted records
+    C_BKT = 67,    // buckets of a batch's bucket finish
     C_TILE = 72,   // tile counters, one per launch
     C_COUNT = 256,
 };
@@ -207,7 +208,8 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, ui
         val = a.old_vals[i];
         return;
     }
-    const int32_t n = 1 << (a.chunk_count && a.chunk_log2 ? a.chunk_log2[seg] : a.log2_size);
+    // (a chunk size above the batch's counts as the batch's: keys stay below the invalid bit)
+    const int32_t n = 1 << (a.chunk_count && a.chunk_log2 ? min((uint32_t)a.chunk_log2[seg], a.log2_size) : a.log2_size);
     key = 1ull << lbits;
     val = 0;
     if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
@@ -916,10 +918,12 @@ __global__ void __launch_bounds__(kBlock, 4) k_unique_bricks(ScanCtx su, uint64_
 }
 
 // ---------------------------------------------------------------------------
-// bucket finish of single builds: partial sort + one key table per bucket
+// bucket finish: partial sort + one key table per bucket
 // ---------------------------------------------------------------------------
 // The radix passes sort only the key bits at and above `lb` (the bucket of a
-// key is key >> lb); inside a bucket the records keep their input order. The
+// key is its segment and key >> lb; a segment of a batch has
+// 2^(max(local bits of its chunk, lb) - lb) buckets, a single build 2^(local
+// bits - lb)); inside a bucket the records keep their input order. The
 // last record of a key is then the one with the highest index in its bucket,
 // which a table of 2^lb slots in LDS finds with one atomic max per record:
 // the same unique voxels (last record wins, attribute 0 deletes, out-of-range
@@ -933,22 +937,74 @@ constexpr int kBucketLoads = 8;   // record loads per thread in flight
 constexpr int kBucketGroup = 16;  // bricks per wave whose winners are fetched together
 constexpr int kBucketRegs = kBucketSlots / kBucketBlock;  // records per thread of a bucket held in registers
 
-// start[b] = first record whose key is at least b << lb, b in [0, buckets]
-// (start[buckets] = first out-of-range record, whose keys sort last)
+// first_bucket[c] = buckets of the segments before c, counters[C_BKT] = all
+// of them (one block). Offsets that k_seg_tiles rejected leave no bucket; a
+// chunk size above log2_size counts as log2_size and raises error bit 128.
+__global__ void __launch_bounds__(kBlock) k_bucket_plan(const uint8_t* chunk_log2, uint32_t count, uint32_t log2_size,
+                                                         uint32_t lb, uint32_t* first_bucket, uint32_t* counters) {
+    __shared__ uint32_t s_wave[kBlock / kWave];
+    uint32_t carry = 0;
+    for (uint32_t b = 0; b < count; b += kBlock) {
+        const uint32_t c = b + threadIdx.x;
+        uint32_t nb = 0;
+        if (c < count) {
+            uint32_t l2 = chunk_log2 ? chunk_log2[c] : log2_size;
+            if (l2 > log2_size) {
+                atomicOr(&counters[C_ERR], 128u);
+                l2 = log2_size;
+            }
+            const uint32_t lc = local_bits(l2);
+            nb = 1u << ((lc > lb ? lc : lb) - lb);
+        }
+        uint32_t v[1] = {nb}, e[1];
+        const uint32_t sum = tile_scan<1>(v, e, s_wave);
+        if (c < count) first_bucket[c] = carry + e[0];
+        carry += sum;
+        __syncthreads();  // s_wave is reused by the next round
+    }
+    if (threadIdx.x == 0) {
+        first_bucket[count] = carry;
+        counters[C_BKT] = counters[C_TILES] ? carry : 0u;
+    }
+}
+
+// Records [start[t], end[t]) of bucket t and its segment bseg[t] (batches):
+// the records of the segment whose keys lie in [j << lb, (j + 1) << lb) for
+// the bucket's index j in its segment. Out-of-range keys sort after every
+// bucket of their segment.
 template <bool PACKED>
-__global__ void __launch_bounds__(kBlock) k_bucket_bounds(const uint32_t* counters, RecordBufs b, uint32_t n,
-                                                           uint32_t lb, uint32_t buckets, uint32_t* start) {
+__global__ void __launch_bounds__(kBlock) k_bucket_bounds(const uint32_t* counters, RecordBufs b, Segs g,
+                                                           const uint32_t* first_bucket, uint32_t lb,
+                                                           uint32_t buckets, uint32_t* start, uint32_t* end,
+                                                           uint32_t* bseg) {
     const uint64_t* keys = b.keys[counters[C_FINAL]];
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i <= buckets; i += gridDim.x * kBlock) {
-        const uint64_t target = (uint64_t)i << lb;
-        uint32_t lo = 0, hi = n;
+    const uint32_t total = g.single ? buckets : counters[C_BKT];
+    auto lower = [&](uint32_t lo, uint32_t hi, uint64_t target) {
         while (lo < hi) {
             const uint32_t mid = lo + (hi - lo) / 2;
             const uint64_t k = PACKED ? keys[mid] >> 32 : keys[mid];
             if (k < target) lo = mid + 1;
             else hi = mid;
         }
-        start[i] = lo;
+        return lo;
+    };
+    for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < total; t += gridDim.x * kBlock) {
+        uint32_t c = 0, j = t;
+        if (!g.single) {  // the segment: last c with first_bucket[c] <= t
+            uint32_t lo = 0, hi = g.count;
+            while (hi - lo > 1) {
+                const uint32_t mid = lo + (hi - lo) / 2;
+                if (first_bucket[mid] <= t) lo = mid;
+                else hi = mid;
+            }
+            c = lo;
+            j = t - first_bucket[c];
+            bseg[t] = c;
+        }
+        const uint32_t s0 = g.single ? 0u : g.off[c], s1 = seg_end(g, c);
+        const uint32_t a = lower(s0, s1, (uint64_t)j << lb);
+        start[t] = a;
+        end[t] = lower(a, s1, (uint64_t)(j + 1) << lb);
     }
 }
 
@@ -962,8 +1018,9 @@ __global__ void __launch_bounds__(kBlock) k_bucket_bounds(const uint32_t* counte
 // and masks. A brick lies inside one bucket, so its mask is written whole.
 template <bool PACKED>
 __global__ void __launch_bounds__(kBucketBlock, 4)
-    k_bucket_unique(ScanCtx su, uint64_t* status_heads, const uint32_t* start, uint32_t buckets, RecordBufs b,
-                    uint32_t lb, uint64_t* brick_key, uint64_t* masks, uint64_t brick_cap, uint32_t n_cap) {
+    k_bucket_unique(ScanCtx su, uint64_t* status_heads, const uint32_t* start, const uint32_t* end,
+                    const uint32_t* bseg, uint32_t buckets, RecordBufs b, uint32_t lb, uint32_t lbits,
+                    uint64_t* brick_key, uint64_t* masks, uint32_t* first_voxel, uint64_t brick_cap, uint32_t n_cap) {
     constexpr uint32_t kBricksMax = kBucketSlots / 64;
     __shared__ uint32_t tab[kBucketSlots];
     __shared__ uint64_t s_mask[kBricksMax];
@@ -978,10 +1035,11 @@ __global__ void __launch_bounds__(kBucketBlock, 4)
     const uint32_t* vals = b.vals[fin];
     uint64_t* ukey = b.keys[fin ^ 1];
     uint32_t* uval = PACKED ? b.vals[0] : b.vals[fin ^ 1];
+    const uint32_t total = bseg ? su.counters[C_BKT] : buckets;
     for (;;) {
         const uint32_t bk = next_tile(su, &s_tile);
-        if (bk >= buckets) break;
-        const uint32_t s = start[bk], e = start[bk + 1];
+        if (bk >= total) break;
+        const uint32_t s = start[bk], e = end[bk];
         uint32_t tot_u = 0, tot_b = 0;
         if (s < e) {  // (uniform in the block)
             for (uint32_t i = threadIdx.x; i < slots / 4; i += kBucketBlock)
@@ -1092,11 +1150,16 @@ __global__ void __launch_bounds__(kBucketBlock, 4)
         __syncthreads();
         const uint64_t pre_u = s_pre[0], pre_b = s_pre[1];
         if (tot_u) {
+            // the segment's key prefix and the bucket's key bits (from its first record)
+            const uint32_t c = bseg ? bseg[bk] : 0u;
+            const uint64_t k0 = PACKED ? keys[s] >> 32 : keys[s];
+            const uint64_t base_key = ((uint64_t)c << lbits) | ((k0 >> lb) << lb);
+            if (first_voxel && threadIdx.x == 0) atomicMin(&first_voxel[c], (uint32_t)pre_u);
             for (uint32_t br = wave; br < nbr; br += kBucketWaves) {
                 const uint64_t m = s_mask[br];
                 if (!m) continue;  // (uniform in the wave)
                 const uint64_t bi = pre_b + s_bh[br], pos = pre_u + s_bu[br] + __popcll(m & lt);
-                const uint64_t key0 = ((uint64_t)bk << lb) | ((uint64_t)br << 6);
+                const uint64_t key0 = base_key | ((uint64_t)br << 6);
                 if (bi >= brick_cap || pre_u + s_bu[br] + __popcll(m) > n_cap) {  // only after a failed look-back
                     if (lane == 0) atomicOr(&su.counters[C_ERR], 16u);
                     continue;
@@ -1111,7 +1174,7 @@ __global__ void __launch_bounds__(kBucketBlock, 4)
                 }
             }
         }
-        if (bk == buckets - 1 && threadIdx.x == 0) {
+        if (bk == total - 1 && threadIdx.x == 0) {
             su.counters[C_U] = (uint32_t)(pre_u + tot_u);
             su.counters[C_B] = (uint32_t)(pre_b + tot_b);
             su.counters[C_NCH + 2] = (uint32_t)(pre_b + tot_b);
@@ -1657,18 +1720,19 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // the keys at the unique step (chunk << lbits | local, which must fit)
     if (lbits + enc.chunk_bits > 63) return fail(PYR_ERROR_INVALID_ARGUMENT, "too many chunks for this chunk size");
     const uint32_t key_bits = lbits + 1;
-    // bucket finish (single builds): sort the bits from lb up only, then one
-    // key table per bucket of 2^lb keys; by default when the buckets hold
-    // 2048 records or more on average
+    // bucket finish: sort the bits from lb up only, then one key table per
+    // bucket of 2^lb keys; by default when a single build's buckets hold 2048
+    // records or more on average (a batch: 512 per bucket of its largest chunks)
     if (tun.finish > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "finish must be 0, 1 or 2");
     const uint32_t lb_try = std::min(kBucketBits, lbits);
-    const bool bucket_ok = enc.chunk_count == 0 && lbits - lb_try <= 20;
+    const uint64_t bucket_cap = (enc.chunk_count ? (uint64_t)enc.chunk_count : 1ull) << (lbits - lb_try);
+    const bool bucket_ok = lbits - lb_try <= 20 && bucket_cap <= (1ull << 24);
     if (tun.finish == 2 && !bucket_ok)
-        return fail(PYR_ERROR_INVALID_ARGUMENT, "the bucket finish needs a single build of at most 2^20 buckets");
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "the bucket finish needs at most 2^24 buckets");
     const bool bucketed =
-        bucket_ok && (tun.finish == 2 || (tun.finish == 0 && ((uint64_t)n >> (lbits - lb_try)) >= 2048));
+        bucket_ok && (tun.finish == 2 || (tun.finish == 0 && (uint64_t)n >= (enc.chunk_count ? 512 : 2048) * bucket_cap));
     const uint32_t lb = bucketed ? lb_try : 0;
-    const uint32_t buckets = bucketed ? 1u << (lbits - lb) : 0;
+    const uint32_t buckets = bucketed ? (uint32_t)bucket_cap : 0;  // a batch's is an upper bound
     const uint32_t passes = (key_bits - lb + rb - 1) / rb;
     // keys of at most 32 bits travel packed with their attribute in one u64
     // (16 instead of 24 bytes per record and pass)
@@ -1722,7 +1786,10 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                            scan_tiles),
         2ull * buckets);
     uint64_t* status = A.get<uint64_t>(status_words);
-    uint32_t* bucket_start = bucketed ? A.get<uint32_t>((uint64_t)buckets + 1) : nullptr;
+    uint32_t* bucket_start = bucketed ? A.get<uint32_t>(buckets) : nullptr;
+    uint32_t* bucket_end = bucketed ? A.get<uint32_t>(buckets) : nullptr;
+    uint32_t* bucket_seg = bucketed && enc.chunk_count ? A.get<uint32_t>(buckets) : nullptr;
+    uint32_t* first_bucket = bucketed && enc.chunk_count ? A.get<uint32_t>(enc.chunk_count + 1) : nullptr;
     uint32_t* status32 = narrow ? A.get<uint32_t>(sort_tiles * radix) : nullptr;
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
@@ -1833,18 +1900,23 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // it becomes Built.keys/attrs
     if (n && bucketed) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
-        const dim3 bgrid(grid_for((uint64_t)buckets + 1, kBlock, 1024));
+        if (enc.chunk_count)
+            hipLaunchKernelGGL(k_bucket_plan, dim3(1), dim3(kBlock), 0, s, enc.chunk_log2, enc.chunk_count, log2_size,
+                               lb, first_bucket, counters);
+        const dim3 bgrid(grid_for(buckets, kBlock, 1024));
         const dim3 ugrid(std::min<uint32_t>(buckets, 1024));
         if (packed) {
-            hipLaunchKernelGGL(k_bucket_bounds<true>, bgrid, dim3(kBlock), 0, s, counters, bufs, n, lb, buckets,
-                               bucket_start);
+            hipLaunchKernelGGL(k_bucket_bounds<true>, bgrid, dim3(kBlock), 0, s, counters, bufs, seg, first_bucket, lb,
+                               buckets, bucket_start, bucket_end, bucket_seg);
             hipLaunchKernelGGL(k_bucket_unique<true>, ugrid, dim3(kBucketBlock), 0, s, sc, status + buckets,
-                               bucket_start, buckets, bufs, lb, brick_key, masks, nb, n);
+                               bucket_start, bucket_end, bucket_seg, buckets, bufs, lb, lbits, brick_key, masks,
+                               first_dev, nb, n);
         } else {
-            hipLaunchKernelGGL(k_bucket_bounds<false>, bgrid, dim3(kBlock), 0, s, counters, bufs, n, lb, buckets,
-                               bucket_start);
+            hipLaunchKernelGGL(k_bucket_bounds<false>, bgrid, dim3(kBlock), 0, s, counters, bufs, seg, first_bucket,
+                               lb, buckets, bucket_start, bucket_end, bucket_seg);
             hipLaunchKernelGGL(k_bucket_unique<false>, ugrid, dim3(kBucketBlock), 0, s, sc, status + buckets,
-                               bucket_start, buckets, bufs, lb, brick_key, masks, nb, n);
+                               bucket_start, bucket_end, bucket_seg, buckets, bufs, lb, lbits, brick_key, masks,
+                               first_dev, nb, n);
         }
     } else if (n) {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};

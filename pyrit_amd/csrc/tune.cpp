@@ -222,9 +222,7 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // (11-bit digits are left out: 2048 look-back words per tile made a
         // pass ten times slower than 8-bit digits on every workload measured)
         const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
-        // (the finish only differs for single builds)
-        const int dims = batch ? 5 : 6;
-        for (int dim = 0; dim < dims; dim++) {
+        for (int dim = 0; dim < 6; dim++) {
             for (int v = 0; v < (dim >= 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];

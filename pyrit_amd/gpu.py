@@ -127,7 +127,7 @@ def _batch(voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None, capacity:
 
 def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None,
                  capacity: int = 0, radix_bits: int = 0, stream: int | None = None, hash_load_pct: int = 0,
-                 status_words: int = 0):
+                 status_words: int = 0, finish: int = 0):
     """Chunk batch (src/gpu_build.zig:98-122). voxels: VOXEL array or device
     Buffer; offsets: u32[count+1] host array. Returns (Built, roots, first)."""
     batch, keep = _batch(voxels, offsets, chunk_log2, capacity)
@@ -135,7 +135,7 @@ def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.nda
     roots = np.zeros(count, np.uint32)
     first = np.zeros(count, np.uint32)
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits, hash_load_pct=hash_load_pct, status_words=status_words)
+    t = _tuning(radix_bits, hash_load_pct=hash_load_pct, status_words=status_words, finish=finish)
     L = lib()
     code = L.pyrhip_build_chunks(log2_size, C.byref(batch), C.byref(t), stream, C.byref(out),
                                  roots.ctypes.data_as(C.POINTER(C.c_uint32)),
@@ -257,11 +257,13 @@ class BuildGraph:
     """A chunk-batch build captured once into a HIP graph (pyrhip_graph_*),
     replayed for any batch of at most `count` chunks and `total_max` voxels."""
 
-    def __init__(self, log2_size: int, count: int, total_max: int, capacity: int = 0, mixed_sizes: bool = False):
+    def __init__(self, log2_size: int, count: int, total_max: int, capacity: int = 0, mixed_sizes: bool = False,
+                 finish: int = 0):
         self.count = count
         self.handle = C.c_void_p()
         L = lib()
-        abi.check(L, L.pyrhip_graph_create(log2_size, count, capacity, total_max, int(mixed_sizes), None,
+        t = _tuning(0, finish=finish)
+        abi.check(L, L.pyrhip_graph_create(log2_size, count, capacity, total_max, int(mixed_sizes), C.byref(t),
                                            C.byref(self.handle)), "pyrhip_graph_create")
 
     def voxels_ptr(self) -> int:

@@ -371,6 +371,42 @@ def test_mixed_size_batch(hip_device):
         assert np.array_equal(attrs[first[c]:first[c] + cpu.voxel_count], cpu.attrs)
 
 
+@pytest.mark.parametrize("finish", [1, 2])
+def test_batch_bucket_finish_bit_exact(hip_device, finish):
+    """Chunk batches through the full sort + unique scan and through the
+    bucket finish (buckets per chunk from its own size): packed and capacity
+    layouts, mixed sizes with chunks below and above 2^14 keys, empty chunks,
+    out-of-range records, duplicates and deletes."""
+    from pyrit_amd import workloads
+
+    g = _gpu()
+    print(f"case: batch finish {finish}, capacity layout", flush=True)
+    vox, cnt, offs, cap = _shell_batch()
+    ref, rroots, rfirst = O.gb_build_chunks(5, vox, offs, capacity=cap)
+    b, roots, first = g.build_chunks(5, vox, offs, capacity=cap, finish=finish)
+    assert_same_build(b, ref, 5)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+    print(f"case: batch finish {finish}, packed 120 chunks of 64^3", flush=True)
+    rng = np.random.default_rng(14)
+    chunks = [random_points(int(rng.integers(0, 6000)), 6, int(rng.integers(1 << 30)), dup_frac=0.2,
+                            delete_frac=0.05, out_frac=0.01) for _ in range(120)]
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in chunks])]).astype(np.uint32)
+    vox = np.concatenate(chunks)
+    ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs)
+    b, roots, first = g.build_chunks(6, vox, offs, finish=finish)
+    assert_same_build(b, ref, 6)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+    print(f"case: batch finish {finish}, mixed sizes 8..64", flush=True)
+    w = workloads.uneven_chunks(seed=9, count=40, edges=(8, 16, 32, 64))
+    vox, offs64 = w.generate_host()
+    offs = offs64.astype(np.uint32)
+    sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
+    ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs, chunk_log2=sizes)
+    b, roots, first = g.build_chunks(6, vox, offs, chunk_log2=sizes, finish=finish)
+    assert_same_build(b, ref, 6)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+
+
 def test_device_generator_matches_host(hip_device):
     print("case: terrain generator host == device", flush=True)
     from pyrit_amd import workloads

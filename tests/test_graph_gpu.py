@@ -89,7 +89,8 @@ def test_graph_replays_capacity_layout(hip_device):
         g.destroy()
 
 
-def test_graph_replays_mixed_sizes(hip_device):
+@pytest.mark.parametrize("finish", [1, 2])
+def test_graph_replays_mixed_sizes(hip_device, finish):
     from pyrit_amd import gpu, workloads
 
     ws = [workloads.uneven_chunks(seed=s, count=24, edges=(8, 16, 32, 64)) for s in (5, 6, 7)]
@@ -97,11 +98,11 @@ def test_graph_replays_mixed_sizes(hip_device):
     for w in ws:
         vox, offs64 = w.generate_host()
         data.append((vox, offs64.astype(np.uint32), np.array([c.log2_size for c in w.chunks], np.uint8)))
-    g = gpu.BuildGraph(6, 24, max(int(o[-1]) for _, o, _ in data), mixed_sizes=True)
+    g = gpu.BuildGraph(6, 24, max(int(o[-1]) for _, o, _ in data), mixed_sizes=True, finish=finish)
     try:
         for rep in range(2):
             for i, (vox, offs, sizes) in enumerate(data):
-                print(f"case: mixed-size graph replay {rep}/{i}", flush=True)
+                print(f"case: mixed-size graph replay {rep}/{i}, finish {finish}", flush=True)
                 ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs, chunk_log2=sizes)
                 out, roots, first = g.launch(vox, offs, chunk_log2=sizes)
                 _check(out, roots, first, ref, rroots, rfirst)
