@@ -129,9 +129,10 @@ def build_section(args, out):
         "autotune": tune,
     }
     # algorithmic bytes of the whole build: encode 16N read + rec*N write, P
-    # passes x 2*rec*N, unique rec*N read + 12U write, brick pass 8U read,
-    # outputs 4W + 8L (brick masks, tables and levels are small beside these)
-    alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 12.0 * U + 8.0 * U + 4.0 * W + 8.0 * Lc
+    # moving passes x 2*rec*N, unique (bucket tables or scan) rec*N read + 12U
+    # write, outputs 4W + 8L (brick keys and masks, hash tables and level
+    # tuples are small beside these)
+    alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 12.0 * U + 4.0 * W + 8.0 * Lc
     out["build"]["algorithmic_bytes"] = alg
     out["build"]["algorithmic_GBps"] = alg / (ms * 1e-3) / 1e9
     narrow = phases[-1].get("status_bytes", 8) == 4
@@ -308,10 +309,10 @@ def batch_section(args, out):
     bt = gpu.last_build_times()
     gpu.set_timing(False)
     # algorithmic bytes as for the single build: encode 16N read + rN write,
-    # P moving passes x 2rN, unique rN read + 12U write, bricks 8U, outputs
+    # P moving passes x 2rN, unique rN read + 12U write, outputs 4W + 8L
     rec, passes = bt["record_bytes"], max(1, bt["sort_passes"])
     U, W, Lc = built.voxel_count, built.node_words, built.leaf_count
-    b_alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 20.0 * U + 4.0 * W + 8.0 * Lc
+    b_alg = (16.0 + rec) * n + passes * 2.0 * rec * n + rec * n + 12.0 * U + 4.0 * W + 8.0 * Lc
     nonempty = roots != 0xFFFFFFFF
     pfirst = np.ascontiguousarray(np.append(first[nonempty], built.voxel_count), np.uint32)
     k = len(pfirst) - 1
@@ -356,7 +357,7 @@ def batch_section(args, out):
         "workload": w.name, "chunks": count, "voxels": n, "batched_ms": b_ms,
         "batched_gvoxels_per_s": n / (b_ms * 1e-3) / 1e9, "per_chunk_ms": p_ms,
         "per_chunk_gvoxels_per_s": n / (p_ms * 1e-3) / 1e9, "speedup_vs_per_chunk": p_ms / b_ms,
-        "sort_passes": passes, "record_bytes": rec, "unique": U, "algorithmic_bytes": b_alg,
+        "sort_passes": passes, "record_bytes": rec, "finish": bt["finish"], "unique": U, "algorithmic_bytes": b_alg,
         "algorithmic_GBps": b_alg / (b_ms * 1e-3) / 1e9, "hbm_frac": b_alg / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         "phase_ms": {k: bt[k] for k in ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "per_chunk_latency_ms_p50": float(np.percentile(lat, 50)),

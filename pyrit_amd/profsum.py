@@ -1,6 +1,6 @@
 """Summarise rocprofv3 output into the short text files kept under profiles/.
 
-  python -m pyrit_amd.profsum stats <kernel_stats.csv | results.db> --out profiles/X.md --cmd "..."
+  python -m pyrit_amd.profsum stats <kernel_stats.csv | results.db> [--trace kernel_trace.csv] --out profiles/X.md --cmd "..."
   python -m pyrit_amd.profsum traffic --fetch <counter csv | db> --write <counter csv | db>
          --voxels N --probe-bytes B --out profiles/traffic.json --cmd "..." --cmd "..."
 
@@ -10,7 +10,9 @@ standard library's sqlite3.
 
 `traffic` scales FETCH_SIZE and WRITE_SIZE (KiB per dispatch) by what the same
 counters read for k_copy_probe, a plain copy kernel that reads and writes
-exactly --probe-bytes, and reports HBM bytes per launch per kernel.
+exactly --probe-bytes, and reports HBM bytes per launch per kernel, averaged
+over the launches that did work (at least 5 % of the kernel's largest
+fetch: a radix pass whose digit is the same for every record returns at once).
 """
 from __future__ import annotations
 
@@ -43,18 +45,49 @@ def _kernel_rows(path: str) -> list[dict]:
     return rows
 
 
-def stats(path: str, out: str, cmds: list[str], top: int = 25) -> None:
+def _trace_resources(path: str) -> dict:
+    """kernel short name -> (arch VGPRs, LDS bytes, [durations ns]) from a kernel trace CSV"""
+    res = {}
+    for r in csv.DictReader(open(path)):
+        e = res.setdefault(short_name(r["Kernel_Name"]), (r.get("VGPR_Count"), r.get("LDS_Block_Size"), []))
+        e[2].append(float(r["End_Timestamp"]) - float(r["Start_Timestamp"]))
+    return res
+
+
+def _working_avg(durations: list) -> tuple:
+    """(count, mean) of the launches that did work: at least 5 % of the longest"""
+    top = max(durations)
+    work = [d for d in durations if d >= 0.05 * top]
+    return len(work), sum(work) / len(work)
+
+
+def stats(path: str, out: str, cmds: list[str], top: int = 25, trace: str | None = None) -> None:
     rows = _kernel_rows(path)
+    res = _trace_resources(trace) if trace else {}
+    for r in rows:
+        e = res.get(short_name(r["name"]))
+        if e:
+            if r["vgpr"] is None:
+                r["vgpr"], r["lds"] = e[0], e[1]
+            r["work"] = _working_avg(e[2])
     total = sum(r["total_ns"] for r in rows) or 1.0
     lines = ["# rocprofv3 kernel statistics", "", "Commands:", ""]
     lines += [f"    {c}" for c in cmds]
-    lines += ["", "| kernel | calls | total ms | avg us | min us | max us | % | VGPRs | LDS B |",
-              "|---|---|---|---|---|---|---|---|---|"]
+    lines += ["", "| kernel | calls | total ms | avg us | min us | max us | % | VGPRs | LDS B |"
+              + (" working calls | avg us of those |" if res else ""),
+              "|---|---|---|---|---|---|---|---|---|" + ("---|---|" if res else "")]
     for r in rows[:top]:
-        lines.append(f"| {short_name(r['name']) or '(unnamed)'} | {r['calls']} | {r['total_ns'] / 1e6:.3f} | "
-                     f"{r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} | "
-                     f"{100.0 * r['total_ns'] / total:.2f} | {r['vgpr'] if r['vgpr'] is not None else ''} | "
-                     f"{r['lds'] if r['lds'] is not None else ''} |")
+        line = (f"| {short_name(r['name']) or '(unnamed)'} | {r['calls']} | {r['total_ns'] / 1e6:.3f} | "
+                f"{r['avg_ns'] / 1e3:.1f} | {r['min_ns'] / 1e3:.1f} | {r['max_ns'] / 1e3:.1f} | "
+                f"{100.0 * r['total_ns'] / total:.2f} | {r['vgpr'] if r['vgpr'] is not None else ''} | "
+                f"{r['lds'] if r['lds'] is not None else ''} |")
+        if res:
+            wk = r.get("work")
+            line += f" {wk[0]} | {wk[1] / 1e3:.1f} |" if wk else " | |"
+        lines.append(line)
+    if res:
+        lines += ["", "Working calls: launches of at least 5 % of the kernel's longest (a radix pass whose digit "
+                  "is the same for every record returns at once)."]
     open(out, "w").write("\n".join(lines) + "\n")
 
 
@@ -86,9 +119,15 @@ def traffic(fetch: str, write: str, voxels: int, probe_bytes: int, out: str, cmd
     for k in f:
         if k == probe or not k:
             continue
-        fb = sum(f[k]) / len(f[k]) * 1024.0 * fscale
-        wb = sum(w.get(k, [0.0])) / max(1, len(w.get(k, [0.0]))) * 1024.0 * wscale
-        kernels.append({"kernel": k, "voxels": voxels, "launches": len(f[k]), "fetch_bytes": fb, "write_bytes": wb,
+        # launches that did work, by fetch; the write counts of the same launches
+        # (both runs launch the same sequence)
+        top_f = max(f[k])
+        work = [i for i, v in enumerate(f[k]) if v >= 0.05 * top_f] or list(range(len(f[k])))
+        wl = w.get(k, [])
+        fb = sum(f[k][i] for i in work) / len(work) * 1024.0 * fscale
+        wsel = [wl[i] for i in work if i < len(wl)]
+        wb = sum(wsel) / max(1, len(wsel)) * 1024.0 * wscale
+        kernels.append({"kernel": k, "voxels": voxels, "launches": len(work), "fetch_bytes": fb, "write_bytes": wb,
                         "bytes_per_launch": fb + wb})
     data = {"calibration": {"kernel": probe, "probe_bytes": probe_bytes, "fetch_KiB": fk, "write_KiB": wk,
                             "fetch_scale": fscale, "write_scale": wscale},
@@ -103,6 +142,7 @@ def main() -> None:
     s.add_argument("input")
     s.add_argument("--out", required=True)
     s.add_argument("--cmd", action="append", default=[])
+    s.add_argument("--trace", default=None, help="kernel trace CSV for VGPR and LDS columns")
     t = sub.add_parser("traffic")
     t.add_argument("--fetch", required=True)
     t.add_argument("--write", required=True)
@@ -112,7 +152,7 @@ def main() -> None:
     t.add_argument("--cmd", action="append", default=[])
     a = ap.parse_args()
     if a.what == "stats":
-        stats(a.input, a.out, a.cmd)
+        stats(a.input, a.out, a.cmd, trace=a.trace)
     else:
         traffic(a.fetch, a.write, a.voxels, a.probe_bytes, a.out, a.cmd)
 
