@@ -543,6 +543,27 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         if constexpr (NARROW) return w & 0x3FFFFFFFu;
         else return w & kValueMask;
     };
+    // publish the tile's counts first, then stage the tile in LDS in digit
+    // order (local offsets only) while the predecessors finish, then look back
+#pragma unroll
+    for (uint32_t j = 0; j < DPT; j++) {
+        const uint32_t d = tid * DPT + j;
+        SW* st;
+        if constexpr (NARROW) st = a.status32 + d;
+        else st = a.status + d;
+        __hip_atomic_store(&st[(uint64_t)tile * R], make(tile == td.w ? 2 : 1, tot[j]), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    __syncthreads();  // s_local is complete
+#pragma unroll
+    for (int i = 0; i < kSortItems; i++) {
+        if (wbase + i * kWave + lane < count) {
+            uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
+            uint32_t pos = s_local[d] + s_wave_cnt[wave][d] + rank[i];
+            s_keys[pos] = k[i];
+            if constexpr (!PACKED) s_vals[pos] = v[i];
+        }
+    }
 #pragma unroll
     for (uint32_t j = 0; j < DPT; j++) {
         const uint32_t d = tid * DPT + j;
@@ -550,10 +571,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         if constexpr (NARROW) st = a.status32 + d;
         else st = a.status + d;
         uint64_t excl = 0;
-        if (tile == td.w) {  // first tile of its segment
-            __hip_atomic_store(&st[(uint64_t)tile * R], make(2, tot[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            __hip_atomic_store(&st[(uint64_t)tile * R], make(1, tot[j]), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (tile != td.w) {  // not the first tile of its segment
             // predecessors are polled newest to oldest and summed until an
             // inclusive prefix: kPollFirst in the first round (the previous
             // tile's prefix is often there already), kPoll (independent loads
@@ -592,17 +610,6 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
                                __HIP_MEMORY_SCOPE_AGENT);
         }
         s_dst[d] = (uint64_t)seg_base[d] + excl;
-    }
-    __syncthreads();
-    // stage the tile in LDS in digit order
-#pragma unroll
-    for (int i = 0; i < kSortItems; i++) {
-        if (wbase + i * kWave + lane < count) {
-            uint32_t d = (uint32_t)(k[i] >> a.shift) & (R - 1);
-            uint32_t pos = s_local[d] + s_wave_cnt[wave][d] + rank[i];
-            s_keys[pos] = k[i];
-            if constexpr (!PACKED) s_vals[pos] = v[i];
-        }
     }
     __syncthreads();
     // coalesced write-out: consecutive slots of one digit are consecutive in HBM
