@@ -223,7 +223,21 @@ def trace_section(args, out):
         "last_frame_hit_fraction": hit_total / (W * H),
     }
     if args.cpu_trace:
-        out["traversal"]["cpu_baseline"] = cpu_trace_baseline(args, built, cams[0], W, H)
+        cb = cpu_trace_baseline(args, built, cams[0], W, H)
+        out["traversal"]["cpu_baseline"] = cb
+        # traversal roofline: 32 B ray + 16 B hit + 4 B per distinct node word,
+        # 8 B per distinct leaf and 4 B per attribute a ray reads, the distinct
+        # counts from the instrumented CPU trace of the same DAG (pose 0, half
+        # resolution); the primary kernel reads no ray (camera rays are made
+        # in registers), so its bytes are the hit plus the DAG reads
+        tc = cb["touch"]
+        per_ray = 16.0 + (4.0 * tc["node_words"] + 8.0 * tc["leaves"] + 4.0 * tc["attributes"]) / tc["rays"]
+        achieved = out["traversal"]["primary_grays_per_s"] * per_ray
+        out["traversal"]["roofline"] = {
+            "kernel": "k_trace_camera", "bound": "hbm", "bytes_per_ray": per_ray, "achieved": achieved,
+            "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
+            "note": "distinct DAG bytes per ray; most of them are served by L2 / MALL, not HBM"}
+        log(f"traversal: {per_ray:.0f} algorithmic bytes per primary ray, {achieved:.0f} GB/s")
     L.pyrhip_built_free(C.byref(built), None)
     for b in (hits, rays, occ):
         b.free()
@@ -254,6 +268,8 @@ def cpu_trace_baseline(args, built, cam, W, H):
         O.trace_rays(scene, rays, threads=threads)
         dt = time.perf_counter() - t0
         res[label] = {"value": len(rays) / dt / 1e6, "cores": threads, "seconds": dt}
+    # the same rays once more, instrumented: distinct words each ray reads
+    res["touch"] = O.trace_touch(scene, rays)
     log(f"cpu traversal {res['one_core']['value']:.2f} Mrays/s (1 core), "
         f"{res['all_cores']['value']:.1f} Mrays/s ({res['all_cores']['cores']} cores)")
     return res

@@ -778,16 +778,49 @@ int or_canon(const uint32_t* nodes, const uint64_t* leaves, uint32_t root, uint3
 
 static inline uint32_t bit_of(uint32_t v, uint32_t s) { return (v >> s) & 1; }
 
+/* Optional per-thread record of the distinct node words, leaves and
+ * attributes one ray's traversal reads (instrumentation for the traversal's
+ * algorithmic bytes, or_trace_touch; off otherwise). Entries are addresses,
+ * so several geometries of a scene never alias. */
+#define TOUCH_SLOTS 16384u
+typedef struct Touch {
+    uintptr_t key[TOUCH_SLOTS];
+    uint32_t gen[TOUCH_SLOTS];
+    uint32_t cur;
+    uint64_t count[3]; /* node words, leaves, attributes */
+} Touch;
+static __thread Touch* g_touch;
+
+static inline void touch(const void* addr, int kind) {
+    Touch* t = g_touch;
+    if (!t) return;
+    uintptr_t k = (uintptr_t)addr;
+    uint32_t h = (uint32_t)((k >> 2) * 0x9E3779B1u) & (TOUCH_SLOTS - 1);
+    for (uint32_t probe = 0; probe < TOUCH_SLOTS; probe++, h = (h + 1) & (TOUCH_SLOTS - 1)) {
+        if (t->gen[h] != t->cur) {
+            t->gen[h] = t->cur;
+            t->key[h] = k;
+            t->count[kind]++;
+            return;
+        }
+        if (t->key[h] == k) return;
+    }
+}
+
 /* attributeRank (dag.zig:70-87); stk[s] = node whose children have edge 2^s */
 static uint32_t attribute_rank(const OrDagView* g, const uint32_t* stk, const uint32_t v[3], uint64_t brick) {
     uint32_t rank = 0;
     for (uint32_t s = g->log2_size - 1; s >= 2; s--) {
         uint32_t node = stk[s];
         uint32_t mask = g->nodes[node] & 0xFF;
+        touch(&g->nodes[node], 0);
         uint32_t idx = bit_of(v[0], s) | (bit_of(v[1], s) << 1) | (bit_of(v[2], s) << 2);
         uint32_t before = (uint32_t)__builtin_popcount(mask & ((1u << idx) - 1));
         for (uint32_t k = 0; k < before; k++) {
             uint32_t ref = g->nodes[node + 2 + k];
+            touch(&g->nodes[node + 2 + k], 0);
+            if (s == 2) touch(&g->leaves[ref], 1);
+            else touch(&g->nodes[ref + 1], 0);
             rank += s == 2 ? (uint32_t)__builtin_popcountll(g->leaves[ref]) : g->nodes[ref + 1];
         }
     }
@@ -841,10 +874,14 @@ int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, 
         if (scale >= 2) {
             uint32_t node = stk[scale];
             uint32_t mask = g->nodes[node] & 0xFF;
+            touch(&g->nodes[node], 0);
             uint32_t idx = (bit_of(cell[0], scale) | (bit_of(cell[1], scale) << 1) | (bit_of(cell[2], scale) << 2)) ^ oct;
             uint32_t bit = 1u << idx;
             exists = (mask & bit) != 0;
-            if (exists) child = g->nodes[node + 2 + __builtin_popcount(mask & (bit - 1))];
+            if (exists) {
+                child = g->nodes[node + 2 + __builtin_popcount(mask & (bit - 1))];
+                touch(&g->nodes[node + 2 + __builtin_popcount(mask & (bit - 1))], 0);
+            }
         } else {
             uint32_t size = 1u << scale, l[3];
             for (int a = 0; a < 3; a++) l[a] = (((oct >> a) & 1) ? n - size - cell[a] : cell[a]) & 3;
@@ -859,7 +896,11 @@ int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, 
                     hit->voxel[a] = (int32_t)v[a];
                 }
                 uint32_t attribute = g->default_attribute;
-                if (want_attr && g->attrs) attribute = g->attrs[g->attr_base + attribute_rank(g, stk, v, brick)];
+                if (want_attr && g->attrs) {
+                    const uint32_t r = attribute_rank(g, stk, v, brick);
+                    touch(&g->attrs[g->attr_base + r], 2);
+                    attribute = g->attrs[g->attr_base + r];
+                }
                 int mirrored = ((oct >> axis) & 1) != 0;
                 hit->t = t;
                 hit->face = 2 * axis + (mirrored ? 0u : 1u);
@@ -868,7 +909,10 @@ int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, 
                 return 1;
             }
             if (scale >= 3) stk[scale - 1] = child;
-            else if (scale == 2) brick = g->leaves[child];
+            else if (scale == 2) {
+                brick = g->leaves[child];
+                touch(&g->leaves[child], 1);
+            }
             scale -= 1;
             uint32_t half = 1u << scale;
             float halff = (float)half;
@@ -988,6 +1032,26 @@ void or_trace_rays(const OrScene* s, const OrRay* rays, OrHit* hits, uint64_t co
     }
     if (threads > 1)
         for (uint32_t k = 0; k < threads; k++) pthread_join(th[k], NULL);
+}
+
+void or_trace_touch(const OrScene* s, const OrRay* rays, uint64_t count, uint32_t ray_mask, uint32_t flags,
+                    uint64_t out[4]) {
+    if (ray_mask == 0) ray_mask = 0xFFFFFFFFu;
+    Touch* t = (Touch*)calloc(1, sizeof(Touch));
+    memset(out, 0, 4 * sizeof(uint64_t));
+    if (!t) return;
+    g_touch = t;
+    for (uint64_t i = 0; i < count; i++) {
+        t->cur++; /* a new ray: every slot reads as empty */
+        OrHit h;
+        trace_one(s, rays[i].origin, rays[i].direction, rays[i].tmin, rays[i].tmax, ray_mask, flags, &h);
+        out[3] += h.instance != 0xFFFFFFFFu;
+    }
+    g_touch = NULL;
+    out[0] = t->count[0];
+    out[1] = t->count[1];
+    out[2] = t->count[2];
+    free(t);
 }
 
 /* ------------------------------------------------------------------------ */

@@ -115,6 +115,13 @@ typedef struct OrScene {
 void or_trace_rays(const OrScene* s, const OrRay* rays, OrHit* hits, uint64_t count, uint32_t ray_mask,
                    uint32_t flags, uint32_t threads);
 
+/* Instrumented trace (one thread): sums over the rays of the distinct node
+ * words, leaves and attributes each ray's traversal reads, and the hit count:
+ * out = {node words, leaves, attributes, hits}. The per-ray distinct bytes
+ * of the traversal's algorithmic-byte model. */
+void or_trace_touch(const OrScene* s, const OrRay* rays, uint64_t count, uint32_t ray_mask, uint32_t flags,
+                    uint64_t out[4]);
+
 /* Camera (types.zig:70-86) and helpers (root.zig:449-481, render.zig:16-32) */
 typedef struct OrCamera {
     float view_to_world[12]; uint32_t projection, width, height;

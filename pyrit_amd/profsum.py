@@ -3,6 +3,7 @@
   python -m pyrit_amd.profsum stats <kernel_stats.csv | results.db> [--trace kernel_trace.csv] --out profiles/X.md --cmd "..."
   python -m pyrit_amd.profsum traffic --fetch <counter csv | db> --write <counter csv | db>
          --voxels N --probe-bytes B --out profiles/traffic.json --cmd "..." --cmd "..."
+  python -m pyrit_amd.profsum counters <counter csv | db> --kernel NAME [--kernel ...] --out X.json --cmd "..."
 
 Inputs are either rocprofv3's CSV files (`--output-format csv`) or its rocpd
 SQLite database (`*_results.db`, the default output format), read with the
@@ -135,6 +136,28 @@ def traffic(fetch: str, write: str, voxels: int, probe_bytes: int, out: str, cmd
     json.dump(data, open(out, "w"), indent=1)
 
 
+def counters(path: str, kernels: list[str], out: str, cmds: list[str]) -> None:
+    """mean per launch of every counter of the given kernels, and the L2 hit
+    rate TCC_HIT / (TCC_HIT + TCC_MISS) when both were collected"""
+    names = set()
+    if path.endswith(".db"):
+        db = sqlite3.connect(path)
+        names = {r[0] for r in db.execute("select distinct counter_name from counters_collection")}
+    else:
+        names = {r["Counter_Name"] for r in csv.DictReader(open(path))}
+    res = {}
+    for c in sorted(names):
+        per = _counter(path, c)
+        for k in kernels:
+            if k in per:
+                res.setdefault(k, {"launches": len(per[k])})[c] = sum(per[k]) / len(per[k])
+    for k, v in res.items():
+        hit, miss = v.get("TCC_HIT_sum"), v.get("TCC_MISS_sum")
+        if hit is not None and miss is not None and hit + miss > 0:
+            v["l2_hit_rate"] = hit / (hit + miss)
+    json.dump({"commands": cmds, "kernels": res}, open(out, "w"), indent=1)
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     sub = ap.add_subparsers(dest="what", required=True)
@@ -150,9 +173,16 @@ def main() -> None:
     t.add_argument("--probe-bytes", type=int, required=True)
     t.add_argument("--out", required=True)
     t.add_argument("--cmd", action="append", default=[])
+    c = sub.add_parser("counters")
+    c.add_argument("input")
+    c.add_argument("--kernel", action="append", required=True)
+    c.add_argument("--out", required=True)
+    c.add_argument("--cmd", action="append", default=[])
     a = ap.parse_args()
     if a.what == "stats":
         stats(a.input, a.out, a.cmd, trace=a.trace)
+    elif a.what == "counters":
+        counters(a.input, a.kernel, a.out, a.cmd)
     else:
         traffic(a.fetch, a.write, a.voxels, a.probe_bytes, a.out, a.cmd)
 

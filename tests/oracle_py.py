@@ -94,6 +94,7 @@ def lib() -> C.CDLL:
         L.or_canon.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(OrDag)]
         L.or_walk.argtypes = [P(OrDagView), P(f32), P(f32), f32, f32, i32, P(OrDagHit), P(u32)]
         L.or_trace_rays.argtypes = [P(OrScene), C.c_void_p, C.c_void_p, u64, u32, u32, u32]
+        L.or_trace_touch.argtypes = [P(OrScene), C.c_void_p, u64, u32, u32, P(u64)]
         L.or_camera_look_at.argtypes = [P(OrCamera), P(f32), P(f32), P(f32)]
         L.or_camera_perspective.argtypes = [P(OrCamera), f32, u32, u32, f32]
         L.or_camera_rays.argtypes = [P(OrCamera), C.c_void_p]
@@ -281,6 +282,15 @@ def trace_rays(scene: OrScene, rays: np.ndarray, ray_mask=0, flags=0, threads=8)
     hits = np.zeros(len(rays), HIT)
     lib().or_trace_rays(C.byref(scene), _p(rays), _p(hits), len(rays), ray_mask, flags, threads)
     return hits
+
+
+def trace_touch(scene: OrScene, rays: np.ndarray, ray_mask=0, flags=0) -> dict:
+    """distinct node words, leaves and attributes read per ray, summed over the
+    rays, and the hit count (one thread)"""
+    rays = np.ascontiguousarray(rays, dtype=RAY)
+    out = (C.c_uint64 * 4)()
+    lib().or_trace_touch(C.byref(scene), _p(rays), len(rays), ray_mask, flags, out)
+    return {"node_words": out[0], "leaves": out[1], "attributes": out[2], "hits": out[3], "rays": len(rays)}
 
 
 def trace_dag_rays(dag: Dag, rays: np.ndarray, flags=0) -> np.ndarray:

@@ -248,6 +248,35 @@ def test_axis_rays_known_answers():
     assert h.inside
 
 
+def test_trace_touch_counts():
+    """Distinct words read per ray (traversal algorithmic bytes), pinned by
+    hand on the one-voxel 8^3 DAG: the root is the only node (mask word +
+    one child ref), the child is the one leaf."""
+    d = O.build_points(3, np.array([(5, 6, 7, 42)], O.VOXEL))
+    scene = O.single_scene(d)
+
+    def rays(*rs):
+        out = np.zeros(len(rs), O.RAY)
+        for i, (o, dv) in enumerate(rs):
+            out[i]["origin"], out[i]["direction"], out[i]["tmin"], out[i]["tmax"] = o, dv, 0.0, 1e30
+        return out
+
+    hit = ((-10, 6.5, 7.5), (1, 0, 0))   # enters an empty octant, then the voxel's: hit
+    miss = ((-10, 6.5, 6.5), (1, 0, 0))  # same octants and leaf, misses the voxel
+    away = ((-10, 20.0, 7.5), (1, 0, 0))  # outside the instance bounds: reads nothing
+    t = O.trace_touch(scene, rays(hit))
+    assert (t["node_words"], t["leaves"], t["attributes"], t["hits"]) == (2, 1, 1, 1)
+    t = O.trace_touch(scene, rays(miss))
+    assert (t["node_words"], t["leaves"], t["attributes"], t["hits"]) == (2, 1, 0, 0)
+    t = O.trace_touch(scene, rays(away))
+    assert (t["node_words"], t["leaves"], t["attributes"], t["hits"]) == (0, 0, 0, 0)
+    # distinct per ray: two equal rays count twice; hits agree with the tracer
+    t = O.trace_touch(scene, rays(hit, hit, miss))
+    assert (t["node_words"], t["leaves"], t["attributes"], t["hits"]) == (6, 3, 2, 2)
+    hits = O.trace_rays(scene, rays(hit, hit, miss), threads=1)
+    assert int((hits["instance"] != 0xFFFFFFFF).sum()) == 2
+
+
 @pytest.mark.parametrize("log2,rays", [(6, 20000), (8, 4000)])
 def test_walk_against_f64_dda(log2, rays):
     # cpu_test.zig:42-103: random rays vs an f64 Amanatides-Woo DDA on the

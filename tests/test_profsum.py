@@ -57,3 +57,19 @@ def test_traffic_scaling_and_working_launches(tmp_path):
     assert k["launches"] == 2 and k["voxels"] == 123
     assert abs(k["fetch_bytes"] - 1100 * 1024 * 2.0) < 1e-6
     assert abs(k["write_bytes"] - 900 * 1024 * 1.0) < 1e-6
+
+
+def test_counters_l2_hit_rate(tmp_path):
+    hdr = ["Kernel_Name", "Counter_Name", "Counter_Value"]
+    src = tmp_path / "c.csv"
+    _write(src, hdr, [["void pyr::k_trace_camera(pyr::DagView)", "TCC_HIT_sum", 300],
+                      ["void pyr::k_trace_camera(pyr::DagView)", "TCC_MISS_sum", 100],
+                      ["void pyr::k_trace_camera(pyr::DagView)", "TCC_HIT_sum", 500],
+                      ["void pyr::k_trace_camera(pyr::DagView)", "TCC_MISS_sum", 100],
+                      ["k_other()", "TCC_HIT_sum", 1]])
+    out = tmp_path / "o.json"
+    profsum.counters(str(src), ["k_trace_camera"], str(out), ["cmd"])
+    d = json.loads(out.read_text())["kernels"]
+    assert list(d) == ["k_trace_camera"]
+    k = d["k_trace_camera"]
+    assert k["TCC_HIT_sum"] == 400 and k["TCC_MISS_sum"] == 100 and abs(k["l2_hit_rate"] - 0.8) < 1e-12
