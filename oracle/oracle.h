@@ -26,6 +26,11 @@
  *   or_palette_pack     : decoding the packed words reproduces every attribute
  *                         (brute force), the 16-value boundary and the word
  *                         counts of context.zig:1352-1357
+ *   or_trace_touch      : word counts derived by hand on a one-voxel DAG (hit,
+ *                         miss through the same leaf, ray outside the bounds),
+ *                         per-ray distinctness, hit count equal to the tracer's
+ *                         (measurement instrument for the traversal's
+ *                         algorithmic bytes, not a checker)
  */
 #ifndef PYRIT_ORACLE_H
 #define PYRIT_ORACLE_H
