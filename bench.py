@@ -465,7 +465,9 @@ def graph_measure(top, vox, offs, sizes):
 def serve_section(args, out):
     """Poisson arrivals (mean 2000 requests/s) of chunk builds with the uneven
     sizes of the batched workload, mixed 80/20 with edits of 1..512 voxels of a
-    resident chunk; 64 GB device budget; latency = submit -> DAG resident."""
+    resident chunk; 64 GB device budget; latency = submit -> DAG resident.
+    Run with the server's graph replays of small batches (its default) and
+    again with every batch built eagerly."""
     from pyrit_amd import serving, workloads
 
     count = 1024 if not args.quick else 128
@@ -473,35 +475,48 @@ def serve_section(args, out):
     log(f"serving: generating {w.name} inputs")
     vox, offs, n = w.generate_device()
     inputs = [(w.chunks[c].log2_size, vox.ptr + int(offs[c]) * 16, int(offs[c + 1] - offs[c])) for c in range(count)]
-    sv = serving.Server(budget=64 << 30, workers=2)
-    # warm the pool and the kernels with one batch
-    for c in range(8):
-        sv.build(c, *inputs[c])
-    sv.drain()
-    sv.poll()
     duration = 3.0 if not args.quick else 1.0
-    res, elapsed, n_req = serving.poisson_run(sv, inputs, rate=2000.0, duration=duration, seed=7)
-    st = sv.stats()
-    lat = np.array([(c.done_s - c.submit_s) * 1e3 for c in res if c.result == 0])
-    builds = [c for c in res if c.user == 0 and c.result == 0]
-    edits = [c for c in res if c.user == 1 and c.result == 0]
-    bl = np.array([(c.done_s - c.submit_s) * 1e3 for c in builds]) if builds else np.zeros(1)
-    el = np.array([(c.done_s - c.submit_s) * 1e3 for c in edits]) if edits else np.zeros(1)
-    sizes = np.array([c.batch_chunks for c in res if c.result == 0])
-    out["serving"] = {
-        "workload": f"Poisson 2000 req/s, 80% builds ({w.name} sizes) / 20% edits of 1..512 voxels, 64 GB budget",
-        "requests": n_req, "completed": len(res), "rejected": int(sum(1 for c in res if c.result != 0)),
-        "elapsed_s": elapsed, "builds_per_s": len(builds) / elapsed, "requests_per_s": len(res) / elapsed,
-        "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
-        "build_latency_ms_p50": float(np.percentile(bl, 50)), "build_latency_ms_p99": float(np.percentile(bl, 99)),
-        "edit_latency_ms_p50": float(np.percentile(el, 50)), "edit_latency_ms_p99": float(np.percentile(el, 99)),
-        "batches": st["batches"], "mean_batch_chunks": float(sizes.mean()) if len(sizes) else 0.0,
-        "device_busy_fraction": st["busy_s"] / elapsed, "voxels_built": st["batched_voxels"],
-        "resident_chunks": st["resident_chunks"], "resident_GB": st["resident_bytes"] / 1e9,
-    }
-    sv.close()
-    vox.free()
 
+    def run(graphs):
+        sv = serving.Server(budget=64 << 30, workers=2, graphs=graphs)
+        # warm the pool and the kernels (and the graphs) with a few batches
+        for c in range(8):
+            sv.build(c, *inputs[c])
+        sv.drain()
+        for c in range(8):
+            sv.build(c, *inputs[c])
+            sv.drain()
+        sv.poll()
+        res, elapsed, n_req = serving.poisson_run(sv, inputs, rate=2000.0, duration=duration, seed=7)
+        st = sv.stats()
+        sv.close()
+        lat = np.array([(c.done_s - c.submit_s) * 1e3 for c in res if c.result == 0])
+        builds = [c for c in res if c.user == 0 and c.result == 0]
+        edits = [c for c in res if c.user == 1 and c.result == 0]
+        bl = np.array([(c.done_s - c.submit_s) * 1e3 for c in builds]) if builds else np.zeros(1)
+        el = np.array([(c.done_s - c.submit_s) * 1e3 for c in edits]) if edits else np.zeros(1)
+        sizes = np.array([c.batch_chunks for c in res if c.result == 0])
+        return {
+            "requests": n_req, "completed": len(res), "rejected": int(sum(1 for c in res if c.result != 0)),
+            "elapsed_s": elapsed, "builds_per_s": len(builds) / elapsed, "requests_per_s": len(res) / elapsed,
+            "latency_ms_p50": float(np.percentile(lat, 50)), "latency_ms_p99": float(np.percentile(lat, 99)),
+            "build_latency_ms_p50": float(np.percentile(bl, 50)),
+            "build_latency_ms_p99": float(np.percentile(bl, 99)),
+            "edit_latency_ms_p50": float(np.percentile(el, 50)), "edit_latency_ms_p99": float(np.percentile(el, 99)),
+            "batches": st["batches"], "mean_batch_chunks": float(sizes.mean()) if len(sizes) else 0.0,
+            "device_busy_fraction": st["busy_s"] / elapsed, "voxels_built": st["batched_voxels"],
+            "resident_chunks": st["resident_chunks"], "resident_GB": st["resident_bytes"] / 1e9,
+        }
+
+    res = run(True)
+    eager = run(False)
+    res["workload"] = (f"Poisson 2000 req/s, 80% builds ({w.name} sizes) / 20% edits of 1..512 voxels, 64 GB budget; "
+                       "small batches replay HIP graphs")
+    res["eager"] = eager
+    out["serving"] = res
+    log(f"serving eager: p50 {eager['latency_ms_p50']:.2f} ms p99 {eager['latency_ms_p99']:.2f} ms, "
+        f"{eager['builds_per_s']:.0f} builds/s")
+    vox.free()
 
 def cpu_section(args, out):
     """The oracle's restatement of dag_builder.buildPoints (single-threaded by
@@ -585,7 +600,7 @@ def main() -> int:
             f"of attribute bytes saved, {pi['ms']:.2f} ms ({pi['GBps']:.0f} GB/s)")
     if "serve" in only:
         serve_section(args, out)
-        log(f"serving p50 {out['serving']['latency_ms_p50']:.1f} ms p99 {out['serving']['latency_ms_p99']:.1f} ms, "
+        log(f"serving p50 {out['serving']['latency_ms_p50']:.2f} ms p99 {out['serving']['latency_ms_p99']:.2f} ms, "
             f"{out['serving']['builds_per_s']:.0f} builds/s")
     if "trace" in only:
         trace_section(args, out)

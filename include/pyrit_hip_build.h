@@ -284,9 +284,15 @@ typedef struct PyrhipServerInfo {
     uint32_t max_batch_chunks;  /* 0 = 1024 */
     uint32_t max_batch_voxels;  /* records per batch, 0 = 2^28 */
     uint32_t workers;           /* batches in flight (one stream each), 0 = 2 */
-    uint32_t flags;
+    uint32_t flags;             /* PYRHIP_SERVER_* */
     PyrhipBuildTuning tuning;
 } PyrhipServerInfo;
+
+/* Build every batch with pyrhip_build_chunks. Without it, a batch of at most
+ * 64 chunks and 2^20 records replays a HIP graph (pyrhip_graph_*, one per
+ * worker and largest chunk edge, about 64 MB each, made on first use and kept
+ * until pyrhip_server_destroy; not counted in memory_budget). */
+#define PYRHIP_SERVER_NO_GRAPHS 1u
 
 #define PYRHIP_REQUEST_BUILD 0u /* (re)build a chunk from voxel records (device memory) */
 #define PYRHIP_REQUEST_EDIT 1u  /* set / remove voxels of a resident chunk (host records, copied) */

@@ -134,6 +134,7 @@ class PyrhipServerStats(C.Structure):
 
 PYRHIP_REQUEST_BUILD = 0
 PYRHIP_REQUEST_EDIT = 1
+PYRHIP_SERVER_NO_GRAPHS = 1
 
 
 class PyrChunkKey(C.Structure):

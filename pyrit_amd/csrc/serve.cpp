@@ -11,7 +11,9 @@
 // src/gpu_build.zig:90-96); build the whole batch with pyrhip_build_chunks
 // (mixed chunk edges); copy the outputs into exact-size buffers (a
 // "generation" shared by the batch's chunks, freed when its last chunk is
-// replaced); complete the requests.
+// replaced); complete the requests. A small batch (at most kGraphChunks
+// chunks and kGraphVoxels records) replays a HIP graph of the batch build
+// instead (one per worker and largest edge), padded with empty chunks.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -32,6 +34,11 @@
 
 namespace pyr {
 namespace {
+
+constexpr uint32_t kGraphChunks = 64;
+constexpr uint32_t kGraphVoxels = 1u << 20;
+
+using GraphCache = std::unordered_map<uint32_t, PyrhipGraph*>;  // largest chunk edge -> graph
 
 double steady_s() {
     using namespace std::chrono;
@@ -182,12 +189,12 @@ struct PyrhipServer {
     void worker_main();
     PyrResult run_batch(std::vector<Req>& batch, hipStream_t s, void** input, uint64_t* input_cap, double* dev_ms,
                         std::shared_ptr<Generation>* gen, std::vector<uint32_t>& roots, std::vector<uint32_t>& first,
-                        std::vector<uint32_t>& counts);
+                        std::vector<uint32_t>& counts, GraphCache* graphs);
 };
 
 PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void** input, uint64_t* input_cap,
                                   double* dev_ms, std::shared_ptr<Generation>* gen_out, std::vector<uint32_t>& roots,
-                                  std::vector<uint32_t>& first, std::vector<uint32_t>& counts) {
+                                  std::vector<uint32_t>& first, std::vector<uint32_t>& counts, GraphCache* graphs) {
     const uint32_t k = (uint32_t)batch.size();
     std::vector<uint32_t> offs(k + 1, 0);
     std::vector<uint8_t> l2(k, 0);
@@ -264,7 +271,28 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
     PyrhipBuilt b{};
-    r = pyrhip_build_chunks(top, &cb, &info.tuning, s, &b, roots.data(), first.data());
+    // a small batch replays the graph of its largest edge (outputs stay in the
+    // graph's buffers: copied below like the eager build's, never freed)
+    PyrhipGraph* graph = nullptr;
+    if (graphs && !(info.flags & PYRHIP_SERVER_NO_GRAPHS) && k <= kGraphChunks && total <= kGraphVoxels) {
+        PyrhipGraph*& slot = (*graphs)[top];
+        if (!slot && pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
+            slot = nullptr;  // not enough memory for it: build eagerly
+            diag_clear();
+        }
+        graph = slot;
+    }
+    if (graph) {
+        std::vector<uint32_t> po(kGraphChunks + 1, (uint32_t)total), gr(kGraphChunks), gf(kGraphChunks);
+        std::vector<uint8_t> pl(kGraphChunks, 3);
+        std::copy(offs.begin(), offs.end(), po.begin());
+        std::copy(l2.begin(), l2.end(), pl.begin());
+        r = pyrhip_graph_launch(graph, (uint64_t)(uintptr_t)vox, po.data(), pl.data(), s, &b, gr.data(), gf.data());
+        std::copy(gr.begin(), gr.begin() + k, roots.begin());
+        std::copy(gf.begin(), gf.begin() + k, first.begin());
+    } else {
+        r = pyrhip_build_chunks(top, &cb, &info.tuning, s, &b, roots.data(), first.data());
+    }
     (void)hipEventRecord(e1, s);
     // copy the outputs into exact-size buffers: the build allocated them by
     // upper bounds, and the generation stays resident
@@ -287,7 +315,7 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         }
         g->bytes = nb + lb + kb + ab;
         const uint32_t batch_voxels = b.voxel_count;
-        pyrhip_built_free(&b, s);  // zeroes b
+        if (!graph) pyrhip_built_free(&b, s);  // zeroes b
         if (ok && hipStreamSynchronize(s) != hipSuccess) ok = false;
         if (!ok) {
             hipGetLastError();
@@ -323,6 +351,7 @@ void PyrhipServer::worker_main() {
     (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     void* input = nullptr;
     uint64_t input_cap = 0;
+    GraphCache graphs;
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
         std::vector<Req> batch;
@@ -338,7 +367,7 @@ void PyrhipServer::worker_main() {
         std::shared_ptr<Generation> g;
         std::vector<uint32_t> roots, first, counts;
         double dev_ms = 0;
-        PyrResult r = run_batch(batch, s, &input, &input_cap, &dev_ms, &g, roots, first, counts);
+        PyrResult r = run_batch(batch, s, &input, &input_cap, &dev_ms, &g, roots, first, counts, &graphs);
         lk.lock();
         uint64_t est = 0;
         for (const Req& q : batch) est += q.est;
@@ -383,6 +412,7 @@ void PyrhipServer::worker_main() {
     lk.unlock();
     pool_free(input, s);
     (void)hipStreamSynchronize(s);
+    for (auto& kv : graphs) pyrhip_graph_destroy(kv.second);
     (void)hipStreamDestroy(s);
 }
 

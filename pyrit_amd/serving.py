@@ -16,10 +16,13 @@ from .device import download
 
 class Server:
     def __init__(self, budget: int = 64 << 30, workers: int = 2, max_batch_chunks: int = 1024,
-                 max_batch_voxels: int = 1 << 28, radix_bits: int = 0, sort_items: int = 0):
+                 max_batch_voxels: int = 1 << 28, radix_bits: int = 0, sort_items: int = 0, graphs: bool = True):
+        """graphs=False: every batch eagerly (PYRHIP_SERVER_NO_GRAPHS), else
+        small batches replay a HIP graph of the batch build"""
         self.L = lib()
         info = abi.PyrhipServerInfo()
         info.memory_budget, info.workers = budget, workers
+        info.flags = 0 if graphs else abi.PYRHIP_SERVER_NO_GRAPHS
         info.max_batch_chunks, info.max_batch_voxels = max_batch_chunks, max_batch_voxels
         info.tuning.radix_bits, info.tuning.sort_items = radix_bits, sort_items
         self.h = C.c_void_p()

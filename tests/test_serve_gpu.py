@@ -68,8 +68,11 @@ def _chunk_canon(server, c):
     return O.canon(nodes, leaves, root, l2), attrs
 
 
-def test_same_alone_and_in_the_mix(hip_device):
-    print("case: server alone vs mixed", flush=True)
+@pytest.mark.parametrize("graphs", [True, False])
+def test_same_alone_and_in_the_mix(hip_device, graphs):
+    """graphs: small batches replay HIP graphs of the batch build (the
+    default), or every batch runs eagerly"""
+    print(f"case: server alone vs mixed, graphs {graphs}", flush=True)
     from pyrit_amd.device import to_device
     from pyrit_amd.serving import Server
 
@@ -77,9 +80,9 @@ def test_same_alone_and_in_the_mix(hip_device):
     dev = [(to_device(p) if len(p) else None, len(p)) for _, p in inputs]
     expect = _final_sets(inputs, script)
 
-    alone = Server(workers=1)
+    alone = Server(workers=1, graphs=graphs)
     ra = _run(alone, dev, inputs, script, alone=True)
-    mixed = Server(workers=2)
+    mixed = Server(workers=2, graphs=graphs)
     rm = _run(mixed, dev, inputs, script, alone=False)
     assert all(c.result == 0 for c in ra) and all(c.result == 0 for c in rm)
     assert len(ra) == len(rm) == len(script)
