@@ -290,7 +290,7 @@ typedef struct PyrhipServerInfo {
 
 /* Build every batch with pyrhip_build_chunks. Without it, a batch of at most
  * 64 chunks and 2^20 records replays a HIP graph (pyrhip_graph_*, one per
- * worker and largest chunk edge, about 64 MB each, made on first use and kept
+ * worker and largest chunk edge, up to about 100 MB each, made on first use and kept
  * until pyrhip_server_destroy; not counted in memory_budget). */
 #define PYRHIP_SERVER_NO_GRAPHS 1u
 
