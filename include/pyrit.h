@@ -378,7 +378,9 @@ typedef struct PyrWorldInfo {
 typedef struct PyrWorldStats {
     uint32_t visible_chunks, resident_chunks, pending_chunks;
     uint32_t built_chunks, built_voxels; /* in the last update */
-    uint32_t overflow_chunks;            /* chunks whose generator output exceeded the capacity (total) */
+    uint32_t overflow_chunks;            /* generator overflows (total): each grew the capacity per chunk and
+                                            generated the job again; a chunk counts once more if it had to be
+                                            clipped (the capacity could not grow) */
     float voxel_pixels;
     uint32_t top_lod;
     float secondary_bias;
