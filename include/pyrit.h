@@ -365,8 +365,9 @@ typedef struct PyrWorldInfo {
     int32_t y_min, y_max;       /* vertical extent in base voxels (required) */
     uint32_t chunks_per_update; /* chunks per generated batch, 0 = chosen from the capacity */
     uint32_t mask;              /* instance mask, 0 = 0x1 */
-    uint32_t secondary_mask;    /* accepted; this implementation keeps one selection */
-    float secondary_pixels;
+    uint32_t secondary_mask;    /* instance mask bits of a second, coarser selection of the resident chunks
+                                   (seen by shadow / GI rays); 0 = none */
+    float secondary_pixels;     /* its target voxel size in pixels, 0 = 4 * voxel_pixels */
     uint32_t chunk_capacity;    /* generator slots per chunk, 0 = 8 * 4^chunk_log2 (at most 8^chunk_log2) */
     uint32_t rt_leaf_log2;      /* accepted (no RT cores on this device) */
     uint32_t keep_frames;       /* frames an unused chunk stays resident, 0 = 8 */
