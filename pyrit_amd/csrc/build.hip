@@ -193,7 +193,10 @@ __device__ __forceinline__ int4 fetch_voxel(const EncodeArgs& a, uint32_t seg, u
     if (i < a.n_old || (a.capacity && local >= a.capacity)) return make_int4(-1, -1, -1, 0);
     const uint64_t src = a.chunk_count == 0 ? (uint64_t)(i - a.n_old)
                                             : (a.capacity ? (uint64_t)seg * a.capacity + local : (uint64_t)i);
-    return a.voxels[src];
+    // (every input record is read once: no reason to keep it in the caches)
+    typedef int v4i __attribute__((ext_vector_type(4)));
+    const v4i r = __builtin_nontemporal_load(reinterpret_cast<const v4i*>(a.voxels) + src);
+    return make_int4(r.x, r.y, r.z, r.w);
 }
 
 // Chunk-local sort key and attribute of record i from its fetched voxel;
@@ -233,7 +236,7 @@ constexpr int kEncodeUnroll = 4;  // records per thread whose loads are in fligh
 __device__ __forceinline__ void emit_record(const EncodeArgs& a, uint32_t* s_hist, uint32_t radix, uint32_t i,
                                             uint64_t key, uint32_t val) {
     if (a.packed) {
-        a.keys[i] = (key << 32) | val;
+        __builtin_nontemporal_store((key << 32) | val, &a.keys[i]);
     } else {
         a.keys[i] = key;
         a.vals[i] = val;
@@ -435,7 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     for (int i = 0; i < kSortItems; i++) {
         const uint32_t li = wbase + i * kWave + lane;
         bool ok = li < count;
-        k[i] = ok ? keys_in[td.y + li] : 0;
+        k[i] = ok ? __builtin_nontemporal_load(&keys_in[td.y + li]) : 0;
         if constexpr (!PACKED) v[i] = ok ? vals_in[td.y + li] : 0;
     }
     // stable ranking inside the wave: up to a.peel digits of the row are
@@ -621,7 +624,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
             atomicOr(a.err, 16u);
             continue;
         }
-        keys_out[dst] = key;
+        __builtin_nontemporal_store(key, &keys_out[dst]);
         if constexpr (!PACKED) vals_out[dst] = s_vals[j];
     }
 }
