@@ -1069,7 +1069,7 @@ __global__ void __launch_bounds__(kBucketBlock, 4)
 #pragma unroll
                 for (int u = 0; u < kBucketRegs; u++) {
                     const uint32_t j = s + u * kBucketBlock + threadIdx.x;
-                    const uint64_t w = j < e ? keys[j] : ~0ull;
+                    const uint64_t w = j < e ? __builtin_nontemporal_load(&keys[j]) : ~0ull;
                     sl[u] = j < e ? phys((uint32_t)(w >> 32) & (slots - 1)) : ~0u;
                     at[u] = (uint32_t)w;
                 }
