@@ -490,9 +490,35 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
         g->own_attr_count = caps[k];
         g->data.root = j.out_root;
         g->data.attribute_offset = j.attr_dst;
-        g->data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;  // 32-bit attributes from here on
+        g->data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;  // 32-bit attributes ...
         g->data.palette_offset = 0;
         g->voxel_count = j.out_count;
+        // ... unless the chunk was palette packed and its edited attributes
+        // still take at most 16 values: packed again (palette.zig:1-152)
+        if (j.palette && j.out_count && !getenv("PYRIT_NO_PALETTE")) {
+            const std::vector<uint32_t> lo = {0u, j.out_count};
+            std::vector<uint32_t> counts;
+            PaletteScratch ps;
+            PyrResult pr = palette_scan(attrs + j.attr_dst, lo, s, &counts, &ps);
+            if (pr == PYR_OK && counts[0] != kPaletteOverflow) {
+                const uint64_t words = palette_words(counts[0], j.out_count);
+                const int64_t pa = ctx->attrs.ra.alloc(words);
+                if (pa >= 0 && palette_pack(attrs + j.attr_dst, lo, {(uint64_t)pa}, attrs, s, &ps) == PYR_OK) {
+                    // (the 32-bit range is reused only by later work on this stream)
+                    ctx->attrs.ra.release(j.attr_dst, caps[k]);
+                    g->own_attr_offset = (uint64_t)pa;
+                    g->own_attr_count = words;
+                    g->data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES | PYR_GEOMETRY_PALETTE;
+                    g->data.palette_offset = (uint32_t)pa;
+                    g->data.attribute_offset = (uint32_t)pa + 16;
+                } else if (pa >= 0) {
+                    ctx->attrs.ra.release((uint64_t)pa, words);
+                }
+            }
+            palette_release(&ps, s);
+            (void)hipGetLastError();
+            diag_clear();
+        }
         if (!g->uses_arena) {
             g->uses_arena = true;
             ctx->arena_users++;
