@@ -7,11 +7,15 @@
 // attributes in DFS rank order. The pipeline is this file's own:
 //
 //   encode      PyrVoxel -> (key, attr) + LDS histograms of every radix digit
-//   sort        P onesweep LSD passes: wave-ballot ranking, per-wave digit
+//   sort        P onesweep LSD passes: in-wave ranking by LDS lane masks (and
+//               ballots for the first digits of a row), per-wave digit
 //               counters in LDS, decoupled look-back per digit, LDS-staged
-//               coalesced scatter
-//   unique      last record of a key wins, attribute 0 deletes (tile scan)
-//   bricks      brick heads (tile scan), 64-bit masks by wave-segmented OR
+//               coalesced scatter; with the bucket finish only the key bits
+//               from 14 up are sorted
+//   unique +    either one fused tile scan over the sorted records (last
+//   bricks      record of a key wins, attribute 0 deletes, brick heads, masks
+//               by wave-segmented OR) or the bucket finish: one LDS key table
+//               per bucket of 2^14 keys (atomic max of the record index)
 //   leaves      open-addressing HBM hash, 64-bit CAS, owner = atomicMin index,
 //               compaction by tile scan
 //   levels      per level: parent tuples (tile scan), tuple hash (CAS of
@@ -21,7 +25,7 @@
 //
 // Every count lives on the device; kernels are launched with host upper bounds
 // and read the true counts, so one build is a fixed launch sequence with a
-// single read-back at the end (graph-capturable).
+// single read-back at the end (graph-capturable: pyrhip_graph_*).
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
