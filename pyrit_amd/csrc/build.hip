@@ -1709,6 +1709,16 @@ struct GraphBuild {
     hipGraphExec_t exec = nullptr;
 };
 
+// Whether a build of n records (chunk_count chunks, 0 = single) takes the
+// bucket finish under the resolved tuning `tun`.
+bool bucket_finish(const PyrhipBuildTuning& tun, uint32_t log2_size, uint64_t n, uint32_t chunk_count) {
+    const uint32_t lbits = local_bits(log2_size);
+    const uint32_t lb = std::min(kBucketBits, lbits);
+    const uint64_t cap = (chunk_count ? (uint64_t)chunk_count : 1ull) << (lbits - lb);
+    if (lbits - lb > 20 || cap > (1ull << 24)) return false;
+    return tun.finish == 2 || (tun.finish == 0 && n >= (chunk_count ? 512ull : 2048ull) * cap);
+}
+
 // One build: single geometry (enc.chunk_count == 0) or chunk batch. With
 // `gb` (batches only) the launches are captured into a graph on `s` instead
 // of run, and the buffers move to gb.
@@ -1751,8 +1761,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const bool bucket_ok = lbits - lb_try <= 20 && bucket_cap <= (1ull << 24);
     if (tun.finish == 2 && !bucket_ok)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "the bucket finish needs at most 2^24 buckets");
-    const bool bucketed =
-        bucket_ok && (tun.finish == 2 || (tun.finish == 0 && (uint64_t)n >= (enc.chunk_count ? 512 : 2048) * bucket_cap));
+    const bool bucketed = bucket_finish(tun, log2_size, n, enc.chunk_count);
     const uint32_t lb = bucketed ? lb_try : 0;
     const uint32_t buckets = bucketed ? (uint32_t)bucket_cap : 0;  // a batch's is an upper bound
     const uint32_t passes = (key_bits - lb + rb - 1) / rb;

@@ -48,6 +48,8 @@ void pool_free(void* p, hipStream_t s);
 // bit length of the chunk count) over the built-in defaults (tune.cpp).
 PyrhipBuildTuning resolve_tuning(const PyrhipBuildTuning* caller, uint32_t log2_size, uint32_t n,
                                  uint32_t chunk_count);
+// whether such a build takes the bucket finish under the resolved tuning (build.hip)
+bool bucket_finish(const PyrhipBuildTuning& tun, uint32_t log2_size, uint64_t n, uint32_t chunk_count);
 
 // Palette packing of a chunk batch's attributes (palette.hip). lo holds the
 // first attribute rank of each chunk and, last, the total (k + 1 entries).

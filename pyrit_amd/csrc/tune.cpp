@@ -88,6 +88,9 @@ struct Workload {
     hipStream_t s;
 };
 
+uint64_t count_of(const Workload& w) { return w.batch ? w.batch->total : w.n; }
+uint32_t chunks_of(const Workload& w) { return w.batch ? w.batch->count : 0u; }
+
 PyrResult build_once(const Workload& w, const PyrhipBuildTuning& t, PyrhipBuilt* b, Signature* sig) {
     if (!w.batch) return pyrhip_build(w.log2_size, 0, 0, 0, w.voxels, w.n, &t, w.s, b);
     std::vector<uint32_t> roots(w.batch->count), first(w.batch->count);
@@ -232,6 +235,8 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
                 if (dim == 4) c.status_words = 2 - v;
                 if (dim == 5) c.finish = 2 - v;
                 if (same_tuning(c, best)) continue;
+                // the scan tile only matters to the unique scan, not to the bucket finish
+                if (dim == 1 && bucket_finish(best, log2_size, count_of(w), chunks_of(w))) continue;
                 float ms = 0;
                 Signature sig;
                 out->candidates++;
