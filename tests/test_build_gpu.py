@@ -200,6 +200,40 @@ def test_bucket_finish_edge_cases(hip_device, case):
     assert_same_build(_gpu().build(log2, pts, finish=2), ref, log2)
 
 
+@pytest.mark.parametrize("case", ["all out of range", "one voxel", "deletes only", "last bucket", "empty chunks"])
+def test_bucket_finish_degenerate_inputs(hip_device, case):
+    """Inputs that leave buckets empty or drop every record: the bucket
+    finish must give the oracle's arrays (an empty DAG where nothing stays)."""
+    print(f"case: bucket finish, {case}", flush=True)
+    g = _gpu()
+    rng = np.random.default_rng(33)
+    if case in ("all out of range", "one voxel", "deletes only", "last bucket"):
+        log2 = 10
+        if case == "all out of range":
+            pts = random_points(3000, 10, 34)
+            pts["x"] = np.where(rng.random(len(pts)) < 0.5, -1, 1024)
+        elif case == "one voxel":
+            pts = np.array([(1023, 0, 511, 77)], O.VOXEL)
+        elif case == "deletes only":
+            pts = random_points(3000, 10, 35)
+            pts["attribute"] = 0
+        else:  # the top corner: the last bucket of the grid, then deleted again
+            pts = np.array([(1023, 1023, 1023, 5), (1022, 1023, 1023, 6), (1023, 1023, 1023, 0)], O.VOXEL)
+        ref = O.gb_build(log2, pts)
+        assert_same_build(g.build(log2, pts, finish=2), ref, log2)
+        return
+    # a batch whose chunks are empty, hold only out-of-range records, or one voxel
+    parts = [np.zeros(0, O.VOXEL), random_points(500, 6, 36), np.zeros(0, O.VOXEL),
+             np.array([(70, 0, 0, 3), (-1, 5, 5, 4)], O.VOXEL), np.array([(63, 63, 63, 9)], O.VOXEL)]
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in parts])]).astype(np.uint32)
+    vox = np.concatenate(parts)
+    ref, rroots, rfirst = O.gb_build_chunks(6, vox, offs)
+    b, roots, first = g.build_chunks(6, vox, offs, finish=2)
+    assert_same_build(b, ref, 6)
+    assert np.array_equal(roots, rroots) and np.array_equal(first, rfirst)
+    assert roots[0] == roots[2] == roots[3] == 0xFFFFFFFF
+
+
 def test_tuning_argument_checks(hip_device):
     g = _gpu()
     pts = random_points(1000, 5, 3)
