@@ -1380,10 +1380,28 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
             const uint64_t pk = k[i] >> 3;
             uint32_t w[kNodeStride];
             uint32_t mask = 0, count = 0, len = 2;
-            for (uint64_t j = idx; j < n && len < kNodeStride && (a.ch.key[j] >> 3) == pk; j++) {
-                mask |= 1u << (uint32_t)(a.ch.key[j] & 7);
-                w[len++] = a.ch.pos[a.ch.rep[j]];
-                count = sat_add(count, a.ch.count[j]);
+            // the (at most 8) children follow the head: their keys, then reps
+            // and counts, then positions, each round's loads issued together
+            uint64_t ck[8];
+            uint32_t nkids = 0;
+#pragma unroll
+            for (int q = 0; q < 8; q++) ck[q] = idx + q < n ? a.ch.key[idx + q] : ~0ull;
+#pragma unroll
+            for (int q = 0; q < 8; q++)
+                if (nkids == (uint32_t)q && (ck[q] >> 3) == pk) nkids++;
+            uint32_t rep[8], cnt[8];
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                rep[q] = (uint32_t)q < nkids ? a.ch.rep[idx + q] : 0u;
+                cnt[q] = (uint32_t)q < nkids ? a.ch.count[idx + q] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                if ((uint32_t)q < nkids) {
+                    mask |= 1u << (uint32_t)(ck[q] & 7);
+                    w[len++] = a.ch.pos[rep[q]];
+                    count = sat_add(count, cnt[q]);
+                }
             }
             w[0] = mask;
             w[1] = count;
