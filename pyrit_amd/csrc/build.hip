@@ -207,7 +207,14 @@ __device__ __forceinline__ int4 fetch_voxel(const EncodeArgs& a, uint32_t seg, u
 // out-of-range records get the invalid bit just above the local key bits, so
 // they sort to the end of their segment. encode (gbuild.zig:184-199) and
 // pack_chunks (gbuild.zig:203-230) without the chunk index in the key.
-__device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, uint32_t i, int4 v, uint64_t& key,
+// Edge of segment seg's chunk (a chunk size above the batch's counts as the
+// batch's: keys stay below the invalid bit).
+__device__ __forceinline__ int32_t chunk_edge(const EncodeArgs& a, uint32_t seg) {
+    return 1 << (a.chunk_count && a.chunk_log2 ? min((uint32_t)a.chunk_log2[seg], a.log2_size) : a.log2_size);
+}
+
+// (n: the edge of the record's chunk, chunk_edge)
+__device__ __forceinline__ void encode_one(const EncodeArgs& a, int32_t n, uint32_t i, int4 v, uint64_t& key,
                                            uint32_t& val) {
     const uint32_t lbits = local_bits(a.log2_size);
     if (i < a.n_old) {  // earlier build's sorted unique voxels (single builds)
@@ -215,8 +222,6 @@ __device__ __forceinline__ void encode_one(const EncodeArgs& a, uint32_t seg, ui
         val = a.old_vals[i];
         return;
     }
-    // (a chunk size above the batch's counts as the batch's: keys stay below the invalid bit)
-    const int32_t n = 1 << (a.chunk_count && a.chunk_log2 ? min((uint32_t)a.chunk_log2[seg], a.log2_size) : a.log2_size);
     key = 1ull << lbits;
     val = 0;
     if (v.x >= 0 && v.y >= 0 && v.z >= 0 && v.x < n && v.y < n && v.z < n) {
@@ -287,6 +292,7 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
             cur = d.x;
         }
         const uint32_t local0 = (t - d.w) * g.tile;
+        const int32_t edge = chunk_edge(a, d.x);
         for (uint32_t j0 = 0; j0 < d.z; j0 += kBlock * kEncodeUnroll) {
             // the input loads of kEncodeUnroll records are issued before any is used
             int4 raw[kEncodeUnroll];
@@ -301,7 +307,7 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
                 if (j < d.z) {
                     uint64_t key;
                     uint32_t val;
-                    encode_one(a, d.x, d.y + j, raw[u], key, val);
+                    encode_one(a, edge, d.y + j, raw[u], key, val);
                     emit_record(a, s_hist, radix, d.y + j, key, val);
                 }
             }
