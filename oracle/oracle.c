@@ -619,11 +619,13 @@ int or_gb_build(uint32_t log2_size, const uint64_t* old_keys, const uint32_t* ol
     return rc;
 }
 
-/* buildChunks (gpu_build.zig:119-122) with pack_chunks (gbuild.zig:203-230) */
+/* buildChunks (gpu_build.zig:119-122) with pack_chunks (gbuild.zig:203-230);
+ * capacity 0 reads chunk c's voxels at voxels[offsets[c] ..] (packed layout,
+ * this repository's extension) instead of voxels[c * capacity ..] */
 int or_gb_build_chunks(uint32_t log2_size, const uint8_t* chunk_log2, uint32_t count, uint32_t capacity,
                        const OrVoxel* voxels, const uint32_t* offsets, OrBuilt* out, uint32_t* roots,
                        uint32_t* first_voxel) {
-    if (log2_size < 3 || log2_size > 20 || count == 0 || capacity == 0) return -1;
+    if (log2_size < 3 || log2_size > 20 || count == 0) return -1;
     uint32_t bits = 0;
     while ((1u << bits) < count) bits++;
     if (local_bits(log2_size) + bits + 1 > 64) return -1;
@@ -644,8 +646,8 @@ int or_gb_build_chunks(uint32_t log2_size, const uint8_t* chunk_log2, uint32_t c
         uint64_t key = invalid;
         uint32_t val = 0;
         int32_t n = 1 << (chunk_log2 ? chunk_log2[c] : log2_size);
-        if (j < capacity) {
-            const OrVoxel* v = &voxels[(uint64_t)c * capacity + j];
+        if (capacity == 0 || j < capacity) {
+            const OrVoxel* v = capacity ? &voxels[(uint64_t)c * capacity + j] : &voxels[i];
             if (v->x >= 0 && v->y >= 0 && v->z >= 0 && v->x < n && v->y < n && v->z < n) {
                 key = ((uint64_t)c << lbits) | local_key((uint32_t)v->x, (uint32_t)v->y, (uint32_t)v->z);
                 val = v->attribute;
@@ -1302,49 +1304,6 @@ void or_range_release(OrRangeAlloc* r, uint64_t offset, uint64_t size) {
 }
 
 void or_range_free(OrRangeAlloc* r) { free(r->free); memset(r, 0, sizeof(*r)); }
-
-/* ------------------------------------------------------------------------ */
-/* World edit overlay (worldedit.zig:26-90), serial                          */
-/* ------------------------------------------------------------------------ */
-
-#define REMOVED 0xFFFFFFFFu
-
-void or_edit_apply(uint32_t* voxels, const uint32_t* counts, uint32_t k, uint32_t cap, const uint32_t* edits,
-                   const uint32_t* edit_offsets, uint32_t* edit_used) {
-    for (uint32_t c = 0; c < k; c++)
-        for (uint32_t local = 0; local < counts[c] && local < cap; local++) {
-            uint32_t* v = &voxels[((uint64_t)c * cap + local) * 4];
-            if (v[0] == REMOVED) continue;
-            for (uint32_t e = edit_offsets[c]; e < edit_offsets[c + 1]; e++) {
-                const uint32_t* q = &edits[(uint64_t)e * 4];
-                if (q[0] == v[0] && q[1] == v[1] && q[2] == v[2]) {
-                    edit_used[e] = 1;
-                    if (q[3] == 0) v[0] = REMOVED; else v[3] = q[3];
-                }
-            }
-        }
-}
-
-void or_edit_compact_append(const uint32_t* voxels, const uint32_t* counts, uint32_t k, uint32_t cap,
-                            const uint32_t* edits, const uint32_t* edit_offsets, const uint32_t* edit_used,
-                            uint32_t* out_voxels, uint32_t* out_counts) {
-    for (uint32_t c = 0; c < k; c++) out_counts[c] = 0;
-    for (uint32_t c = 0; c < k; c++)
-        for (uint32_t local = 0; local < counts[c] && local < cap; local++) {
-            const uint32_t* v = &voxels[((uint64_t)c * cap + local) * 4];
-            if (v[0] == REMOVED) continue;
-            uint32_t o = out_counts[c]++;
-            if (o < cap) memcpy(&out_voxels[((uint64_t)c * cap + o) * 4], v, 16);
-        }
-    for (uint32_t c = 0; c < k; c++)
-        for (uint32_t e = edit_offsets[c]; e < edit_offsets[c + 1]; e++) {
-            if (edit_used[e]) continue;
-            const uint32_t* q = &edits[(uint64_t)e * 4];
-            if (q[3] == 0) continue;
-            uint32_t o = out_counts[c]++;
-            if (o < cap) memcpy(&out_voxels[((uint64_t)c * cap + o) * 4], q, 16);
-        }
-}
 
 /* palette.zig:65-152 scan + pack, context.zig:1352-1357 word counts */
 void or_palette_pack(const uint32_t* attrs, const uint32_t* first, uint32_t k, uint32_t* count,

@@ -22,6 +22,16 @@
  *   or_walk / scene     : axis-ray KATs (cpu_test.zig:15-40), f64 DDA on a dense
  *                         grid (tests/common.zig:104-153, cpu_test.zig:42-103)
  *   or_camera_*         : closed forms (centre pixel looks at target; unit dirs)
+ *   or_trace_rays with  : three transformed, masked instances against the f64
+ *   transforms            DDA in each instance's object space (cpu_test.zig:
+ *                         105-171): nearest instance and voxel attribute, hit
+ *                         point on the voxel face within 2e-3
+ *   or_inverse          : known answer and singular case (xform.zig:60-68),
+ *                         equal to the f64 LU inverse on random transforms
+ *   or_transformed_box  : the box of the 8 transformed corners widened by the
+ *                         stated margin (brute force)
+ *   or_render_primary   : depth = t / |(vx, vy, -1)| (closed form of the
+ *                         pinhole view-space depth), hits = plain trace
  *   or_range_*          : the allocation sequence of range_alloc.zig:61-78
  *   or_palette_pack     : decoding the packed words reproduces every attribute
  *                         (brute force), the 16-value boundary and the word
@@ -78,8 +88,9 @@ int  or_gb_build(uint32_t log2_size, const uint64_t* old_keys, const uint32_t* o
                  const OrVoxel* edits, uint32_t n_new, OrBuilt* out);
 /* Chunk batch (gpu_build.zig:98-122). chunk_log2 may be NULL (all chunks
  * log2_size); otherwise chunk c has edge 2^chunk_log2[c] <= 2^log2_size (this
- * repository's mixed-size extension). roots/first_voxel get 0xFFFFFFFF for
- * empty chunks. */
+ * repository's mixed-size extension). capacity 0: packed layout, chunk c's
+ * voxels at offsets[c] (also an extension). roots/first_voxel get 0xFFFFFFFF
+ * for empty chunks. */
 int  or_gb_build_chunks(uint32_t log2_size, const uint8_t* chunk_log2, uint32_t count, uint32_t capacity,
                         const OrVoxel* voxels, const uint32_t* offsets, OrBuilt* out, uint32_t* roots,
                         uint32_t* first_voxel);
@@ -152,13 +163,6 @@ void     or_range_init(OrRangeAlloc* r, uint64_t capacity);
 int64_t  or_range_alloc(OrRangeAlloc* r, uint64_t size); /* -1 = full */
 void     or_range_release(OrRangeAlloc* r, uint64_t offset, uint64_t size);
 void     or_range_free(OrRangeAlloc* r);
-
-/* world edit overlay passes (worldedit.zig:26-90) over one batch */
-void or_edit_apply(uint32_t* voxels /*[k*cap][4]*/, const uint32_t* counts, uint32_t k, uint32_t cap,
-                   const uint32_t* edits /*[m][4]*/, const uint32_t* edit_offsets, uint32_t* edit_used);
-void or_edit_compact_append(const uint32_t* voxels, const uint32_t* counts, uint32_t k, uint32_t cap,
-                            const uint32_t* edits, const uint32_t* edit_offsets, const uint32_t* edit_used,
-                            uint32_t* out_voxels, uint32_t* out_counts);
 
 /* palette packing of a batch's attributes (palette.zig:1-152, layout of
  * context.zig:1311-1370): chunk j owns ranks [first[j], first[j+1]);

@@ -107,9 +107,6 @@ def lib() -> C.CDLL:
         L.or_range_alloc.argtypes = [P(OrRangeAlloc), u64]
         L.or_range_release.argtypes = [P(OrRangeAlloc), u64, u64]
         L.or_range_free.argtypes = [P(OrRangeAlloc)]
-        L.or_edit_apply.argtypes = [C.c_void_p, C.c_void_p, u32, u32, C.c_void_p, C.c_void_p, C.c_void_p]
-        L.or_edit_compact_append.argtypes = [C.c_void_p, C.c_void_p, u32, u32, C.c_void_p, C.c_void_p, C.c_void_p,
-                                             C.c_void_p, C.c_void_p]
         L.or_palette_pack.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p]
         _lib = L
     return _lib
@@ -212,14 +209,6 @@ def gb_build_chunks(log2_size: int, voxels: np.ndarray, offsets: np.ndarray, chu
     cl = None if chunk_log2 is None else np.ascontiguousarray(chunk_log2, dtype=np.uint8)
     roots = np.zeros(count, np.uint32)
     first = np.zeros(count, np.uint32)
-    if capacity == 0:
-        # the oracle takes the segmented layout: place chunk c at c * cap
-        counts = np.diff(offsets.astype(np.int64))
-        cap = max(1, int(counts.max()) if count else 1)
-        seg = np.zeros(count * cap, VOXEL)
-        for c in range(count):
-            seg[c * cap:c * cap + counts[c]] = voxels[offsets[c]:offsets[c + 1]]
-        voxels, capacity = seg, cap
     rc = lib().or_gb_build_chunks(log2_size, _p(cl), count, capacity, _p(voxels), _p(offsets), C.byref(b),
                                   _p(roots), _p(first))
     if rc != 0:

@@ -351,3 +351,166 @@ def test_range_alloc_sequence():
     L.or_range_release(C.byref(r), c, 30)
     assert r.n == 1 and r.used == 0
     L.or_range_free(C.byref(r))
+
+
+def make_transform(axis, a, s, t):
+    """Rotation about `axis` by a, uniform scale s, translation t, in f64 and
+    rounded to f32 (tests/common.zig:169-191 makeTransform)"""
+    ax = np.asarray(axis, np.float64)
+    ax = ax / np.sqrt((ax * ax).sum())
+    c, sn, k = np.cos(a), np.sin(a), 1 - np.cos(a)
+    x, y, z = ax
+    r = np.array([[c + x * x * k, x * y * k - z * sn, x * z * k + y * sn],
+                  [y * x * k + z * sn, c + y * y * k, y * z * k - x * sn],
+                  [z * x * k - y * sn, z * y * k + x * sn, c + z * z * k]])
+    m = np.zeros((3, 4))
+    m[:, :3] = r * s
+    m[:, 3] = t
+    return m.astype(np.float32).ravel()
+
+
+def apply_f64(m, p, w):
+    """tests/common.zig:163-167 applyF64: f32 matrix on an f64 point (w = 1) or vector (w = 0)"""
+    m = np.asarray(m, np.float64).reshape(3, 4)
+    return m[:, :3] @ np.asarray(p, np.float64) + m[:, 3] * w
+
+
+def transformed_scene(dag, n, xforms, masks):
+    OL = O.lib()
+    inst = (O.OrInstance * len(xforms))()
+    for k, m in enumerate(xforms):
+        inst[k].object_to_world[:] = [float(v) for v in m]
+        assert OL.or_inverse(inst[k].object_to_world, inst[k].world_to_object) == 1
+        OL.or_transformed_box(inst[k].object_to_world, float(n), inst[k].bounds_min, inst[k].bounds_max)
+        inst[k].geometry, inst[k].mask, inst[k].flags = 0, masks[k], 1
+    s = O.single_scene(dag)
+    s.instances, s.instance_count = inst, len(xforms)
+    s._keep_inst = inst
+    return s, inst
+
+
+def test_transformed_instances_against_f64_dda():
+    # cpu_test.zig:105-171: three transformed instances of the 64^3 test scene
+    # (one masked to 0x2), random world rays (odd rays mask 0xFF, even 0x1);
+    # reference: the f64 DDA in each instance's object space, nearest hit.
+    # Mismatches <= rays / 5000, attributes exact, the object-space hit
+    # point of a hit from outside lies on the voxel face within 2e-3.
+    n = 64
+    grid = scene_grid(n)
+    dag = O.build_dense(6, grid)
+    xf = [make_transform((0, 1, 0), 0.0, 1.0, (-80, 0, 0)),
+          make_transform((1, 2, 3), 0.7, 0.5, (20, 10, -5)),
+          make_transform((-1, 0.3, 0.2), 2.1, 1.7, (0, -40, 60))]
+    s, inst = transformed_scene(dag, n, xf, [0xFF, 0xFF, 0x2])
+    rays_n = 50000
+    rng = np.random.default_rng(11)
+    o = (rng.random((rays_n, 3)) * 2 - 1) * 200
+    d = (rng.random((rays_n, 3)) * 2 - 1) * 70 - o
+    rays = np.zeros(rays_n, O.RAY)
+    rays["origin"], rays["direction"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 0.0, np.float32(3.402823466e38)
+    masks = np.where(np.arange(rays_n) & 1, 0xFF, 0x1)
+    got = np.concatenate([O.trace_rays(s, rays[masks == m], ray_mask=int(m)) for m in (0xFF, 0x1)])
+    order = np.concatenate([np.nonzero(masks == m)[0] for m in (0xFF, 0x1)])
+    hits = np.zeros(rays_n, O.HIT)
+    hits[order] = got
+    w2o = [np.array(inst[k].world_to_object[:], np.float64) for k in range(3)]
+    mism = nhits = inside = 0
+    for i in range(rays_n):
+        of, df = o[i].astype(np.float32).astype(np.float64), d[i].astype(np.float32).astype(np.float64)
+        best, best_k = None, 0xFFFFFFFF
+        for k in range(3):
+            if inst[k].mask & int(masks[i]) == 0:
+                continue
+            r = O.ref_dda(grid, n, apply_f64(w2o[k], of, 1), apply_f64(w2o[k], df, 0), 0.0,
+                          best[0] if best else 1e30)
+            if r is not None and (best is None or r[0] < best[0]):
+                best, best_k = r, k
+        h = hits[i]
+        if (best is None) != (h["instance"] == 0xFFFFFFFF) or (
+                best is not None and (h["instance"] != best_k or h["attribute"] != attr_of(*best[1]))):
+            mism += 1
+            continue
+        if best is None:
+            continue
+        nhits += 1
+        if h["meta"] & 0x20:
+            inside += 1
+            continue
+        face = int(h["meta"] & 7)
+        ax = face >> 1
+        p = apply_f64(w2o[best_k], of, 1) + float(h["t"]) * apply_f64(w2o[best_k], df, 0)
+        assert abs(p[ax] - (best[1][ax] + (0 if face & 1 else 1))) < 2e-3
+    assert mism * 5000 <= rays_n
+    assert nhits > rays_n // 20
+    # every instance is hit, and the masked one only by 0xFF rays
+    assert set(hits["instance"][hits["instance"] != 0xFFFFFFFF].tolist()) == {0, 1, 2}
+    assert not np.any(hits["instance"][masks == 0x1] == 2)
+
+
+def test_inverse_known_answer_and_against_lu():
+    # xform.zig:60-68: m * inverse(m) applied to a point is the point; a
+    # singular matrix has no inverse
+    L = O.lib()
+    m = (C.c_float * 12)(2, 0, 0, 1, 0, 0, -3, 2, 0, 1, 0, 3)
+    inv = (C.c_float * 12)()
+    assert L.or_inverse(m, inv) == 1
+    p = np.array([0.5, -1.25, 7], np.float32)
+    q = apply_f64(np.array(m[:]), apply_f64(np.array(inv[:]), p, 1), 1)
+    assert np.allclose(q, p, atol=1e-5)
+    assert L.or_inverse((C.c_float * 12)(1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0), inv) == 0
+    # random well-conditioned transforms: equal to the f64 LU inverse rounded to f32
+    rng = np.random.default_rng(3)
+    for _ in range(200):
+        m = make_transform(rng.normal(size=3), rng.uniform(0, 6), rng.uniform(0.2, 4), rng.uniform(-100, 100, 3))
+        assert L.or_inverse((C.c_float * 12)(*m.tolist()), inv) == 1
+        a = np.eye(4)
+        a[:3, :] = m.reshape(3, 4)
+        want = np.linalg.inv(a)[:3, :]
+        assert np.allclose(np.array(inv[:]).reshape(3, 4), want, rtol=1e-6, atol=1e-6 * (1 + np.abs(want).max()))
+
+
+def test_transformed_box_encloses_corners_brute_force():
+    # xform.zig:37-58: world AABB of the cube [0, size]^3 under m (Arvo's
+    # method) plus a margin of 1e-5 * max(1, |lo|, |hi|): equal to the box of
+    # the eight transformed corners, widened by that margin
+    L = O.lib()
+    rng = np.random.default_rng(4)
+    lo, hi = (C.c_float * 3)(), (C.c_float * 3)()
+    for _ in range(300):
+        m = rng.uniform(-3, 3, 12).astype(np.float32)
+        size = float(np.float32(rng.choice([1.0, 16.0, 64.0, 4096.0])))
+        L.or_transformed_box((C.c_float * 12)(*m.tolist()), size, lo, hi)
+        corners = np.array([[x, y, z] for x in (0, size) for y in (0, size) for z in (0, size)])
+        w = np.array([apply_f64(m, c, 1) for c in corners])
+        cmin, cmax = w.min(0), w.max(0)
+        eps = 1e-5 * np.maximum(1.0, np.maximum(np.abs(cmin), np.abs(cmax)))
+        tol = 1e-6 * (np.abs(cmin) + np.abs(cmax) + size * 3 * np.abs(m).max() + 1)
+        assert np.all(np.abs(np.array(lo[:]) - (cmin - eps)) <= tol)
+        assert np.all(np.abs(np.array(hi[:]) - (cmax + eps)) <= tol)
+        assert np.all(np.array(lo[:]) < cmin) and np.all(np.array(hi[:]) > cmax)
+
+
+def test_render_depth_closed_form():
+    # render.zig:143-166: depth = -z of the hit point in view space. For a
+    # pinhole camera the primary ray of pixel (px, py) has view direction
+    # (vx, vy, -1) / |(vx, vy, -1)|, so depth = t / sqrt(vx^2 + vy^2 + 1)
+    # with vx = (2 px / w - 1) scale_x, vy = (1 - 2 py / h) scale_y
+    d = O.build_dense(6, scene_grid(64))
+    s = O.single_scene(d)
+    w, h = 96, 64
+    cam = O.camera((-40, 90, -50), (32, 20, 32), (0, 1, 0), 1.0, w, h)
+    hits = np.zeros(w * h, O.HIT)
+    depth = np.zeros(w * h, np.float32)
+    O.lib().or_render_primary(C.byref(s), C.byref(cam), hits.ctypes.data_as(C.c_void_p),
+                              depth.ctypes.data_as(C.c_void_p), 2)
+    py, px = np.divmod(np.arange(w * h), w)
+    vx = (2 * (px + 0.5) / w - 1) * cam.scale[0]
+    vy = (1 - 2 * (py + 0.5) / h) * cam.scale[1]
+    hit = hits["instance"] != 0xFFFFFFFF
+    assert hit.sum() > w * h // 8 and (~hit).sum() > 0
+    want = hits["t"].astype(np.float64) / np.sqrt(vx * vx + vy * vy + 1)
+    assert np.allclose(depth[hit], want[hit], rtol=2e-5)
+    assert np.all(depth[~hit] == np.float32(3.402823466e38))
+    # hits equal the plain trace of the camera rays
+    assert np.array_equal(hits.view(np.uint32), O.trace_rays(s, O.camera_rays(cam)).view(np.uint32))
