@@ -145,30 +145,6 @@ def build_section(args, out):
     return n, ms
 
 
-def camera_poses(w, count, width, height, seed=30):
-    from pyrit_amd import abi, lib, workloads
-
-    L = lib()
-    n = 1 << w.chunks[0].log2_size
-    s = n / 4096.0
-    rng = workloads.SplitMix64(seed)
-    cams = []
-    for _ in range(count):
-        x = int(n * 0.0625 + rng.uniform() * n * 0.875)
-        z = int(n * 0.0625 + rng.uniform() * n * 0.875)
-        h = workloads.height_host(w, 0, x, z)
-        y = h + (50 + 350 * rng.uniform()) * s
-        yaw = 2 * math.pi * rng.uniform()
-        pitch = math.radians(-40 + 35 * rng.uniform())
-        eye = (float(x), float(y), float(z))
-        tgt = (x + math.cos(pitch) * math.sin(yaw), y + math.sin(pitch), z + math.cos(pitch) * math.cos(yaw))
-        c = abi.PyrCamera()
-        L.pyr_camera_look_at(C.byref(c), (C.c_float * 3)(*eye), (C.c_float * 3)(*tgt), (C.c_float * 3)(0, 1, 0))
-        L.pyr_camera_perspective(C.byref(c), math.pi / 3, width, height, 0.0)
-        cams.append(c)
-    return cams
-
-
 def trace_section(args, out):
     from pyrit_amd import abi, gpu, lib, workloads
     from pyrit_amd.device import Buffer, Events, synchronize
@@ -188,7 +164,7 @@ def trace_section(args, out):
     dag.root, dag.log2_size, dag.default_attribute = built.root, log2, 1
     W, H = 1920, 1080
     poses = 30 if not args.quick else 6
-    cams = camera_poses(w, poses, W, H)
+    cams = workloads.camera_poses(w, poses, W, H)
     hits = Buffer(W * H * 16)
     rays = Buffer(W * H * 32)
     occ = Buffer(W * H * 4)

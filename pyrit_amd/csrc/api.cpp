@@ -39,6 +39,27 @@ struct Geometry {
     // after a path edit: nodes also in the path-edit arena, attributes in a range of its own
     bool uses_arena = false;
     uint64_t own_attr_offset = 0, own_attr_count = 0;
+    // destroyed, its ranges not yet released: the slot is not reused before
+    // the release, so the committed geometry table keeps its entry valid
+    bool held = false;
+};
+
+// Pool ranges and buffers that the committed scene may still reach: the
+// previous version of an edited geometry, a destroyed geometry, attributes
+// a path edit replaced. Traces and renders launched before the next
+// pyr_commit still read them (docs/API.md:91), so they are released by that
+// commit, once its table uploads and everything queued before them on the
+// context stream have finished (context.zig:972-1005, 1446-1545).
+struct Deferred {
+    uint64_t node_offset = 0, node_words = 0;
+    uint64_t leaf_offset = 0, leaf_count = 0;
+    uint64_t attr_offset = 0, attr_count = 0;
+    uint64_t own_attr_offset = 0, own_attr_count = 0;
+    int32_t shared = -1;      // a batch reference to drop
+    bool arena_user = false;  // a path-edit arena user to drop
+    uint64_t* keys = nullptr;
+    uint32_t* vals = nullptr;
+    int32_t slot = -1;        // a destroyed geometry's slot, free again after the release
 };
 
 // pool ranges of one installed chunk batch; every chunk's geometry refers to
@@ -133,7 +154,8 @@ struct PyrContext {
     float* w2v_dev = nullptr;
     uint32_t history = 0;
     uint32_t views = 0;
-    uint32_t arena_users = 0;  // geometries with nodes in the path-edit arena
+    uint32_t arena_users = 0;  // geometries with nodes in the path-edit arena (deferred ones included)
+    std::vector<Deferred> deferred;
 };
 
 namespace {
@@ -183,24 +205,50 @@ void release_shared(PyrContext* ctx, int32_t id) {
     r = SharedRange();
 }
 
-void release_geometry_ranges(PyrContext* ctx, Geometry& g) {
-    if (g.own_attr_count) {
-        ctx->attrs.ra.release(g.own_attr_offset, g.own_attr_count);
-        g.own_attr_offset = g.own_attr_count = 0;
+// everything a geometry holds in the pools and on the device, moved into a
+// deferred release; g keeps its table entry (data) but owns nothing
+Deferred take_geometry(Geometry& g) {
+    Deferred d;
+    d.own_attr_offset = g.own_attr_offset;
+    d.own_attr_count = g.own_attr_count;
+    d.arena_user = g.uses_arena;
+    d.shared = g.shared;
+    if (g.shared < 0) {
+        d.node_offset = g.data.node_offset;
+        d.node_words = g.node_words;
+        d.leaf_offset = g.data.leaf_offset;
+        d.leaf_count = g.leaf_count;
+        d.attr_offset = g.data.attribute_offset;
+        d.attr_count = g.attr_count;
     }
-    if (g.uses_arena) {
-        g.uses_arena = false;
-        ctx->arena_users--;
-    }
-    if (g.shared >= 0) {
-        release_shared(ctx, g.shared);
-        g.shared = -1;
-        return;
-    }
-    ctx->nodes.ra.release(g.data.node_offset, g.node_words);
-    ctx->leaves.ra.release(g.data.leaf_offset, g.leaf_count);
-    ctx->attrs.ra.release(g.data.attribute_offset, g.attr_count);
+    d.keys = g.keys;
+    d.vals = g.vals;
+    g.own_attr_offset = g.own_attr_count = 0;
+    g.uses_arena = false;
+    g.shared = -1;
     g.node_words = g.leaf_count = g.attr_count = 0;
+    g.keys = nullptr;
+    g.vals = nullptr;
+    return d;
+}
+
+void release_deferred(PyrContext* ctx, const Deferred& d) {
+    if (d.node_words) ctx->nodes.ra.release(d.node_offset, d.node_words);
+    if (d.leaf_count) ctx->leaves.ra.release(d.leaf_offset, d.leaf_count);
+    if (d.attr_count) ctx->attrs.ra.release(d.attr_offset, d.attr_count);
+    if (d.own_attr_count) ctx->attrs.ra.release(d.own_attr_offset, d.own_attr_count);
+    if (d.shared >= 0) release_shared(ctx, d.shared);
+    if (d.arena_user) ctx->arena_users--;
+    pool_free(d.keys, ctx->stream);
+    pool_free(d.vals, ctx->stream);
+    if (d.slot >= 0) ctx->geos[(size_t)d.slot].held = false;
+}
+
+// a geometry slot that is neither live nor held by a deferred release
+uint32_t free_geometry_slot(PyrContext* ctx) {
+    uint32_t slot = 0;
+    while (slot < ctx->geos.size() && (ctx->geos[slot].live || ctx->geos[slot].held)) slot++;
+    return slot;
 }
 
 // copy a finished build into the pools (context.zig:930-952, installBuilt)
@@ -375,8 +423,7 @@ uint64_t ctx_batch_bytes(PyrContext* ctx, int32_t id) {
 // palette entries and the index words (palette), or 32-bit values.
 PyrResult ctx_geometry_from_batch(PyrContext* ctx, int32_t id, uint32_t root, uint64_t attr_word, bool palette,
                                   uint32_t voxel_count, uint32_t log2_size, PyrGeometry* out) {
-    uint32_t slot = 0;
-    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    uint32_t slot = free_geometry_slot(ctx);
     if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
     if (slot == ctx->geos.size()) ctx->geos.emplace_back();
     SharedRange& sr = ctx->shared[(size_t)id];
@@ -485,7 +532,12 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
             ctx->attrs.ra.release(j.attr_dst, caps[k]);
             continue;
         }
-        if (g->own_attr_count) ctx->attrs.ra.release(g->own_attr_offset, g->own_attr_count);
+        if (g->own_attr_count) {  // the committed scene may still read them
+            Deferred d;
+            d.own_attr_offset = g->own_attr_offset;
+            d.own_attr_count = g->own_attr_count;
+            ctx->deferred.push_back(d);
+        }
         g->own_attr_offset = j.attr_dst;
         g->own_attr_count = caps[k];
         g->data.root = j.out_root;
@@ -615,6 +667,10 @@ PYR_API void pyr_destroy(PyrContext* ctx) {
         pool_free(g.keys, ctx->stream);
         pool_free(g.vals, ctx->stream);
     }
+    for (const Deferred& d : ctx->deferred) {
+        pool_free(d.keys, ctx->stream);
+        pool_free(d.vals, ctx->stream);
+    }
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ctx->nodes.base);
     (void)hipFree(ctx->leaves.base);
@@ -647,8 +703,7 @@ PYR_API PyrResult pyr_geometry_build(PyrContext* ctx, uint32_t log2_size, const 
     diag_clear();
     if (!ctx || !out || (count && !voxels)) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
     if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
-    uint32_t slot = 0;
-    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    uint32_t slot = free_geometry_slot(ctx);
     if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
     void* dev = nullptr;
     bool owned = false;
@@ -690,20 +745,17 @@ PYR_API PyrResult pyr_geometry_edit(PyrContext* ctx, PyrGeometry geometry, const
                          g->voxel_count, (uint64_t)(uintptr_t)dev, count, nullptr, ctx->stream, &b);
     if (owned) pool_free(dev, ctx->stream);
     if (r != PYR_OK) return r;
-    Geometry old = *g;
-    release_geometry_ranges(ctx, *g);
+    // the new version goes into fresh ranges; the old one stays readable by
+    // the committed scene until the next pyr_commit releases it
+    const Geometry old = *g;
+    Deferred d = take_geometry(*g);
     r = install(ctx, *g, b, true);
     if (r != PYR_OK) {
-        // pools stay as they were: put the old ranges back
-        *g = old;
-        ctx->nodes.ra.take(old.data.node_offset, old.node_words);
-        ctx->leaves.ra.take(old.data.leaf_offset, old.leaf_count);
-        ctx->attrs.ra.take(old.data.attribute_offset, old.attr_count);
+        *g = old;  // nothing of the old version was released
         pyrhip_built_free(&b, ctx->stream);
         return r;
     }
-    pool_free(old.keys, ctx->stream);
-    pool_free(old.vals, ctx->stream);
+    ctx->deferred.push_back(d);
     return PYR_OK;
 }
 
@@ -712,8 +764,7 @@ PYR_API PyrResult pyr_geometry_create(PyrContext* ctx, const PyrDag* dag, PyrGeo
     diag_clear();
     if (!ctx || !dag || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
     if (dag->nodes.empty() || dag->root >= dag->nodes.size()) return fail(PYR_ERROR_INVALID_ARGUMENT, "empty DAG");
-    uint32_t slot = 0;
-    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    uint32_t slot = free_geometry_slot(ctx);
     if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
     const uint64_t nw = dag->nodes.size(), nl = dag->leaves.size(), na = dag->has_attrs ? dag->attrs.size() : 0;
     const int64_t no = ctx->nodes.ra.alloc(nw), lo = ctx->leaves.ra.alloc(nl), ao = ctx->attrs.ra.alloc(na);
@@ -775,8 +826,7 @@ PYR_API PyrResult pyr_geometry_downsample(PyrContext* ctx, PyrGeometry geometry,
     Geometry* src = geo_of(ctx, geometry);
     if (!src) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
     if (!src->keys) return fail(PYR_ERROR_INVALID_ARGUMENT, "source is not editable (build it with PYR_BUILD_EDITABLE)");
-    uint32_t slot = 0;
-    while (slot < ctx->geos.size() && ctx->geos[slot].live) slot++;
+    uint32_t slot = free_geometry_slot(ctx);
     if (slot >= ctx->max_geometries) return fail(PYR_ERROR_CAPACITY, "max_geometries reached");
     PyrhipBuilt b{};
     PyrResult r = pyrhip_downsample((uint64_t)(uintptr_t)src->keys, (uint64_t)(uintptr_t)src->vals, src->voxel_count,
@@ -864,13 +914,13 @@ PYR_API PyrResult pyr_geometry_destroy(PyrContext* ctx, PyrGeometry geometry) {
     Geometry* g = geo_of(ctx, geometry);
     if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
     if (g->users) return fail(PYR_ERROR_IN_USE, "geometry still has %u instances", g->users);
-    (void)hipStreamSynchronize(ctx->stream);
-    release_geometry_ranges(ctx, *g);
-    pool_free(g->keys, ctx->stream);
-    pool_free(g->vals, ctx->stream);
-    g->keys = nullptr;
-    g->vals = nullptr;
+    // released by the next pyr_commit: instances destroyed since the last
+    // commit still reach it through the committed tables (context.zig:1446-1487)
+    Deferred d = take_geometry(*g);
+    d.slot = (int32_t)(g - ctx->geos.data());
     g->live = false;
+    g->held = true;
+    ctx->deferred.push_back(d);
     return PYR_OK;
 }
 
@@ -993,6 +1043,12 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
         r = hip_check(hipMemcpyAsync(ctx->scene_dev, &sc, sizeof(sc), hipMemcpyHostToDevice, s), "scene");
     // host staging above is stack memory: finish the copies before returning
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "commit");
+    // the new tables are in place and the work queued before them is done:
+    // nothing reaches the deferred ranges any more
+    if (r == PYR_OK) {
+        for (const Deferred& d : ctx->deferred) release_deferred(ctx, d);
+        ctx->deferred.clear();
+    }
     return r;
 }
 
