@@ -89,6 +89,34 @@ PYR_API PyrResult pyrhip_build(uint32_t log2_size, uint64_t old_keys, uint64_t o
                                uint64_t voxels, uint32_t n_new, const PyrhipBuildTuning* tuning, void* stream,
                                PyrhipBuilt* out);
 
+/* Caller-owned device buffers that pyrhip_build_static writes its results
+ * into: node words, leaves, and (optional, 0 = not wanted) the sorted unique
+ * voxel keys and their attributes, with their capacities in elements. */
+typedef struct PyrhipBuildOutput {
+    uint64_t nodes;
+    uint64_t node_capacity;
+    uint64_t leaves;
+    uint64_t leaf_capacity;
+    uint64_t keys;
+    uint64_t attrs;
+    uint64_t voxel_capacity;
+} PyrhipBuildOutput;
+
+/* pyrhip_build without any device allocation: no counterpart in the
+ * reference. For callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). Every
+ * scratch buffer of the build comes from a static device array of
+ * pyrhip_build_static_limit() bytes (64 MiB) that each device holds until the
+ * process ends; the results are copied into `dst` (out points at dst's
+ * buffers). A build whose scratch does not fit fails with PYR_ERROR_CAPACITY
+ * and a message naming the bytes it needs; so does output larger than dst's
+ * capacities. Calls on one device share that array: calls that overlap on
+ * one GPU (several host threads, or streams that run concurrently) race. */
+PYR_API PyrResult pyrhip_build_static(uint32_t log2_size, uint64_t old_keys, uint64_t old_vals, uint32_t n_old,
+                                      uint64_t voxels, uint32_t n_new, const PyrhipBuildTuning* tuning, void* stream,
+                                      const PyrhipBuildOutput* dst, PyrhipBuilt* out);
+PYR_API uint64_t pyrhip_build_static_limit(void);
+
 /* gpu_build.buildChunks (src/gpu_build.zig:119-122). All chunks share the
  * node, leaf and attribute arrays; roots[c] / first_voxel[c] (host arrays of
  * count) get each chunk's root word and first attribute rank, 0xFFFFFFFF for

@@ -106,6 +106,11 @@ class PyrhipBuildTimes(C.Structure):
                 ("status_bytes", u32), ("finish", u32)]
 
 
+class PyrhipBuildOutput(C.Structure):
+    _fields_ = [("nodes", u64), ("node_capacity", u64), ("leaves", u64), ("leaf_capacity", u64), ("keys", u64),
+                ("attrs", u64), ("voxel_capacity", u64)]
+
+
 class PyrhipDag(C.Structure):
     _fields_ = [("nodes", u64), ("leaves", u64), ("attrs", u64), ("root", u32), ("log2_size", u32),
                 ("default_attribute", u32), ("reserved", u32), ("palette", u64)]
@@ -223,6 +228,9 @@ FUNCTIONS = {
     "pyr_camera_look_at": (None, [P(PyrCamera), P(f32), P(f32), P(f32)]),
     "pyr_camera_perspective": (None, [P(PyrCamera), f32, u32, u32, f32]),
     "pyrhip_build": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
+    "pyrhip_build_static": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuildOutput),
+                                  P(PyrhipBuilt)]),
+    "pyrhip_build_static_limit": (u64, []),
     "pyrhip_build_chunks": (i32, [u32, P(PyrhipChunkBatch), P(PyrhipBuildTuning), vp, P(PyrhipBuilt), P(u32),
                                   P(u32)]),
     "pyrhip_built_free": (None, [P(PyrhipBuilt), vp]),

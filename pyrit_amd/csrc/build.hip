@@ -1612,19 +1612,37 @@ void fill64(hipStream_t s, uint64_t* p, uint64_t v, uint64_t n) {
     if (n) hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256 * 4, 4096)), dim3(256), 0, s, p, v, n);
 }
 
+// Device memory of one build: from the library's cache, or (base set) carved
+// out of a fixed device range in 256-byte aligned pieces, with no allocation.
 struct Alloc {
     std::vector<void*> ptrs;
     hipStream_t s;
     bool failed = false;
-    template <class T>
-    T* get(uint64_t count) {
+    uint8_t* base = nullptr;
+    uint64_t size = 0, used = 0;
+    void* raw(uint64_t bytes) {
+        bytes = std::max<uint64_t>(bytes, 256);
+        if (base) {
+            const uint64_t at = (used + 255) & ~255ull;
+            if (at + bytes > size) {
+                failed = true;
+                used = at + bytes;  // keep counting: the error reports the need
+                return nullptr;
+            }
+            used = at + bytes;
+            return base + at;
+        }
         void* p = nullptr;
-        if (pool_alloc(&p, std::max<uint64_t>(count * sizeof(T), 256), s) != hipSuccess) {
+        if (pool_alloc(&p, bytes, s) != hipSuccess) {
             failed = true;
             return nullptr;
         }
         ptrs.push_back(p);
-        return (T*)p;
+        return p;
+    }
+    template <class T>
+    T* get(uint64_t count) {
+        return (T*)raw(count * sizeof(T));
     }
     void release() {
         for (void* p : ptrs) pool_free(p, s);
@@ -1743,11 +1761,22 @@ bool bucket_finish(const PyrhipBuildTuning& tun, uint32_t log2_size, uint64_t n,
     return tun.finish == 2 || (tun.finish == 0 && n >= (chunk_count ? 512ull : 2048ull) * cap);
 }
 
+// A build without allocation (pyrhip_build_static): every buffer, outputs
+// included, is carved out of [base, base + size); the results are copied into
+// the caller's buffers at the end.
+struct FixedMemory {
+    uint8_t* base = nullptr;
+    uint64_t size = 0;
+    const PyrhipBuildOutput* dst = nullptr;
+};
+
 // One build: single geometry (enc.chunk_count == 0) or chunk batch. With
 // `gb` (batches only) the launches are captured into a graph on `s` instead
-// of run, and the buffers move to gb.
+// of run, and the buffers move to gb. With `fixed` (single builds only) no
+// memory is allocated.
 PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBuildTuning* tuning, hipStream_t s,
-                    PyrhipBuilt* out, uint32_t* roots_host, uint32_t* first_host, GraphBuild* gb = nullptr) {
+                    PyrhipBuilt* out, uint32_t* roots_host, uint32_t* first_host, GraphBuild* gb = nullptr,
+                    const FixedMemory* fixed = nullptr) {
     memset(out, 0, sizeof(*out));
     EncodeArgs enc = enc_in;
     const uint64_t n64 = (uint64_t)enc.n_old + enc.n_new;
@@ -1808,6 +1837,10 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
 
     const auto host_t0 = std::chrono::steady_clock::now();
     Alloc A{{}, s};
+    if (fixed) {
+        A.base = fixed->base;
+        A.size = fixed->size;
+    }
     uint32_t* counters = A.get<uint32_t>(C_COUNT);
     const uint32_t segs = enc.chunk_count ? enc.chunk_count : 1;
     const uint64_t hist_words = (uint64_t)passes * segs * radix;
@@ -1870,14 +1903,19 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     uint64_t* leaves_out = nullptr;
     uint32_t* nodes_out = nullptr;
     bool out_ok = true;
+    auto out_alloc = [&](void** p, uint64_t bytes) -> bool {
+        if (fixed) return (*p = A.raw(bytes)) != nullptr;
+        return pool_alloc(p, bytes, s) == hipSuccess;
+    };
     for (int q = 0; q < 2; q++) {
-        out_ok &= pool_alloc((void**)&kb[q], (uint64_t)nmax * 8, s) == hipSuccess;
+        out_ok &= out_alloc((void**)&kb[q], (uint64_t)nmax * 8);
         // packed records need one value array (for the unique attributes)
-        if (q == 0 || !packed) out_ok &= pool_alloc((void**)&vb[q], (uint64_t)nmax * 4, s) == hipSuccess;
+        if (q == 0 || !packed) out_ok &= out_alloc((void**)&vb[q], (uint64_t)nmax * 4);
     }
-    out_ok &= pool_alloc((void**)&leaves_out, nb * 8, s) == hipSuccess;
-    out_ok &= pool_alloc((void**)&nodes_out, word_bound * 4, s) == hipSuccess;
+    out_ok &= out_alloc((void**)&leaves_out, nb * 8);
+    out_ok &= out_alloc((void**)&nodes_out, word_bound * 4);
     auto free_outputs = [&]() {
+        if (fixed) return;
         for (int q = 0; q < 2; q++) {
             pool_free(kb[q], s);
             pool_free(vb[q], s);
@@ -1885,6 +1923,9 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         pool_free(leaves_out, s);
         pool_free(nodes_out, s);
     };
+    if (fixed && (A.failed || !out_ok))
+        return fail(PYR_ERROR_CAPACITY, "a build of %u voxels at log2_size %u needs %llu bytes of scratch; the static scratch holds %llu",
+                    n, log2_size, (unsigned long long)A.used, (unsigned long long)A.size);
     if (A.failed || !out_ok) {
         A.release();
         free_outputs();
@@ -1903,7 +1944,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     }
     Timer tm;
     tm.s = s;
-    if (gb) tm.on = false;
+    if (gb || fixed) tm.on = false;
     tm.mark();
     // zero state
     fill32(s, counters, 0, C_COUNT);
@@ -2072,7 +2113,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     const auto host_t3 = std::chrono::steady_clock::now();
     if (e == hipSuccess) e = hipGetLastError();
-    if (g_timing && e == hipSuccess) {
+    if (g_timing && !fixed && e == hipSuccess) {
         std::lock_guard<std::mutex> lk(g_times_mutex);
         g_times.encode_ms = tm.between(0, 1);
         g_times.sort_ms = tm.between(1, 2);
@@ -2096,6 +2137,41 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         if (e != hipSuccess) return fail(PYR_ERROR_CUDA, "build: %s", hipGetErrorString(e));
         return fail(PYR_ERROR_CUDA, "build: device error flags 0x%x", c[C_ERR]);
     }
+    if (fixed) {
+        // results into the caller's buffers (the scratch keeps them until the
+        // next call on this device)
+        const PyrhipBuildOutput& d = *fixed->dst;
+        const uint32_t fin = c[C_FINAL] & 1;
+        const uint64_t* keys = kb[fin ^ 1];
+        const uint32_t* attrs = packed ? vb[0] : vb[fin ^ 1];
+        if (c[C_WORDS] > d.node_capacity || c[C_LEAVES] > d.leaf_capacity || c[C_U] > d.voxel_capacity)
+            return fail(PYR_ERROR_CAPACITY, "build output of %u node words, %u leaves, %u voxels exceeds the buffers (%llu, %llu, %llu)",
+                        c[C_WORDS], c[C_LEAVES], c[C_U], (unsigned long long)d.node_capacity,
+                        (unsigned long long)d.leaf_capacity, (unsigned long long)d.voxel_capacity);
+        e = hipMemcpyAsync((void*)(uintptr_t)d.nodes, nodes_out, (uint64_t)c[C_WORDS] * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && c[C_LEAVES])
+            e = hipMemcpyAsync((void*)(uintptr_t)d.leaves, leaves_out, (uint64_t)c[C_LEAVES] * 8,
+                               hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && c[C_U] && d.keys)
+            e = hipMemcpyAsync((void*)(uintptr_t)d.keys, keys, (uint64_t)c[C_U] * 8, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess && c[C_U] && d.attrs)
+            e = hipMemcpyAsync((void*)(uintptr_t)d.attrs, attrs, (uint64_t)c[C_U] * 4, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(PYR_ERROR_CUDA, "build output copy: %s", hipGetErrorString(e));
+        }
+        out->log2_size = log2_size;
+        out->root = c[C_ROOT];
+        out->nodes = d.nodes;
+        out->node_words = c[C_WORDS];
+        out->leaves = d.leaves;
+        out->leaf_count = c[C_LEAVES];
+        out->keys = d.keys;
+        out->attrs = d.attrs;
+        out->voxel_count = c[C_U];
+        return PYR_OK;
+    }
     out->log2_size = log2_size;
     out->root = c[C_ROOT];
     out->nodes = (uint64_t)(uintptr_t)nodes_out;
@@ -2112,6 +2188,46 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
 }
 
 }  // namespace pyr
+
+// Scratch of pyrhip_build_static: every buffer of such a build (records,
+// histograms, look-back words, hash tables, level arrays, outputs before the
+// copy) is carved out of this array, one copy per device, held until the
+// process ends. In host code `scratch` names a host-side stand-in, not device
+// memory: the device address comes from hipGetSymbolAddress
+// (static_scratch_address) and only that address is passed to kernels.
+constexpr uint64_t kStaticScratchBytes = 64ull << 20;
+static __device__ __align__(256) uint8_t scratch[kStaticScratchBytes];
+
+namespace {
+
+std::mutex g_static_mutex;
+std::vector<uint8_t*> g_static_ptr;  // per device; null until looked up
+
+// Device address of the current device's copy of `scratch`, looked up once
+// per device (with that device current).
+PyrResult static_scratch_address(uint8_t** out) {
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return pyr::fail(PYR_ERROR_CUDA, "hipGetDevice: %s", hipGetErrorString(e));
+    }
+    std::lock_guard<std::mutex> lk(g_static_mutex);
+    if ((size_t)dev >= g_static_ptr.size()) g_static_ptr.resize((size_t)dev + 1, nullptr);
+    if (!g_static_ptr[(size_t)dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            (void)hipGetLastError();
+            return pyr::fail(PYR_ERROR_CUDA, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+        }
+        g_static_ptr[(size_t)dev] = (uint8_t*)p;
+    }
+    *out = g_static_ptr[(size_t)dev];
+    return PYR_OK;
+}
+
+}  // namespace
 
 using namespace pyr;
 
@@ -2134,6 +2250,34 @@ PYR_API PyrResult pyrhip_build(uint32_t log2_size, uint64_t old_keys, uint64_t o
     enc.log2_size = log2_size;
     return run_build(enc, log2_size, tuning, (hipStream_t)stream, out, nullptr, nullptr);
 }
+
+PYR_API PyrResult pyrhip_build_static(uint32_t log2_size, uint64_t old_keys, uint64_t old_vals, uint32_t n_old,
+                                      uint64_t voxels, uint32_t n_new, const PyrhipBuildTuning* tuning, void* stream,
+                                      const PyrhipBuildOutput* dst, PyrhipBuilt* out) {
+    diag_clear();
+    if (!out || !dst) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (!dst->nodes || !dst->leaves) return fail(PYR_ERROR_INVALID_ARGUMENT, "null output buffer");
+    if (log2_size < 3 || log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size must be in [3, 20]");
+    if ((n_old && (!old_keys || !old_vals)) || (n_new && !voxels))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "null input with a non-zero count");
+    uint8_t* base = nullptr;
+    const PyrResult r = static_scratch_address(&base);
+    if (r != PYR_OK) return r;
+    EncodeArgs enc{};
+    enc.old_keys = (const uint64_t*)(uintptr_t)old_keys;
+    enc.old_vals = (const uint32_t*)(uintptr_t)old_vals;
+    enc.n_old = n_old;
+    enc.voxels = (const int4*)(uintptr_t)voxels;
+    enc.n_new = n_new;
+    enc.log2_size = log2_size;
+    FixedMemory fm;
+    fm.base = base;
+    fm.size = kStaticScratchBytes;
+    fm.dst = dst;
+    return run_build(enc, log2_size, tuning, (hipStream_t)stream, out, nullptr, nullptr, nullptr, &fm);
+}
+
+PYR_API uint64_t pyrhip_build_static_limit(void) { return kStaticScratchBytes; }
 
 PYR_API PyrResult pyrhip_build_chunks(uint32_t log2_size, const PyrhipChunkBatch* batch,
                                       const PyrhipBuildTuning* tuning, void* stream, PyrhipBuilt* out,
