@@ -468,59 +468,51 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
     std::vector<size_t> job_req;
     std::vector<uint64_t> caps;
     std::vector<uint32_t> sorted;
-    std::vector<uint32_t> order;
     for (size_t i = 0; i < reqs.size(); i++) {
         const PathEditRequest& q = reqs[i];
         Geometry* g = geo_of(ctx, q.geometry);
         // only chunks of installed batches: their node and leaf ranges lie below the arena
         if (!g || g->shared < 0 || q.count == 0 || q.count > kPathEditMax) continue;
         if (!(g->data.flags & PYR_GEOMETRY_HAS_ATTRIBUTES) || g->data.log2_size < 3) continue;
+        if ((uint64_t)q.first + q.count > edits.size() / 4) continue;
+        const uint32_t* e = &edits[(size_t)q.first * 4];
         const uint32_t n = 1u << g->data.log2_size;
-        bool in_range = (uint64_t)q.first + q.count <= edits.size() / 4;
-        for (uint32_t e = 0; in_range && e < q.count; e++) {
-            const uint32_t* v = &edits[((size_t)q.first + e) * 4];
-            in_range = v[0] < n && v[1] < n && v[2] < n;
-        }
+        bool in_range = true;
+        for (uint32_t k = 0; in_range && k < q.count; k++) in_range = e[k * 4] < n && e[k * 4 + 1] < n && e[k * 4 + 2] < n;
         if (!in_range) continue;
-        const uint64_t levels = g->data.log2_size - 2;
-        const uint64_t node_cap = (uint64_t)q.count * (levels * 10 + 4) + 8, leaf_cap = q.count;
-        if (ctx->nodes.arena_next + node_cap > ctx->nodes.arena_end ||
-            ctx->leaves.arena_next + leaf_cap > ctx->leaves.arena_end)
+        const size_t at = sorted.size();
+        sort_path_edits(e, q.count, &sorted);
+        uint64_t words = 0, leaves = 0;
+        path_edit_capacity(&sorted[at], q.count, g->data.log2_size, &words, &leaves);
+        const uint64_t most = (uint64_t)g->voxel_count + q.count;  // attributes after the edits, at most
+        const int64_t a = ctx->nodes.arena_next + words <= ctx->nodes.arena_end &&
+                                  ctx->leaves.arena_next + leaves <= ctx->leaves.arena_end
+                              ? ctx->attrs.ra.alloc(most)
+                              : -1;
+        if (a < 0) {
+            sorted.resize(at);
             continue;
-        const uint64_t cap = (uint64_t)g->voxel_count + q.count;
-        const int64_t a = ctx->attrs.ra.alloc(cap);
-        if (a < 0) continue;
+        }
         PathEditJob j;
-        j.node_base = g->data.node_offset;
-        j.leaf_base = g->data.leaf_offset;
         j.root = g->data.root;
         j.log2_size = g->data.log2_size;
+        j.node_base = g->data.node_offset;
+        j.leaf_base = g->data.leaf_offset;
+        j.node_arena = (uint32_t)ctx->nodes.arena_next;
+        j.node_end = (uint32_t)(ctx->nodes.arena_next + words);
+        j.leaf_arena = (uint32_t)ctx->leaves.arena_next;
+        j.leaf_end = (uint32_t)(ctx->leaves.arena_next + leaves);
         j.attr_src = g->data.attribute_offset;
         j.attr_count = g->voxel_count;
+        if (g->data.flags & PYR_GEOMETRY_PALETTE) j.palette_src = g->data.palette_offset;
         j.attr_dst = (uint32_t)a;
-        j.palette = (g->data.flags & PYR_GEOMETRY_PALETTE) ? 1u : 0u;
-        j.pal_src = g->data.palette_offset;
-        j.node_arena = (uint32_t)ctx->nodes.arena_next;
-        j.node_cap = (uint32_t)node_cap;
-        j.leaf_arena = (uint32_t)ctx->leaves.arena_next;
-        j.leaf_cap = (uint32_t)leaf_cap;
-        j.edit_first = (uint32_t)(sorted.size() / 4);
+        j.edit_first = (uint32_t)(at / 4);
         j.edit_count = q.count;
-        ctx->nodes.arena_next += node_cap;
-        ctx->leaves.arena_next += leaf_cap;
-        // the kernel takes a job's edits grouped by brick, in order inside a brick
-        order.resize(q.count);
-        for (uint32_t e = 0; e < q.count; e++) order[e] = e;
-        auto brick = [&](uint32_t e) {
-            const uint32_t* v = &edits[((size_t)q.first + e) * 4];
-            return ((uint64_t)(v[0] >> 2) << 42) | ((uint64_t)(v[1] >> 2) << 21) | (v[2] >> 2);
-        };
-        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return brick(x) < brick(y); });
-        for (uint32_t e : order)
-            sorted.insert(sorted.end(), &edits[((size_t)q.first + e) * 4], &edits[((size_t)q.first + e) * 4] + 4);
+        ctx->nodes.arena_next += words;
+        ctx->leaves.arena_next += leaves;
         jobs.push_back(j);
         job_req.push_back(i);
-        caps.push_back(cap);
+        caps.push_back(most);
     }
     if (jobs.empty()) return PYR_OK;
     uint32_t* attrs = (uint32_t*)ctx->attrs.base;
@@ -528,7 +520,7 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
     for (size_t k = 0; k < jobs.size(); k++) {
         const PathEditJob& j = jobs[k];
         Geometry* g = geo_of(ctx, reqs[job_req[k]].geometry);
-        if (r != PYR_OK || j.out_flags || !g) {
+        if (r != PYR_OK || j.status || !g) {
             ctx->attrs.ra.release(j.attr_dst, caps[k]);
             continue;
         }
@@ -540,20 +532,20 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
         }
         g->own_attr_offset = j.attr_dst;
         g->own_attr_count = caps[k];
-        g->data.root = j.out_root;
+        g->data.root = j.new_root;
         g->data.attribute_offset = j.attr_dst;
         g->data.flags = PYR_GEOMETRY_HAS_ATTRIBUTES;  // 32-bit attributes ...
         g->data.palette_offset = 0;
-        g->voxel_count = j.out_count;
+        g->voxel_count = j.new_count;
         // ... unless the chunk was palette packed and its edited attributes
         // still take at most 16 values: packed again (palette.zig:1-152)
-        if (j.palette && j.out_count && !getenv("PYRIT_NO_PALETTE")) {
-            const std::vector<uint32_t> lo = {0u, j.out_count};
+        if (j.palette_src != 0xFFFFFFFFu && j.new_count && !getenv("PYRIT_NO_PALETTE")) {
+            const std::vector<uint32_t> lo = {0u, j.new_count};
             std::vector<uint32_t> counts;
             PaletteScratch ps;
             PyrResult pr = palette_scan(attrs + j.attr_dst, lo, s, &counts, &ps);
             if (pr == PYR_OK && counts[0] != kPaletteOverflow) {
-                const uint64_t words = palette_words(counts[0], j.out_count);
+                const uint64_t words = palette_words(counts[0], j.new_count);
                 const int64_t pa = ctx->attrs.ra.alloc(words);
                 if (pa >= 0 && palette_pack(attrs + j.attr_dst, lo, {(uint64_t)pa}, attrs, s, &ps) == PYR_OK) {
                     // (the 32-bit range is reused only by later work on this stream)
@@ -576,7 +568,7 @@ PyrResult ctx_path_edit(PyrContext* ctx, const std::vector<PathEditRequest>& req
             ctx->arena_users++;
         }
         (*applied)[job_req[k]] = 1;
-        (*voxels)[job_req[k]] = j.out_count;
+        (*voxels)[job_req[k]] = j.new_count;
     }
     return r;
 }

@@ -73,20 +73,30 @@ PyrResult palette_pack(const uint32_t* attrs, const std::vector<uint32_t>& lo, c
 void palette_release(PaletteScratch* scratch, hipStream_t s);
 
 // Path edits (pathedit.hip): voxel edits applied to a finished DAG by
-// rewriting the root-to-brick paths into an arena. Offsets are in words of
-// the pools passed to run_path_edits; node and leaf references inside the
-// DAG are relative to node_base / leaf_base. A job's edits (x, y, z,
-// attribute; chunk-local) are edits[edit_first ..][4], grouped by brick with
-// their order kept inside a brick; the array is overwritten.
+// writing new copies of the touched bricks and nodes into arena space. A
+// job's edits are edits[edit_first ..][4] (x, y, z, attribute; chunk-local),
+// sorted by brick Morton key with their order kept inside a brick
+// (sort_path_edits). Node and leaf references inside the DAG are relative
+// to node_base / leaf_base; arenas, attribute offsets and palette_src are
+// offsets in the pools passed to run_path_edits. status: 0 applied
+// (new_root, new_count valid), kPathEmpty, kPathOverflow (nothing usable).
 constexpr uint32_t kPathEmpty = 2;     // the chunk became empty
-constexpr uint32_t kPathOverflow = 4;  // the job's arena share was too small
+constexpr uint32_t kPathOverflow = 4;  // the arena space given to the job was too small
 struct PathEditJob {
-    uint32_t node_base = 0, leaf_base = 0, root = 0, log2_size = 0;
-    uint32_t attr_src = 0, attr_count = 0, attr_dst = 0, palette = 0;  // palette: source is 4-bit indices
-    uint32_t pal_src = 0, node_arena = 0, node_cap = 0, leaf_arena = 0;
-    uint32_t leaf_cap = 0, edit_first = 0, edit_count = 0, reserved = 0;
-    uint32_t out_root = 0, out_count = 0, out_flags = 0, reserved2 = 0;
+    uint32_t root = 0, log2_size = 0;
+    uint32_t node_base = 0, leaf_base = 0;
+    uint32_t node_arena = 0, node_end = 0;  // arena words [node_arena, node_end)
+    uint32_t leaf_arena = 0, leaf_end = 0;
+    uint32_t attr_src = 0, attr_count = 0;
+    uint32_t palette_src = 0xFFFFFFFFu;  // else: source attributes are 4-bit indices into 16 entries here
+    uint32_t attr_dst = 0;               // new_count words
+    uint32_t edit_first = 0, edit_count = 0;
+    uint32_t new_root = 0, new_count = 0, status = 0, bricks = 0;
 };
+void sort_path_edits(const uint32_t* edits, uint32_t count, std::vector<uint32_t>* out);
+// arena node words and leaves a job of sorted edits can need at most
+void path_edit_capacity(const uint32_t* sorted, uint32_t count, uint32_t log2_size, uint64_t* node_words,
+                        uint64_t* leaves);
 PyrResult run_path_edits(std::vector<PathEditJob>& jobs, const std::vector<uint32_t>& edits, uint32_t* nodes_pool,
                          uint64_t* leaves_pool, const uint32_t* src_pool, uint32_t* dst_pool, hipStream_t s);
 

@@ -1,5 +1,6 @@
-"""Path edits (src/device/pathedit.zig): voxels set, inserted and removed in a
-finished DAG by rewriting the root-to-brick paths into spare capacity. The
+"""Path edits (the capability of src/device/pathedit.zig): voxels set,
+inserted and removed in a finished DAG by writing new copies of the touched
+bricks and nodes into spare capacity. The
 edited DAG must describe exactly the edited voxel set: canonical form equal
 to the CPU builder's DAG of that set, attributes equal in DFS rank order."""
 import ctypes as C
@@ -38,7 +39,7 @@ def _edit(log2, pts, edits, node_extra=None, leaf_extra=None):
     nodes, leaves, attrs = b.nodes(), b.leaves(), b.attrs()
     ne = len(edits)
     levels = log2 - 2
-    ncap = len(nodes) + (node_extra if node_extra is not None else ne * (levels * 10 + 4) + 8)
+    ncap = len(nodes) + (node_extra if node_extra is not None else (ne + 1) * levels * 10)
     lcap = len(leaves) + (leaf_extra if leaf_extra is not None else ne)
     nb = to_device(np.concatenate([nodes, np.zeros(ncap - len(nodes), np.uint32)]))
     lb = to_device(np.concatenate([leaves, np.zeros(max(lcap - len(leaves), 1), np.uint64)]))
@@ -93,6 +94,24 @@ def test_path_edits_random(hip_device, log2, seed):
                           np.stack([pts["x"], pts["y"], pts["z"]], 1)[rng.choice(len(pts), k)]]).astype(np.uint32)
     att = np.where(rng.random(2 * k) < 0.35, 0, rng.integers(1, 50, 2 * k)).astype(np.uint32)
     _check(log2, pts, np.concatenate([pos, att[:, None]], axis=1))
+
+
+@pytest.mark.parametrize("log2,points,edits,seed", [(8, 40000, 4000, 4), (12, 5000, 1500, 5), (5, 2000, 3000, 6)])
+def test_path_edits_many_bricks(hip_device, log2, points, edits, seed):
+    """Thousands of edits over many bricks (many rounds of 64 bricks per wave),
+    deep trees whose edits create whole new paths (4096^3), and a dense 32^3
+    grid where most voxels are touched several times."""
+    print(f"case: path edits log2={log2} edits={edits} seed={seed}", flush=True)
+    rng = np.random.default_rng(seed)
+    n = 1 << log2
+    pts = _apply(np.zeros(0, O.VOXEL), np.concatenate(
+        [rng.integers(0, n, (points, 3)), rng.integers(1, 1 << 20, (points, 1))], axis=1).astype(np.uint32))
+    half = edits // 2
+    pos = np.concatenate([rng.integers(0, n, (half, 3)),
+                          np.stack([pts["x"], pts["y"], pts["z"]], 1)[rng.choice(len(pts), edits - half)]])
+    att = np.where(rng.random(edits) < 0.4, 0, rng.integers(1, 1 << 20, edits))
+    e = np.concatenate([pos, att[:, None]], axis=1).astype(np.uint32)
+    _check(log2, pts, e[rng.permutation(edits)])
 
 
 def test_path_edit_empty_and_overflow(hip_device):
