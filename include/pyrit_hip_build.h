@@ -79,6 +79,10 @@ typedef struct PyrhipBuildTuning {
                                bricks from a table per bucket of 2^14 keys of a chunk; 1: full sort and unique
                                scan; 0 (default): 2 when a single build has 2048 records per bucket on average
                                (a batch 512 per bucket of chunks of its largest size) */
+    uint32_t counted_pass;  /* 1 (default): with 8-bit digits the first radix pass takes every tile's output
+                               offsets from per-tile digit counts the encode step writes, later passes use a
+                               decoupled look-back; 3: later passes are counted too, by one read of their input;
+                               2: decoupled look-back in every pass */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique

@@ -188,6 +188,11 @@ struct EncodeArgs {
     uint32_t passes, radix_bits;
     uint32_t shift0;          // first key bit of the lowest digit (bucket finish: bits below are not sorted)
     uint32_t packed;          // keys[i] = key << 32 | attribute, vals unused
+    // pass-0 digit counts per tile (8-bit digits; null: not counted): for
+    // tile t of block b's run, tile_run[t][d] = records of digit d in the
+    // run's tiles before t; run_sum[b][d] = the whole run's
+    uint32_t* tile_run;
+    uint32_t* run_sum;
 };
 
 // Input record i of the record array (the `local`-th record of segment
@@ -272,6 +277,11 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
     const uint32_t t0 = blockIdx.x * tiles_per_block;
     const uint32_t t1 = min(tiles, t0 + tiles_per_block);
     uint32_t cur = 0xFFFFFFFFu;
+    // counted pass 0 (radix 256: thread = digit): the run's count of the
+    // thread's digit so far, and the LDS bin as last seen (bins are cleared
+    // when the run enters the next segment)
+    const bool counted = a.tile_run != nullptr;
+    uint32_t run_acc = 0, seen = 0;
     auto flush = [&]() {
         __syncthreads();
         for (uint32_t b = threadIdx.x; b < bins; b += kBlock) {
@@ -282,6 +292,7 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
                 s_hist[b] = 0;
             }
         }
+        seen = 0;
         __syncthreads();
     };
     __syncthreads();
@@ -291,6 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
             if (cur != 0xFFFFFFFFu) flush();
             cur = d.x;
         }
+        if (counted) a.tile_run[(uint64_t)t * kBlock + threadIdx.x] = run_acc;
         const uint32_t local0 = (t - d.w) * g.tile;
         const int32_t edge = chunk_edge(a, d.x);
         for (uint32_t j0 = 0; j0 < d.z; j0 += kBlock * kEncodeUnroll) {
@@ -312,8 +324,36 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
                 }
             }
         }
+        if (counted) {
+            __syncthreads();
+            const uint32_t v = s_hist[threadIdx.x];
+            run_acc += v - seen;
+            seen = v;
+            __syncthreads();  // every bin read before the next tile counts into it
+        }
     }
     if (cur != 0xFFFFFFFFu) flush();
+    if (counted) a.run_sum[(uint64_t)blockIdx.x * kBlock + threadIdx.x] = run_acc;
+}
+
+// Exclusive prefix of the encode runs' pass-0 digit counts over the runs,
+// per digit: run_off[b][d] = records of digit d in runs before b. One block;
+// thread (q, d) sums a quarter of the runs for digit d, then rescans it.
+__global__ void __launch_bounds__(1024) k_run_scan(const uint32_t* run_sum, uint32_t* run_off, uint32_t runs) {
+    __shared__ uint32_t s_q[4][kBlock];
+    const uint32_t d = threadIdx.x % kBlock, q = threadIdx.x / kBlock;
+    const uint32_t lo = runs * q / 4, hi = runs * (q + 1) / 4;
+    uint32_t sum = 0;
+    for (uint32_t b = lo; b < hi; b++) sum += run_sum[(uint64_t)b * kBlock + d];
+    s_q[q][d] = sum;
+    __syncthreads();
+    uint32_t acc = 0;
+    for (uint32_t k = 0; k < q; k++) acc += s_q[k][d];
+    for (uint32_t b = lo; b < hi; b++) {
+        const uint32_t v = run_sum[(uint64_t)b * kBlock + d];
+        run_off[(uint64_t)b * kBlock + d] = acc;
+        acc += v;
+    }
 }
 
 // Per (pass, segment): exclusive scan of the digit counts plus the segment's
@@ -379,6 +419,11 @@ struct SortArgs {
     uint32_t* tile_ctr;
     uint32_t* err;
     uint32_t peel;  // digits matched by ballots before the LDS match, per row
+    // counted pass (pass 0 over records in input order, 8-bit digits): the
+    // tile's digit offsets come from the encode counts, no look-back
+    const uint32_t* tile_run;
+    const uint32_t* run_off;
+    uint32_t run_tiles;  // tiles per encode run
 };
 
 // The passes of the bucket finish sort the high key bits only, which
@@ -558,14 +603,17 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
     };
     // publish the tile's counts first, then stage the tile in LDS in digit
     // order (local offsets only) while the predecessors finish, then look back
+    // (a counted pass knows its offsets: nothing to publish or poll)
+    const bool counted = a.tile_run != nullptr;
 #pragma unroll
     for (uint32_t j = 0; j < DPT; j++) {
         const uint32_t d = tid * DPT + j;
         SW* st;
         if constexpr (NARROW) st = a.status32 + d;
         else st = a.status + d;
-        __hip_atomic_store(&st[(uint64_t)tile * R], make(tile == td.w ? 2 : 1, tot[j]), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        if (!counted)
+            __hip_atomic_store(&st[(uint64_t)tile * R], make(tile == td.w ? 2 : 1, tot[j]), __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
     }
     __syncthreads();  // s_local is complete
 #pragma unroll
@@ -584,7 +632,15 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         if constexpr (NARROW) st = a.status32 + d;
         else st = a.status + d;
         uint64_t excl = 0;
-        if (tile != td.w) {  // not the first tile of its segment
+        if (counted) {
+            // records of digit d in the segment's tiles before this one: the
+            // difference of two prefixes over all tiles (run prefix + the
+            // run-local prefix), this tile's and the segment's first tile's
+            const uint32_t f = td.w;
+            const uint32_t pt = a.run_off[(uint64_t)(tile / a.run_tiles) * R + d] + a.tile_run[(uint64_t)tile * R + d];
+            const uint32_t pf = a.run_off[(uint64_t)(f / a.run_tiles) * R + d] + a.tile_run[(uint64_t)f * R + d];
+            excl = pt - pf;
+        } else if (tile != td.w) {  // not the first tile of its segment
             // predecessors are polled newest to oldest and summed until an
             // inclusive prefix: kPollFirst in the first round (the previous
             // tile's prefix is often there already), kPoll (independent loads
@@ -637,6 +693,62 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         __builtin_nontemporal_store(key, &keys_out[dst]);
         if constexpr (!PACKED) vals_out[dst] = s_vals[j];
     }
+}
+
+// Per-tile digit counts of radix pass `pass` (8-bit digits) over the records
+// that pass reads, in the run layout of the encode step's counts (block b
+// counts tiles [b * run_tiles, (b + 1) * run_tiles)): tile_run[t][d] = records
+// of digit d in the run's tiles before t, run_sum[b][d] = the run's total. A
+// pass that moves nothing is left alone.
+__global__ void __launch_bounds__(kBlock) k_tile_count(RecordBufs b, const uint32_t* plan, uint32_t pass,
+                                                        uint32_t shift, Segs g, uint32_t* tile_run, uint32_t* run_sum,
+                                                        uint32_t run_tiles) {
+    constexpr int W = kBlock / kWave;
+    constexpr int kItems = 8;  // loads per thread in flight together
+    __shared__ uint32_t s_cnt[W][kBlock];
+    if (!((plan[C_PASS] >> pass) & 1)) return;
+    const uint64_t* keys = b.keys[(plan[C_PSRC] >> pass) & 1];
+    const uint32_t tid = threadIdx.x, wave = tid / kWave;
+    const uint32_t tiles = tile_count(g);
+    const uint32_t t0 = blockIdx.x * run_tiles, t1 = min(tiles, t0 + run_tiles);
+    uint32_t run_acc = 0;
+    for (uint32_t t = t0; t < t1; t++) {
+        const uint4 td = tile_desc(g, t);
+        tile_run[(uint64_t)t * kBlock + tid] = run_acc;
+#pragma unroll
+        for (int w = 0; w < W; w++) s_cnt[w][tid] = 0;
+        __syncthreads();
+        for (uint32_t j0 = 0; j0 < td.z; j0 += kBlock * kItems) {
+            uint64_t k[kItems];
+#pragma unroll
+            for (int u = 0; u < kItems; u++) {
+                const uint32_t j = j0 + u * kBlock + tid;
+                k[u] = j < td.z ? __builtin_nontemporal_load(&keys[td.y + j]) : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < kItems; u++) {
+                const uint32_t j = j0 + u * kBlock + tid;
+                if (j >= td.z) continue;
+                const uint32_t d = (uint32_t)(k[u] >> shift) & (kBlock - 1);
+                // a digit shared by every active lane costs one LDS atomic
+                const uint64_t active = __ballot(1);
+                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
+                if (__ballot(d == d0) == active) {
+                    if (lane_id() == (uint32_t)__ffsll((unsigned long long)active) - 1)
+                        atomicAdd(&s_cnt[wave][d0], (uint32_t)__popcll(active));
+                } else {
+                    atomicAdd(&s_cnt[wave][d], 1u);
+                }
+            }
+        }
+        __syncthreads();
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < W; w++) c += s_cnt[w][tid];
+        run_acc += c;
+        __syncthreads();  // counts read before the next tile clears them
+    }
+    run_sum[(uint64_t)blockIdx.x * kBlock + tid] = run_acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1799,6 +1911,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (load_pct < 25 || load_pct > 95) return fail(PYR_ERROR_INVALID_ARGUMENT, "hash_load_pct must be in [25, 95]");
     if (tun.status_words < 1 || tun.status_words > 2)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "status_words must be 1 or 2");
+    if (tun.counted_pass > 3) return fail(PYR_ERROR_INVALID_ARGUMENT, "counted_pass must be 1, 2 or 3");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -1879,6 +1992,12 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     uint32_t* bucket_seg = bucketed && enc.chunk_count ? A.get<uint32_t>(buckets) : nullptr;
     uint32_t* first_bucket = bucketed && enc.chunk_count ? A.get<uint32_t>(enc.chunk_count + 1) : nullptr;
     uint32_t* status32 = narrow ? A.get<uint32_t>(sort_tiles * radix) : nullptr;
+    // pass 0 counted by encode (8-bit digits): per-tile run prefixes and per-run totals
+    const uint32_t eblocks = (uint32_t)std::min<uint64_t>(sort_tiles, 2048);
+    const uint32_t per_block = (uint32_t)((sort_tiles + eblocks - 1) / eblocks);
+    const bool counted = radix == (uint32_t)kBlock && tun.counted_pass != 2;
+    uint32_t* tile_run = counted ? A.get<uint32_t>(sort_tiles * radix) : nullptr;
+    uint32_t* run_sum = counted ? A.get<uint32_t>((uint64_t)eblocks * radix * 2) : nullptr;
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
     uint64_t* table = A.get<uint64_t>(cap);
@@ -1972,10 +2091,13 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                                enc.offsets, tile_first, enc.chunk_count, sort_tile, counters + C_TILES,
                                (uint4*)seg.desc);
         }
-        const uint32_t eblocks = (uint32_t)std::min<uint64_t>(sort_tiles, 2048);
-        const uint32_t per_block = (uint32_t)((sort_tiles + eblocks - 1) / eblocks);
+        enc.tile_run = tile_run;
+        enc.run_sum = run_sum;
         hipLaunchKernelGGL(k_encode, dim3(eblocks), dim3(kBlock), passes * radix * sizeof(uint32_t), s, enc, seg,
                            per_block);
+        if (counted)
+            hipLaunchKernelGGL(k_run_scan, dim3(1), dim3(1024), 0, s, run_sum, run_sum + (uint64_t)eblocks * radix,
+                               eblocks);
         hipLaunchKernelGGL(k_digit_scan, dim3(passes * segs), dim3(kBlock), 0, s, hist, digit_base, seg, passes,
                            radix, counters);
         hipLaunchKernelGGL(k_pass_plan, dim3(1), dim3(kWave), 0, s, passes, counters);
@@ -1986,9 +2108,20 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // (the device skips passes that move nothing and picks each pass's
     // source buffer, so the buffers are named by index here)
     for (uint32_t p = 0; p < passes && n; p++) {
-        if (narrow) fill32(s, status32, 0, sort_tiles * radix);
+        // pass 0 reads the records in input order (counted by encode); later
+        // passes are counted by a pass over their input when asked to
+        const bool use_counts = counted && (p == 0 || tun.counted_pass == 3);
+        if (use_counts && p > 0) {
+            hipLaunchKernelGGL(k_tile_count, dim3(eblocks), dim3(kBlock), 0, s, bufs, counters, p,
+                               (packed ? 32 : 0) + lb + p * rb, seg, tile_run, run_sum, per_block);
+            hipLaunchKernelGGL(k_run_scan, dim3(1), dim3(1024), 0, s, run_sum, run_sum + (uint64_t)eblocks * radix,
+                               eblocks);
+        }
+        if (narrow && !use_counts) fill32(s, status32, 0, sort_tiles * radix);
         SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + lb + p * rb, digit_base + (uint64_t)p * segs * radix,
-                    status, status32, ++epoch, tile_ctr(), counters + C_ERR, bucketed ? kSortPeel : 0u};
+                    status, status32, ++epoch, tile_ctr(), counters + C_ERR, bucketed ? kSortPeel : 0u,
+                    use_counts ? tile_run : nullptr, use_counts ? run_sum + (uint64_t)eblocks * radix : nullptr,
+                    per_block};
         launch_sort(s, rb, sort_items, packed, (uint32_t)sort_tiles, sa);
     }
     tm.mark();

@@ -23,7 +23,7 @@
 namespace pyr {
 namespace {
 
-constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2, 0};
+constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2, 0, 1};
 
 struct Table {
     std::mutex m;
@@ -56,12 +56,13 @@ void overlay(PyrhipBuildTuning& t, const PyrhipBuildTuning& o) {
     if (o.hash_load_pct) t.hash_load_pct = o.hash_load_pct;
     if (o.status_words) t.status_words = o.status_words;
     if (o.finish) t.finish = o.finish;
+    if (o.counted_pass) t.counted_pass = o.counted_pass;
 }
 
 bool same_tuning(const PyrhipBuildTuning& a, const PyrhipBuildTuning& b) {
     return a.radix_bits == b.radix_bits && a.sort_items == b.sort_items && a.pair_layout == b.pair_layout &&
            a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct && a.status_words == b.status_words &&
-           a.finish == b.finish;
+           a.finish == b.finish && a.counted_pass == b.counted_pass;
 }
 
 // what a build produced, enough to tell two settings' outputs apart
@@ -225,8 +226,8 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // (11-bit digits are left out: 2048 look-back words per tile made a
         // pass ten times slower than 8-bit digits on every workload measured)
         const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
-        for (int dim = 0; dim < 6; dim++) {
-            for (int v = 0; v < (dim >= 4 ? 2 : 4); v++) {
+        for (int dim = 0; dim < 7; dim++) {
+            for (int v = 0; v < (dim == 6 ? 3 : dim >= 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];
                 if (dim == 1) c.scan_items = scans[v];
@@ -234,6 +235,7 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
                 if (dim == 3) c.hash_load_pct = loads[v];
                 if (dim == 4) c.status_words = 2 - v;
                 if (dim == 5) c.finish = 2 - v;
+                if (dim == 6) c.counted_pass = 1 + v;
                 if (same_tuning(c, best)) continue;
                 // the scan tile only matters to the unique scan, not to the bucket finish
                 if (dim == 1 && bucket_finish(best, log2_size, count_of(w), chunks_of(w))) continue;

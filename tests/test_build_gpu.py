@@ -120,6 +120,47 @@ def test_scan_tile_sizes_identical(hip_device, scan_items, sort_items):
     assert_same_build(b, ref, 8)
 
 
+@pytest.mark.parametrize("counted_pass", [1, 2, 3])
+def test_counted_and_lookback_first_pass_identical(hip_device, counted_pass):
+    """The first radix pass takes its tile offsets from the encode step's
+    per-tile counts (1), every pass from counts (3), or every pass from the
+    decoupled look-back (2): single builds
+    with out-of-range records, edits, both finishes, batches (tiles of many
+    chunks in one encode run) and mixed chunk sizes, against the oracle."""
+    from pyrit_amd import workloads
+
+    g = _gpu()
+    print(f"case: counted pass {counted_pass}, single and edit", flush=True)
+    pts = random_points(400000, 9, 91, dup_frac=0.3, delete_frac=0.1, out_frac=0.01)
+    ref = O.gb_build(9, pts)
+    for finish in (1, 2):
+        b = g.build(9, pts, counted_pass=counted_pass, finish=finish)
+        assert_same_build(b, ref, 9)
+    ed = random_points(30000, 9, 92, dup_frac=0.2, delete_frac=0.3)
+    b2 = g.build(9, ed, old=b, counted_pass=counted_pass)
+    assert_same_build(b2, O.gb_build(9, ed, ref["keys"], ref["attrs"]), 9)
+    print(f"case: counted pass {counted_pass}, packed batch of 700 chunks", flush=True)
+    rng = np.random.default_rng(93)
+    chunks = [random_points(int(rng.integers(0, 9000)), 6, int(rng.integers(1 << 30)), dup_frac=0.1,
+                            delete_frac=0.05, out_frac=0.01) for _ in range(700)]
+    offs = np.concatenate([[0], np.cumsum([len(c) for c in chunks])]).astype(np.uint32)
+    vox = np.concatenate(chunks)
+    rb_, rr, rf = O.gb_build_chunks(6, vox, offs)
+    for finish in (1, 2):
+        bb, roots, first = g.build_chunks(6, vox, offs, counted_pass=counted_pass, finish=finish)
+        assert_same_build(bb, rb_, 6)
+        assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+    print(f"case: counted pass {counted_pass}, mixed sizes", flush=True)
+    w = workloads.uneven_chunks(seed=94, count=48, edges=(8, 16, 32, 64, 128))
+    vox, offs64 = w.generate_host()
+    offs = offs64.astype(np.uint32)
+    sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
+    rb_, rr, rf = O.gb_build_chunks(7, vox, offs, chunk_log2=sizes)
+    bb, roots, first = g.build_chunks(7, vox, offs, chunk_log2=sizes, counted_pass=counted_pass)
+    assert_same_build(bb, rb_, 7)
+    assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+
+
 @pytest.mark.parametrize("load", [25, 50, 95])
 def test_hash_load_identical(hip_device, load):
     # the hash capacity changes probe lengths only: owners are the smallest
@@ -238,7 +279,7 @@ def test_tuning_argument_checks(hip_device):
     g = _gpu()
     pts = random_points(1000, 5, 3)
     for kw in ({"hash_load_pct": 10}, {"hash_load_pct": 99}, {"pair_layout": 3}, {"radix_bits": 12},
-               {"sort_items": 10}, {"scan_items": 24}, {"status_words": 3}, {"finish": 3}):
+               {"sort_items": 10}, {"scan_items": 24}, {"status_words": 3}, {"finish": 3}, {"counted_pass": 4}):
         with pytest.raises(RuntimeError, match="INVALID_ARGUMENT"):
             g.build(5, pts, **kw)
     # the bucket finish: single builds of at most 2^20 buckets (log2 <= 11)
@@ -250,7 +291,7 @@ def test_autotune_installs_an_identical_setting(hip_device):
     print("case: autotune single build and batch", flush=True)
     g = _gpu()
     default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80,
-               "status_words": 2, "finish": 0}
+               "status_words": 2, "finish": 0, "counted_pass": 1}
     g.tuning_clear()
     try:
         pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
