@@ -80,9 +80,8 @@ typedef struct PyrhipBuildTuning {
                                scan; 0 (default): 2 when a single build has 2048 records per bucket on average
                                (a batch 512 per bucket of chunks of its largest size) */
     uint32_t counted_pass;  /* 1 (default): with 8-bit digits the first radix pass takes every tile's output
-                               offsets from per-tile digit counts the encode step writes, later passes use a
-                               decoupled look-back; 3: later passes are counted too, by one read of their input;
-                               2: decoupled look-back in every pass */
+                               offsets from per-tile digit counts the encode step writes (later passes use the
+                               decoupled look-back); 2: decoupled look-back in every pass */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique
@@ -230,8 +229,9 @@ typedef struct PyrhipBuildTimes {
     uint32_t finish;       /* 2: partial sort + bucket tables, 1: full sort + unique scan */
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
-/* Plain device-to-device copy kernel of `bytes` (multiple of 8): the
- * calibration probe for HBM byte counters (reads = writes = bytes). No
+/* Plain device-to-device copy kernel of `bytes` (16-byte aligned buffers,
+ * a multiple of 16 bytes): the calibration probe for HBM byte counters
+ * (reads = writes = bytes) and the copy ceiling of the device. No
  * counterpart in the reference. */
 PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream);
 PYR_API void pyrhip_last_build_times(PyrhipBuildTimes* out);

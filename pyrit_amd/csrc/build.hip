@@ -336,23 +336,22 @@ __global__ void __launch_bounds__(kBlock) k_encode(EncodeArgs a, Segs g, uint32_
     if (counted) a.run_sum[(uint64_t)blockIdx.x * kBlock + threadIdx.x] = run_acc;
 }
 
-// Exclusive prefix of the encode runs' pass-0 digit counts over the runs,
-// per digit: run_off[b][d] = records of digit d in runs before b. One block;
-// thread (q, d) sums a quarter of the runs for digit d, then rescans it.
-__global__ void __launch_bounds__(1024) k_run_scan(const uint32_t* run_sum, uint32_t* run_off, uint32_t runs) {
-    __shared__ uint32_t s_q[4][kBlock];
-    const uint32_t d = threadIdx.x % kBlock, q = threadIdx.x / kBlock;
-    const uint32_t lo = runs * q / 4, hi = runs * (q + 1) / 4;
-    uint32_t sum = 0;
-    for (uint32_t b = lo; b < hi; b++) sum += run_sum[(uint64_t)b * kBlock + d];
-    s_q[q][d] = sum;
-    __syncthreads();
-    uint32_t acc = 0;
-    for (uint32_t k = 0; k < q; k++) acc += s_q[k][d];
+// Exclusive prefix of the runs' digit counts over the runs, per digit:
+// run_off[b][d] = records of digit d in runs before b. One block per digit;
+// thread t sums a contiguous group of runs, a block scan of the groups, then
+// every thread writes its group's prefixes.
+__global__ void __launch_bounds__(kBlock) k_run_scan(const uint32_t* run_sum, uint32_t* run_off, uint32_t runs) {
+    __shared__ uint32_t s_wave[kBlock / kWave];
+    const uint32_t d = blockIdx.x, per = (runs + kBlock - 1) / kBlock;
+    const uint32_t lo = min(runs, threadIdx.x * per), hi = min(runs, lo + per);
+    uint32_t v[1] = {0}, e[1];
+    for (uint32_t b = lo; b < hi; b++) v[0] += run_sum[(uint64_t)b * kBlock + d];
+    tile_scan<1>(v, e, s_wave);
+    uint32_t acc = e[0];
     for (uint32_t b = lo; b < hi; b++) {
-        const uint32_t v = run_sum[(uint64_t)b * kBlock + d];
+        const uint32_t c = run_sum[(uint64_t)b * kBlock + d];
         run_off[(uint64_t)b * kBlock + d] = acc;
-        acc += v;
+        acc += c;
     }
 }
 
@@ -693,62 +692,6 @@ __global__ void __launch_bounds__(kBlock) k_sort_pass(SortArgs a) {
         __builtin_nontemporal_store(key, &keys_out[dst]);
         if constexpr (!PACKED) vals_out[dst] = s_vals[j];
     }
-}
-
-// Per-tile digit counts of radix pass `pass` (8-bit digits) over the records
-// that pass reads, in the run layout of the encode step's counts (block b
-// counts tiles [b * run_tiles, (b + 1) * run_tiles)): tile_run[t][d] = records
-// of digit d in the run's tiles before t, run_sum[b][d] = the run's total. A
-// pass that moves nothing is left alone.
-__global__ void __launch_bounds__(kBlock) k_tile_count(RecordBufs b, const uint32_t* plan, uint32_t pass,
-                                                        uint32_t shift, Segs g, uint32_t* tile_run, uint32_t* run_sum,
-                                                        uint32_t run_tiles) {
-    constexpr int W = kBlock / kWave;
-    constexpr int kItems = 8;  // loads per thread in flight together
-    __shared__ uint32_t s_cnt[W][kBlock];
-    if (!((plan[C_PASS] >> pass) & 1)) return;
-    const uint64_t* keys = b.keys[(plan[C_PSRC] >> pass) & 1];
-    const uint32_t tid = threadIdx.x, wave = tid / kWave;
-    const uint32_t tiles = tile_count(g);
-    const uint32_t t0 = blockIdx.x * run_tiles, t1 = min(tiles, t0 + run_tiles);
-    uint32_t run_acc = 0;
-    for (uint32_t t = t0; t < t1; t++) {
-        const uint4 td = tile_desc(g, t);
-        tile_run[(uint64_t)t * kBlock + tid] = run_acc;
-#pragma unroll
-        for (int w = 0; w < W; w++) s_cnt[w][tid] = 0;
-        __syncthreads();
-        for (uint32_t j0 = 0; j0 < td.z; j0 += kBlock * kItems) {
-            uint64_t k[kItems];
-#pragma unroll
-            for (int u = 0; u < kItems; u++) {
-                const uint32_t j = j0 + u * kBlock + tid;
-                k[u] = j < td.z ? __builtin_nontemporal_load(&keys[td.y + j]) : 0;
-            }
-#pragma unroll
-            for (int u = 0; u < kItems; u++) {
-                const uint32_t j = j0 + u * kBlock + tid;
-                if (j >= td.z) continue;
-                const uint32_t d = (uint32_t)(k[u] >> shift) & (kBlock - 1);
-                // a digit shared by every active lane costs one LDS atomic
-                const uint64_t active = __ballot(1);
-                const uint32_t d0 = __builtin_amdgcn_readfirstlane(d);
-                if (__ballot(d == d0) == active) {
-                    if (lane_id() == (uint32_t)__ffsll((unsigned long long)active) - 1)
-                        atomicAdd(&s_cnt[wave][d0], (uint32_t)__popcll(active));
-                } else {
-                    atomicAdd(&s_cnt[wave][d], 1u);
-                }
-            }
-        }
-        __syncthreads();
-        uint32_t c = 0;
-#pragma unroll
-        for (int w = 0; w < W; w++) c += s_cnt[w][tid];
-        run_acc += c;
-        __syncthreads();  // counts read before the next tile clears them
-    }
-    run_sum[(uint64_t)blockIdx.x * kBlock + tid] = run_acc;
 }
 
 // ---------------------------------------------------------------------------
@@ -1688,12 +1631,27 @@ __global__ void __launch_bounds__(kBlock) k_decode(const uint64_t* keys, const u
     }
 }
 
-// plain streaming copy of n u64 words: the calibration kernel for the HBM
-// byte counters (bytes read = bytes written = 8n)
-__global__ void __launch_bounds__(kBlock) k_copy_probe(const uint64_t* __restrict__ src, uint64_t* __restrict__ dst,
+// plain streaming copy of n 16-byte words: the calibration kernel for the HBM
+// byte counters (bytes read = bytes written = 16n) and the copy ceiling.
+// 16-byte words per lane (the widest global access), non-temporal loads and
+// stores, each block copying one contiguous 256 KiB chunk with eight loads in
+// flight per lane: the fastest of the copy shapes measured on the MI355X
+// (5.5 TB/s against 4.6 TB/s for an 8-byte grid-stride copy)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+constexpr uint64_t kProbeChunk = (256u << 10) / 16;  // 16-byte words per block
+__global__ void __launch_bounds__(kBlock) k_copy_probe(const u32x4* __restrict__ src, u32x4* __restrict__ dst,
                                                         uint64_t n) {
-    const uint64_t stride = (uint64_t)gridDim.x * kBlock;
-    for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+    constexpr int U = 8;
+    const uint64_t lo = (uint64_t)blockIdx.x * kProbeChunk, hi = min(lo + kProbeChunk, n);
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += (uint64_t)kBlock * U) {
+        u32x4 v[U];
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (i + u * kBlock < hi) v[u] = __builtin_nontemporal_load(&src[i + u * kBlock]);
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (i + u * kBlock < hi) __builtin_nontemporal_store(v[u], &dst[i + u * kBlock]);
+    }
 }
 
 __global__ void k_fill_u32(uint32_t* p, uint32_t v, uint64_t n) {
@@ -1911,7 +1869,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (load_pct < 25 || load_pct > 95) return fail(PYR_ERROR_INVALID_ARGUMENT, "hash_load_pct must be in [25, 95]");
     if (tun.status_words < 1 || tun.status_words > 2)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "status_words must be 1 or 2");
-    if (tun.counted_pass > 3) return fail(PYR_ERROR_INVALID_ARGUMENT, "counted_pass must be 1, 2 or 3");
+    if (tun.counted_pass > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "counted_pass must be 1 or 2");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -2096,7 +2054,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         hipLaunchKernelGGL(k_encode, dim3(eblocks), dim3(kBlock), passes * radix * sizeof(uint32_t), s, enc, seg,
                            per_block);
         if (counted)
-            hipLaunchKernelGGL(k_run_scan, dim3(1), dim3(1024), 0, s, run_sum, run_sum + (uint64_t)eblocks * radix,
+            hipLaunchKernelGGL(k_run_scan, dim3(kBlock), dim3(kBlock), 0, s, run_sum, run_sum + (uint64_t)eblocks * radix,
                                eblocks);
         hipLaunchKernelGGL(k_digit_scan, dim3(passes * segs), dim3(kBlock), 0, s, hist, digit_base, seg, passes,
                            radix, counters);
@@ -2108,15 +2066,12 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     // (the device skips passes that move nothing and picks each pass's
     // source buffer, so the buffers are named by index here)
     for (uint32_t p = 0; p < passes && n; p++) {
-        // pass 0 reads the records in input order (counted by encode); later
-        // passes are counted by a pass over their input when asked to
-        const bool use_counts = counted && (p == 0 || tun.counted_pass == 3);
-        if (use_counts && p > 0) {
-            hipLaunchKernelGGL(k_tile_count, dim3(eblocks), dim3(kBlock), 0, s, bufs, counters, p,
-                               (packed ? 32 : 0) + lb + p * rb, seg, tile_run, run_sum, per_block);
-            hipLaunchKernelGGL(k_run_scan, dim3(1), dim3(1024), 0, s, run_sum, run_sum + (uint64_t)eblocks * radix,
-                               eblocks);
-        }
+        // pass 0 reads the records in input order, counted by encode; later
+        // passes look back (counting them too was measured slower: their
+        // tiles hold most digits, and tiles that write whenever they are ready
+        // scatter their short runs over the whole output at once, while the
+        // look-back keeps the write-outs of neighbouring tiles together)
+        const bool use_counts = counted && p == 0;
         if (narrow && !use_counts) fill32(s, status32, 0, sort_tiles * radix);
         SortArgs sa{bufs, p, counters, seg, (packed ? 32 : 0) + lb + p * rb, digit_base + (uint64_t)p * segs * radix,
                     status, status32, ++epoch, tile_ctr(), counters + C_ERR, bucketed ? kSortPeel : 0u,
@@ -2619,11 +2574,14 @@ PYR_API PyrResult pyrhip_downsample(uint64_t src_keys, uint64_t src_vals, uint32
 
 PYR_API PyrResult pyrhip_copy_probe(uint64_t src, uint64_t dst, uint64_t bytes, void* stream) {
     diag_clear();
-    if (!src || !dst || (bytes & 7)) return fail(PYR_ERROR_INVALID_ARGUMENT, "copy probe needs 8-byte multiples");
-    const uint64_t n = bytes / 8;
+    if (!src || !dst || (bytes & 15) || ((src | dst) & 15))
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "copy probe needs 16-byte aligned buffers and sizes");
+    const uint64_t n = bytes / 16;
+    const uint64_t blocks = (n + kProbeChunk - 1) / kProbeChunk;
+    if (blocks > 0x7FFFFFFFull) return fail(PYR_ERROR_INVALID_ARGUMENT, "copy probe size");
     if (n)
-        hipLaunchKernelGGL(k_copy_probe, dim3(grid_for(n, kBlock * 8, 8192)), dim3(kBlock), 0, (hipStream_t)stream,
-                           (const uint64_t*)(uintptr_t)src, (uint64_t*)(uintptr_t)dst, n);
+        hipLaunchKernelGGL(k_copy_probe, dim3((uint32_t)blocks), dim3(kBlock), 0, (hipStream_t)stream,
+                           (const u32x4*)(uintptr_t)src, (u32x4*)(uintptr_t)dst, n);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "copy probe: %s", hipGetErrorString(e));
 }

@@ -227,7 +227,7 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // pass ten times slower than 8-bit digits on every workload measured)
         const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
         for (int dim = 0; dim < 7; dim++) {
-            for (int v = 0; v < (dim == 6 ? 3 : dim >= 4 ? 2 : 4); v++) {
+            for (int v = 0; v < (dim >= 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];
                 if (dim == 1) c.scan_items = scans[v];
