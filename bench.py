@@ -127,6 +127,9 @@ def build_section(args, out):
                      ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "sort_pass_ms": pass_ms,
         "autotune": tune,
+        # every buffer the build requests (scratch and outputs), all held at
+        # once: the build's peak device memory
+        "device_bytes": phases[-1]["device_bytes"], "device_bytes_per_voxel": phases[-1]["device_bytes"] / n,
     }
     # algorithmic bytes of the whole build: encode 16N read + rec*N write, P
     # moving passes x 2*rec*N, unique (bucket tables or scan) rec*N read + 12U
@@ -200,6 +203,8 @@ def trace_section(args, out):
     }
     if args.cpu_trace:
         cb = cpu_trace_baseline(args, built, cams[0], W, H)
+        out["traversal"]["parity_mismatches"] = cb["parity"]["mismatches"]
+        out["traversal"]["parity"] = cb.pop("parity")
         out["traversal"]["cpu_baseline"] = cb
         # traversal roofline: 32 B ray + 16 B hit + 4 B per distinct node word,
         # 8 B per distinct leaf and 4 B per attribute a ray reads, the distinct
@@ -218,6 +223,15 @@ def trace_section(args, out):
     for b in (hits, rays, occ):
         b.free()
     return dag, w, cams
+
+
+def _dag_of(built):
+    from pyrit_amd import abi
+
+    dag = abi.PyrhipDag()
+    dag.nodes, dag.leaves, dag.attrs = built.nodes, built.leaves, built.attrs
+    dag.root, dag.log2_size, dag.default_attribute = built.root, built.log2_size, 1
+    return dag
 
 
 def cpu_trace_baseline(args, built, cam, W, H):
@@ -241,9 +255,20 @@ def cpu_trace_baseline(args, built, cam, W, H):
            "kind": "port"}
     for label, threads in (("one_core", 1), ("all_cores", os.cpu_count() or 1)):
         t0 = time.perf_counter()
-        O.trace_rays(scene, rays, threads=threads)
+        want = O.trace_rays(scene, rays, threads=threads)
         dt = time.perf_counter() - t0
         res[label] = {"value": len(rays) / dt / 1e6, "cores": threads, "seconds": dt}
+    # parity of the timed kernel on the same rays: the camera kernel traces
+    # the half-resolution pose and its hits are compared with the oracle's
+    # (instance, attribute, face/inside bits and fp32 t, bit for bit)
+    from pyrit_amd import abi, gpu
+
+    hc = abi.PyrCamera.from_buffer_copy(bytes(oc))
+    got = gpu.trace_camera(_dag_of(built), hc)
+    gw, ww = got.view(np.uint32).reshape(-1, 4), want.view(np.uint32).reshape(-1, 4)
+    res["parity"] = {"rays": int(len(rays)), "mismatches": int(np.count_nonzero((gw != ww).any(axis=1))),
+                     "hits": int(np.count_nonzero(want["instance"] != 0xFFFFFFFF)),
+                     "check": "k_trace_camera hits vs the oracle's traceScene on the same DAG, bit for bit"}
     # the same rays once more, instrumented: distinct words each ray reads
     res["touch"] = O.trace_touch(scene, rays)
     log(f"cpu traversal {res['one_core']['value']:.2f} Mrays/s (1 core), "
@@ -350,6 +375,7 @@ def batch_section(args, out):
         "batched_gvoxels_per_s": n / (b_ms * 1e-3) / 1e9, "per_chunk_ms": p_ms,
         "per_chunk_gvoxels_per_s": n / (p_ms * 1e-3) / 1e9, "speedup_vs_per_chunk": p_ms / b_ms,
         "sort_passes": passes, "record_bytes": rec, "finish": bt["finish"], "unique": U, "algorithmic_bytes": b_alg,
+        "device_bytes": bt["device_bytes"], "device_bytes_per_voxel": bt["device_bytes"] / max(1, n),
         "algorithmic_GBps": b_alg / (b_ms * 1e-3) / 1e9, "hbm_frac": b_alg / (b_ms * 1e-3) / 1e9 / HBM_PEAK_GBPS,
         "phase_ms": {k: bt[k] for k in ("encode_ms", "sort_ms", "unique_ms", "leaves_ms", "levels_ms", "total_ms")},
         "per_chunk_latency_ms_p50": float(np.percentile(lat, 50)),

@@ -227,6 +227,8 @@ typedef struct PyrhipBuildTimes {
     float host_alloc_ms, host_enqueue_ms, host_wait_ms;
     uint32_t status_bytes; /* bytes per sort look-back word: 4 or 8 */
     uint32_t finish;       /* 2: partial sort + bucket tables, 1: full sort + unique scan */
+    uint64_t device_bytes; /* device memory the build requested, scratch and outputs (all held at once:
+                              the peak) */
 } PyrhipBuildTimes;
 PYR_API void pyrhip_set_timing(int enable);
 /* Plain device-to-device copy kernel of `bytes` (16-byte aligned buffers,

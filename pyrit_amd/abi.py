@@ -103,7 +103,7 @@ class PyrhipBuildTimes(C.Structure):
     _fields_ = [("encode_ms", f32), ("sort_ms", f32), ("unique_ms", f32), ("leaves_ms", f32), ("levels_ms", f32),
                 ("total_ms", f32), ("sort_passes", u32), ("radix_bits", u32), ("record_bytes", u32),
                 ("sort_items", u32), ("host_alloc_ms", f32), ("host_enqueue_ms", f32), ("host_wait_ms", f32),
-                ("status_bytes", u32), ("finish", u32)]
+                ("status_bytes", u32), ("finish", u32), ("device_bytes", u64)]
 
 
 class PyrhipBuildOutput(C.Structure):

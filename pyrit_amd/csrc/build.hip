@@ -1690,8 +1690,10 @@ struct Alloc {
     bool failed = false;
     uint8_t* base = nullptr;
     uint64_t size = 0, used = 0;
+    uint64_t requested = 0;  // bytes asked for so far
     void* raw(uint64_t bytes) {
         bytes = std::max<uint64_t>(bytes, 256);
+        requested += bytes;
         if (base) {
             const uint64_t at = (used + 255) & ~255ull;
             if (at + bytes > size) {
@@ -1900,11 +1902,41 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         uint64_t per = 1ull << (3 * (log2_size - std::min(lv, log2_size)));
         return std::max<uint64_t>(std::min<uint64_t>(nmax, per * k), 1);
     };
-    const uint64_t nb = cells(2);                 // bricks
-    const uint64_t np2 = std::min(nb, cells(3));  // parents of bricks
-    uint64_t word_bound = 2;
-    for (uint32_t lv = 2; lv < log2_size; lv++) word_bound += std::min(nb, cells(lv + 1)) * kNodeStride;
-    const uint64_t cap = hash_cap_for(nb, load_pct);
+    const uint64_t nb = cells(2);  // bricks, at most: the unique step writes this many brick slots
+    // Sizes of the leaf and level buffers for b bricks: parents of bricks,
+    // hash capacity, node words (each level's parents, at most one per child
+    // and one per cell of the level above, take 2 words plus one per child).
+    // The brick count is an upper bound from the record count unless the
+    // build reads the real count back after the unique step.
+    struct LevelSizes {
+        uint64_t nb, np2, cap, words;
+    };
+    auto level_sizes = [&](uint64_t b) {
+        LevelSizes z;
+        z.nb = std::max<uint64_t>(b, 1);
+        z.np2 = std::min(z.nb, cells(3));
+        z.cap = hash_cap_for(z.nb, load_pct);
+        z.words = 2;
+        uint64_t c = z.nb;
+        for (uint32_t lv = 2; lv < log2_size; lv++) {
+            const uint64_t par = std::min(c, cells(lv + 1));
+            z.words += 2 * par + c;
+            c = par;
+        }
+        return z;
+    };
+    // bytes of the buffers sized by the brick count (table + owner, slot,
+    // brick elements, parent elements and tuples, leaves, node words)
+    auto level_bytes = [](const LevelSizes& z) {
+        return z.cap * 12 + z.nb * (4 + 12 + 8) + z.np2 * (20 + (uint64_t)kNodeStride * 4 + 4) + z.words * 4;
+    };
+    LevelSizes lz = level_sizes(nb);
+    // when the bounds would hold more than 8 bytes per record for the
+    // brick-sized buffers (sparse inputs with about one brick per record,
+    // batches whose chunk cells far exceed their bricks) the build reads the
+    // brick count back once after the unique step and sizes them exactly;
+    // graph captures and the static scratch keep the bounds
+    const bool exact_levels = !gb && !fixed && level_bytes(lz) > 8ull * nmax + (4ull << 20);
 
     const auto host_t0 = std::chrono::steady_clock::now();
     Alloc A{{}, s};
@@ -1958,19 +1990,12 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     uint32_t* run_sum = counted ? A.get<uint32_t>((uint64_t)eblocks * radix * 2) : nullptr;
     uint64_t* brick_key = A.get<uint64_t>(nb);
     uint64_t* masks = A.get<uint64_t>(nb);
-    uint64_t* table = A.get<uint64_t>(cap);
-    uint32_t* owner = A.get<uint32_t>(cap);
-    uint32_t* slot = A.get<uint32_t>(nb);
+    uint64_t* table = nullptr;
+    uint32_t* owner = nullptr;
+    uint32_t* slot = nullptr;
     LevelArrays set[2];
-    const uint64_t set_cap[2] = {nb, np2};
-    for (int q = 0; q < 2; q++) {
-        set[q].key = q == 0 ? brick_key : A.get<uint64_t>(np2);
-        set[q].rep = A.get<uint32_t>(q == 0 ? nb : np2);
-        set[q].pos = A.get<uint32_t>(q == 0 ? nb : np2);
-        set[q].count = A.get<uint32_t>(q == 0 ? nb : np2);
-    }
-    uint32_t* node_tmp = A.get<uint32_t>(np2 * kNodeStride);
-    uint32_t* node_len = A.get<uint32_t>(np2);
+    uint32_t* node_tmp = nullptr;
+    uint32_t* node_len = nullptr;
     uint32_t* roots_dev = enc.chunk_count ? A.get<uint32_t>(enc.chunk_count * 2) : nullptr;
     uint32_t* first_dev = enc.chunk_count ? roots_dev + enc.chunk_count : nullptr;
     // caller-owned outputs; of the two key/value buffer pairs one ends up
@@ -1982,6 +2007,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     bool out_ok = true;
     auto out_alloc = [&](void** p, uint64_t bytes) -> bool {
         if (fixed) return (*p = A.raw(bytes)) != nullptr;
+        A.requested += std::max<uint64_t>(bytes, 256);
         return pool_alloc(p, bytes, s) == hipSuccess;
     };
     for (int q = 0; q < 2; q++) {
@@ -1989,8 +2015,25 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         // packed records need one value array (for the unique attributes)
         if (q == 0 || !packed) out_ok &= out_alloc((void**)&vb[q], (uint64_t)nmax * 4);
     }
-    out_ok &= out_alloc((void**)&leaves_out, nb * 8);
-    out_ok &= out_alloc((void**)&nodes_out, word_bound * 4);
+    // the buffers sized by the brick count, the leaves and node words among them
+    auto alloc_levels = [&]() {
+        bool ok = true;
+        table = A.get<uint64_t>(lz.cap);
+        owner = A.get<uint32_t>(lz.cap);
+        slot = A.get<uint32_t>(lz.nb);
+        for (int q = 0; q < 2; q++) {
+            set[q].key = q == 0 ? brick_key : A.get<uint64_t>(lz.np2);
+            set[q].rep = A.get<uint32_t>(q == 0 ? lz.nb : lz.np2);
+            set[q].pos = A.get<uint32_t>(q == 0 ? lz.nb : lz.np2);
+            set[q].count = A.get<uint32_t>(q == 0 ? lz.nb : lz.np2);
+        }
+        node_tmp = A.get<uint32_t>(lz.np2 * kNodeStride);
+        node_len = A.get<uint32_t>(lz.np2);
+        ok &= out_alloc((void**)&leaves_out, lz.nb * 8);
+        ok &= out_alloc((void**)&nodes_out, lz.words * 4);
+        return ok && !A.failed;
+    };
+    if (!exact_levels) out_ok &= alloc_levels();
     auto free_outputs = [&]() {
         if (fixed) return;
         for (int q = 0; q < 2; q++) {
@@ -2108,20 +2151,40 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                              first_dev, nb);
     }
     tm.mark();
+    // the real brick count, when the brick-sized buffers are sized by it
+    if (exact_levels) {
+        uint32_t bricks = 0;
+        hipError_t be = hipMemcpyAsync(&bricks, counters + C_B, sizeof(bricks), hipMemcpyDeviceToHost, s);
+        if (be == hipSuccess) be = hipStreamSynchronize(s);
+        if (be != hipSuccess) {
+            (void)hipGetLastError();
+            A.release();
+            free_outputs();
+            return fail(PYR_ERROR_CUDA, "build: %s", hipGetErrorString(be));
+        }
+        lz = level_sizes(std::min<uint64_t>(bricks, nb));  // (a failed unique step may report anything)
+        if (!alloc_levels()) {
+            A.release();
+            free_outputs();
+            (void)hipGetLastError();
+            return fail(PYR_ERROR_OUT_OF_MEMORY, "device memory for a build of %u voxels", n);
+        }
+    }
     // 4. leaves
-    const uint32_t brick_grid = grid_for(nb, kScanTile, 2048);
-    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(cap, 1024, 2048)), dim3(kBlock), 0, s, counters, (uint32_t)C_B,
-                       (void*)table, 8u, owner, cap, load_pct);
-    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(nb, kBlock * 4, 4096)), dim3(kBlock), 0, s, counters, masks,
-                       table, owner, slot, set[0].count, cap, (uint32_t)nb, load_pct);
+    const uint32_t brick_grid = grid_for(lz.nb, kScanTile, 2048);
+    hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(lz.cap, 1024, 2048)), dim3(kBlock), 0, s, counters, (uint32_t)C_B,
+                       (void*)table, 8u, owner, lz.cap, load_pct);
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(lz.nb, kBlock * 4, 4096)), dim3(kBlock), 0, s, counters, masks,
+                       table, owner, slot, set[0].count, lz.cap, (uint32_t)lz.nb, load_pct);
     {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_leaf_compact, dim3(brick_grid), dim3(kBlock), 0, s, sc, masks, owner, slot,
-                           leaves_out, set[0].rep, set[0].pos, (uint32_t)nb);
+                           leaves_out, set[0].rep, set[0].pos, (uint32_t)lz.nb);
     }
     tm.mark();
     // 5. node levels
     int ch = 0;
+    const uint64_t set_cap[2] = {lz.nb, lz.np2};
     for (uint32_t lv = 2; lv < log2_size; lv++) {
         LevelArgs la{};
         la.lv = lv;
@@ -2136,26 +2199,26 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         la.owner = owner;
         la.slot = slot;
         la.ch_cap = (uint32_t)set_cap[ch];
-        la.par_cap = (uint32_t)std::min(set_cap[ch ^ 1], np2);
-        la.table_cap = cap;
-        la.word_cap = word_bound;
+        la.par_cap = (uint32_t)std::min(set_cap[ch ^ 1], lz.np2);
+        la.table_cap = lz.cap;
+        la.word_cap = lz.words;
         la.chunk_count = enc.chunk_count;
         la.load_pct = load_pct;
-        const uint64_t nch = lv == 2 ? nb : std::min(nb, cells(lv));
+        const uint64_t nch = lv == 2 ? lz.nb : std::min(lz.nb, cells(lv));
         const uint32_t g = grid_for(nch, kScanTile, 2048);
         hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(hash_cap_for(nch, load_pct), 1024, 2048)), dim3(kBlock), 0, s,
-                           counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, cap, load_pct);
+                           counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, lz.cap, load_pct);
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_build, dim3(g), dim3(kBlock), 0, s, sc, la);
-        hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(nb, cells(lv + 1)), kBlock * 4, 4096)),
+        hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(lz.nb, cells(lv + 1)), kBlock * 4, 4096)),
                            dim3(kBlock), 0, s, counters, la);
         ScanCtx sc2{status, ++epoch, tile_ctr(), counters};
-        hipLaunchKernelGGL(k_level_compact, dim3(grid_for(std::min(nb, cells(lv + 1)), kScanTile, 2048)),
+        hipLaunchKernelGGL(k_level_compact, dim3(grid_for(std::min(lz.nb, cells(lv + 1)), kScanTile, 2048)),
                            dim3(kBlock), 0, s, sc2, la, nodes_out);
         ch ^= 1;
     }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, s, counters, log2_size, set[ch], roots_dev,
-                       nodes_out, (uint32_t)std::min(set_cap[ch], np2), enc.chunk_count, word_bound);
+                       nodes_out, (uint32_t)std::min(set_cap[ch], lz.np2), enc.chunk_count, lz.words);
     tm.mark();
 
     // single read-back (gpu_build.zig:338-344); a graph copies into its pinned buffer
@@ -2215,6 +2278,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         g_times.sort_items = sort_items;
         g_times.status_bytes = narrow ? 4 : 8;
         g_times.finish = bucketed ? 2 : 1;
+        g_times.device_bytes = A.requested;
         using ms = std::chrono::duration<float, std::milli>;
         g_times.host_alloc_ms = ms(host_t1 - host_t0).count();
         g_times.host_enqueue_ms = ms(host_t2 - host_t1).count();

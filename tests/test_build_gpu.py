@@ -166,6 +166,33 @@ def test_counted_and_lookback_first_pass_identical(hip_device, counted_pass):
     assert np.array_equal(roots, rr) and np.array_equal(first, rf)
 
 
+def test_build_memory_per_voxel(hip_device):
+    """The reference's bound on the build's device memory (tests/cpu_test.zig:
+    1559-1629): every buffer the build requests, scratch and outputs, held
+    at once, below 130 bytes per voxel for the surface of a sphere in a 256^3
+    grid (y = 127.5 + sqrt(110^2 - dx^2 - dz^2) per column, f32)."""
+    print("case: build memory per voxel, sphere surface 256^3", flush=True)
+    g = _gpu()
+    c = np.arange(256, dtype=np.float32) - np.float32(127.5)
+    dz, dx = np.meshgrid(c, c, indexing="ij")
+    r2 = np.float32(110.0 * 110.0) - dx * dx - dz * dz
+    z, x = np.nonzero(r2 > 0)
+    y = (np.float32(127.5) + np.sqrt(r2[z, x])).astype(np.uint32)
+    pts = np.zeros(len(x), O.VOXEL)
+    pts["x"], pts["y"], pts["z"], pts["attribute"] = x, y, z, 0x00FF00FF
+    g.set_timing(True)
+    try:
+        b = g.build(8, pts)
+        t = g.last_build_times()
+    finally:
+        g.set_timing(False)
+    per_voxel = t["device_bytes"] / len(pts)
+    print(f"  {len(pts)} voxels, {t['device_bytes'] / 2**20:.1f} MiB, {per_voxel:.0f} bytes per voxel", flush=True)
+    assert b.voxel_count == len(pts)
+    assert per_voxel < 130
+    assert_same_build(b, O.gb_build(8, pts), 8)
+
+
 @pytest.mark.parametrize("load", [25, 50, 95])
 def test_hash_load_identical(hip_device, load):
     # the hash capacity changes probe lengths only: owners are the smallest
