@@ -50,6 +50,8 @@ def parse():
                     help="build with the built-in defaults instead of the setting the autotuner picks for the "
                          "workload class (tuning runs before the timed region)")
     ap.add_argument("--log2", type=int, default=10, help="edge of the headline grid (10 = 1024^3)")
+    ap.add_argument("--serve-eager-only", action="store_true",
+                    help="serving section: only the run that builds every batch eagerly (no graph replays)")
     return ap.parse_args()
 
 
@@ -479,7 +481,7 @@ def serve_section(args, out):
     inputs = [(w.chunks[c].log2_size, vox.ptr + int(offs[c]) * 16, int(offs[c + 1] - offs[c])) for c in range(count)]
     duration = 3.0 if not args.quick else 1.0
 
-    def run(graphs):
+    def run(graphs, rate=2000.0, seconds=duration):
         sv = serving.Server(budget=64 << 30, workers=2, graphs=graphs)
         # warm the pool and the kernels (and the graphs) with a few batches
         for c in range(8):
@@ -489,7 +491,7 @@ def serve_section(args, out):
             sv.build(c, *inputs[c])
             sv.drain()
         sv.poll()
-        res, elapsed, n_req = serving.poisson_run(sv, inputs, rate=2000.0, duration=duration, seed=7)
+        res, elapsed, n_req = serving.poisson_run(sv, inputs, rate=rate, duration=seconds, seed=7)
         st = sv.stats()
         sv.close()
         lat = np.array([(c.done_s - c.submit_s) * 1e3 for c in res if c.result == 0])
@@ -508,10 +510,29 @@ def serve_section(args, out):
             "batches": st["batches"], "mean_batch_chunks": float(sizes.mean()) if len(sizes) else 0.0,
             "device_busy_fraction": st["busy_s"] / elapsed, "voxels_built": st["batched_voxels"],
             "resident_chunks": st["resident_chunks"], "resident_GB": st["resident_bytes"] / 1e9,
+            "offered_rate": rate,
         }
 
+    if args.serve_eager_only:
+        eager = run(False)
+        eager["workload"] = (f"Poisson 2000 req/s, 80% builds ({w.name} sizes) / 20% edits of 1..512 voxels, "
+                             "64 GB budget; every batch built eagerly")
+        out["serving"] = eager
+        vox.free()
+        return
     res = run(True)
     eager = run(False)
+    # the same stream at higher arrival rates: batches grow as requests wait
+    # for a free worker, which is what keeps up with the load
+    sweep = []
+    for rate in (4000.0, 8000.0, 16000.0):
+        r = run(True, rate=rate, seconds=min(duration, 2.0))
+        sweep.append({k: r[k] for k in ("offered_rate", "requests", "rejected", "builds_per_s", "requests_per_s",
+                                         "mean_batch_chunks", "latency_ms_p50", "latency_ms_p99",
+                                         "device_busy_fraction", "voxels_built", "elapsed_s")})
+        log(f"serving at {rate:.0f}/s: {r['requests_per_s']:.0f} req/s, mean batch {r['mean_batch_chunks']:.2f}, "
+            f"p50 {r['latency_ms_p50']:.2f} ms p99 {r['latency_ms_p99']:.2f} ms, busy {r['device_busy_fraction']:.2f}")
+    res["load_sweep"] = [{k: res[k] for k in sweep[0]}] + sweep
     res["workload"] = (f"Poisson 2000 req/s, 80% builds ({w.name} sizes) / 20% edits of 1..512 voxels, 64 GB budget; "
                        "small batches replay HIP graphs")
     res["eager"] = eager

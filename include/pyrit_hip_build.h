@@ -150,7 +150,7 @@ PYR_API PyrResult pyrhip_graph_create(uint32_t log2_size, uint32_t count, uint32
 /* device address of the graph's voxel input buffer */
 PYR_API uint64_t pyrhip_graph_voxels(PyrhipGraph* graph);
 /* offsets: host u32[count + 1]; chunk_log2: host u8[count] for mixed sizes, else NULL; voxels: device
- * (0 = already in the input buffer) */
+ * (0 = already in the input buffer). The replay runs on `stream` and has finished when the call returns. */
 PYR_API PyrResult pyrhip_graph_launch(PyrhipGraph* graph, uint64_t voxels, const uint32_t* offsets,
                                       const uint8_t* chunk_log2, void* stream, PyrhipBuilt* out, uint32_t* roots,
                                       uint32_t* first_voxel);

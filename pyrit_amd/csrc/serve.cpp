@@ -25,6 +25,7 @@
 #include <deque>
 #include <memory>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <unordered_map>
 #include <unordered_set>
@@ -85,6 +86,8 @@ struct PyrhipServer {
     PyrhipServerInfo info{};
     int device = 0;
     std::mutex m;
+    // graph captures exclusive of every worker's graph replays
+    std::shared_mutex graph_mx;
     std::condition_variable cv_work, cv_done;
     std::deque<Req> pending;
     std::unordered_map<uint64_t, ChunkState> chunks;
@@ -276,9 +279,12 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
     PyrhipGraph* graph = nullptr;
     if (graphs && !(info.flags & PYRHIP_SERVER_NO_GRAPHS) && k <= kGraphChunks && total <= kGraphVoxels) {
         PyrhipGraph*& slot = (*graphs)[top];
-        if (!slot && pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
-            slot = nullptr;  // not enough memory for it: build eagerly
-            diag_clear();
+        if (!slot) {
+            std::unique_lock<std::shared_mutex> gl(graph_mx);
+            if (pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
+                slot = nullptr;  // not enough memory for it: build eagerly
+                diag_clear();
+            }
         }
         graph = slot;
     }
@@ -287,7 +293,11 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         std::vector<uint8_t> pl(kGraphChunks, 3);
         std::copy(offs.begin(), offs.end(), po.begin());
         std::copy(l2.begin(), l2.end(), pl.begin());
-        r = pyrhip_graph_launch(graph, (uint64_t)(uintptr_t)vox, po.data(), pl.data(), s, &b, gr.data(), gf.data());
+        {
+            std::shared_lock<std::shared_mutex> gl(graph_mx);
+            r = pyrhip_graph_launch(graph, (uint64_t)(uintptr_t)vox, po.data(), pl.data(), s, &b, gr.data(),
+                                    gf.data());
+        }
         std::copy(gr.begin(), gr.begin() + k, roots.begin());
         std::copy(gf.begin(), gf.begin() + k, first.begin());
     } else {
@@ -412,7 +422,10 @@ void PyrhipServer::worker_main() {
     lk.unlock();
     pool_free(input, s);
     (void)hipStreamSynchronize(s);
-    for (auto& kv : graphs) pyrhip_graph_destroy(kv.second);
+    {
+        std::unique_lock<std::shared_mutex> gl(graph_mx);
+        for (auto& kv : graphs) pyrhip_graph_destroy(kv.second);
+    }
     (void)hipStreamDestroy(s);
 }
 
