@@ -71,10 +71,12 @@ struct Req {
     uint64_t records = 0, est = 0;  // set when the request joins a batch
 };
 
-// device bytes a batch record may need: two u64 + one/two u32 record buffers,
-// brick and level arrays, hash table and status words (all bounded by the
-// record count), and the exact-size copy of the outputs
-constexpr uint64_t kBytesPerRecord = 96;
+// device bytes a batch record may need: what a build requests per record
+// (PyrhipBuildTimes.device_bytes, all of it held at once; measured 23 B on
+// the 1024^3 grid, 21-42 B on the 1024-chunk batch, 57 B on the sparse
+// 256^3 sphere surface, the worst case measured) plus the exact-size copy of
+// the outputs the server keeps (12 B per unique voxel, nodes and leaves)
+constexpr uint64_t kBytesPerRecord = 72;
 constexpr uint64_t kBytesPerRequest = 1ull << 20;
 
 }  // namespace

@@ -141,7 +141,7 @@ def test_admission_rejects_what_cannot_fit(hip_device):
     small = random_points(500, 5, 6)
     b1, b2 = to_device(pts), to_device(small)
     # budget below what a 20000-record request needs, above a 500-record one
-    sv = Server(budget=(1 << 20) + 500 * 96 + 200000, workers=2)
+    sv = Server(budget=(1 << 20) + 500 * 72 + 200000, workers=2)
     t_big = sv.build(1, 6, b1.ptr, len(pts))
     t_small = sv.build(2, 5, b2.ptr, len(small))
     sv.drain()
