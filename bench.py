@@ -146,8 +146,44 @@ def build_section(args, out):
                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                        "traffic": measured_traffic(kname, n), "bytes_per_launch": pass_bytes,
                        "avg_launch_ms": pass_ms}
+    rt_prims_section(args, out, vox, n)
     vox.free()
     return n, ms
+
+
+def rt_prims_section(args, out, vox, n):
+    """Box primitives of the built grid (rt_prims.extract as its own
+    operator): wall time of the counting call plus the writing call, each of
+    which ends synchronised, at the reference's default edge 2^7 and at 2^3."""
+    from pyrit_amd import abi, gpu, lib
+    from pyrit_amd.device import Buffer
+
+    L = lib()
+    built = abi.PyrhipBuilt()
+    abi.check(L, L.pyrhip_build(args.log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build")
+    b = gpu.Built(built)
+    res = {}
+    for req in (7, 3):
+        count, k = C.c_uint32(), C.c_uint32()
+        d = b.dag()
+        abi.check(L, L.pyrhip_rt_prims(C.byref(d), b.node_words, b.leaf_count, req, 0, 0, 0, None, C.byref(count),
+                                       C.byref(k)), "rt prims")
+        cap = count.value
+        pb, ab = Buffer(max(cap, 1) * 16), Buffer(max(cap, 1) * 24)
+        times = []
+        for i in range(6):
+            t0 = time.perf_counter()
+            abi.check(L, L.pyrhip_rt_prims(C.byref(d), b.node_words, b.leaf_count, req, 0, 0, 0, None,
+                                           C.byref(count), C.byref(k)), "rt prims")
+            abi.check(L, L.pyrhip_rt_prims(C.byref(d), b.node_words, b.leaf_count, req, pb.ptr, ab.ptr, cap, None,
+                                           C.byref(count), C.byref(k)), "rt prims")
+            times.append((time.perf_counter() - t0) * 1e3)
+        res[f"k{req}"] = {"primitives": count.value, "ms": float(np.median(times[1:]))}
+        pb.free()
+        ab.free()
+    b.free()
+    out["rt_prims"] = res
+    log(f"rt prims: {res}")
 
 
 def trace_section(args, out):
@@ -637,7 +673,7 @@ def main() -> int:
         "config": {"workload": "1024^3 heightfield grid (4 octaves, amplitude 256, 2000 caves), full-depth build",
                    "voxels": n, "log2_size": args.log2, "parallelism": "single GPU"},
     }
-    for k in ("roofline", "cpu_baseline", "build", "traversal", "batched", "serving", "copy_probe"):
+    for k in ("roofline", "cpu_baseline", "build", "rt_prims", "traversal", "batched", "serving", "copy_probe"):
         if k in out:
             res[k] = out[k]
     print(json.dumps(res), flush=True)

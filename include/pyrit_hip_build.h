@@ -273,6 +273,32 @@ PYR_API PyrResult pyrhip_path_edit(const PyrhipDag* dag, uint32_t voxel_count, u
                                    const uint32_t* edits, uint32_t count, uint64_t attrs_out, void* stream,
                                    uint32_t* out_root, uint32_t* out_count, uint32_t* out_flags);
 
+/* One box primitive of a DAG (src/device/types.zig:518-525 RtPrim): the
+ * subtree root `node` (word offset), the DFS attribute rank of its first
+ * voxel, and its cell in units of the primitive edge, x | y << 21 | z << 42
+ * (src/device/rt.zig:16-18 packCell). 16 bytes. */
+typedef struct PyrhipRtPrim {
+    uint32_t node;
+    uint32_t attr_base;
+    uint64_t cell;
+} PyrhipRtPrim;
+
+/* Box primitives of a finished DAG (src/rt_prims.zig:127-137 extract; the
+ * same list the reference's GPU build emits at rt_log2, src/device/
+ * gbuild.zig:574-605, 251-257): every non-empty subtree of edge 2^k,
+ * k = clamp(requested_log2, 3, log2_size) (*out_log2), in depth-first order,
+ * with its voxel box (max exclusive) widened by 1/256: aabbs[6 i ..] = min
+ * x, y, z, max x, y, z as float (the OptixAabb layout, rt_prims.zig:14).
+ * node_words / leaf_count bound every reference the call reads. prims and
+ * aabbs (device, capacity entries) may both be 0: then only *out_count is
+ * set. More primitives than capacity: PYR_ERROR_CAPACITY with *out_count the
+ * number needed, nothing written. For a chunk batch, call once per chunk root
+ * and append: the running count before a chunk is its first primitive
+ * (gbuild.zig:251-257 prim_start). */
+PYR_API PyrResult pyrhip_rt_prims(const PyrhipDag* dag, uint32_t node_words, uint32_t leaf_count,
+                                  uint32_t requested_log2, uint64_t prims, uint64_t aabbs, uint32_t capacity,
+                                  void* stream, uint32_t* out_count, uint32_t* out_log2);
+
 /* PyrRay[count] -> PyrHit[count] (device pointers) against one DAG, the
  * traversal of src/device/dag.zig:160-351 plus toHit (src/device/trace.zig:
  * 115-118) with instance 0. flags: PYR_TRACE_ANY_HIT / PYR_TRACE_NO_ATTRIBUTE. */

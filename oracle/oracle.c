@@ -1360,3 +1360,148 @@ void or_palette_pack(const uint32_t* attrs, const uint32_t* first, uint32_t k, u
     }
     word_offset[k] = words;
 }
+
+/* ------------------------------------------------------------------------ */
+/* RT primitives: rt_prims.zig:47-137                                        */
+/* ------------------------------------------------------------------------ */
+
+/* voxel box, max exclusive (rt_prims.zig:31-44) */
+typedef struct OrBox { uint32_t lo[3], hi[3]; } OrBox;
+
+static OrBox box_empty(void) {
+    OrBox b = {{0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu}, {0, 0, 0}};
+    return b;
+}
+
+static void box_merge(OrBox* a, const OrBox* b, const uint32_t off[3]) {
+    for (int i = 0; i < 3; i++) {
+        if (b->lo[i] + off[i] < a->lo[i]) a->lo[i] = b->lo[i] + off[i];
+        if (b->hi[i] + off[i] > a->hi[i]) a->hi[i] = b->hi[i] + off[i];
+    }
+}
+
+/* rt_prims.zig:46-55: bit = x + 4y + 16z */
+static OrBox leaf_box(uint64_t mask) {
+    OrBox b = box_empty();
+    const uint32_t zero[3] = {0, 0, 0};
+    for (uint64_t m = mask; m; m &= m - 1) {
+        const uint32_t bit = (uint32_t)__builtin_ctzll(m);
+        OrBox v = {{bit & 3, (bit >> 2) & 3, bit >> 4}, {(bit & 3) + 1, ((bit >> 2) & 3) + 1, (bit >> 4) + 1}};
+        box_merge(&b, &v, zero);
+    }
+    return b;
+}
+
+/* memo of boxes per (level, node), rt_prims.zig:61 */
+typedef struct BoxMemo { uint64_t* key; OrBox* box; uint64_t cap, n; } BoxMemo;
+
+static uint64_t memo_slot(const BoxMemo* m, uint64_t key) {
+    uint64_t h = (key * 0x9E3779B97F4A7C15ull) >> 20;
+    for (;; h++) {
+        const uint64_t s = h & (m->cap - 1);
+        if (m->key[s] == 0 || m->key[s] == key) return s;
+    }
+}
+
+static int memo_grow(BoxMemo* m) {
+    BoxMemo g = {calloc(m->cap * 2, 8), malloc(m->cap * 2 * sizeof(OrBox)), m->cap * 2, m->n};
+    if (!g.key || !g.box) {
+        free(g.key);
+        free(g.box);
+        return -1;
+    }
+    for (uint64_t s = 0; s < m->cap; s++)
+        if (m->key[s]) {
+            const uint64_t t = memo_slot(&g, m->key[s]);
+            g.key[t] = m->key[s];
+            g.box[t] = m->box[s];
+        }
+    free(m->key);
+    free(m->box);
+    *m = g;
+    return 0;
+}
+
+typedef struct Extract {
+    const uint32_t* nodes;
+    const uint64_t* leaves;
+    uint32_t k;
+    BoxMemo memo;
+    OrRtPrim* prims;
+    float* aabbs;
+    uint64_t capacity, count;
+    int oom;
+} Extract;
+
+/* rt_prims.zig:73-90: box of a node covering a cube of edge 2^level */
+static OrBox ex_box(Extract* e, uint32_t node, uint32_t level) {
+    const uint64_t key = ((uint64_t)level << 32 | node) + 1; /* +1: 0 marks an empty memo slot */
+    uint64_t s = memo_slot(&e->memo, key);
+    if (e->memo.key[s]) return e->memo.box[s];
+    OrBox b = box_empty();
+    const uint32_t mask = e->nodes[node] & 0xFF, child = 1u << (level - 1);
+    uint32_t j = 0;
+    for (uint32_t idx = 0; idx < 8; idx++) {
+        if (!(mask >> idx & 1)) continue;
+        const uint32_t ref = e->nodes[node + 2 + j++];
+        const OrBox cb = level == 3 ? leaf_box(e->leaves[ref]) : ex_box(e, ref, level - 1);
+        const uint32_t off[3] = {(idx & 1) * child, (idx >> 1 & 1) * child, (idx >> 2) * child};
+        box_merge(&b, &cb, off);
+    }
+    if (2 * (e->memo.n + 1) > e->memo.cap && memo_grow(&e->memo) != 0) {
+        e->oom = 1;
+        return b;
+    }
+    s = memo_slot(&e->memo, key);
+    e->memo.key[s] = key;
+    e->memo.box[s] = b;
+    e->memo.n++;
+    return b;
+}
+
+/* rt_prims.zig:92-124: DFS to level k, one primitive per non-empty subtree */
+static void ex_visit(Extract* e, uint32_t node, uint32_t level, const uint32_t origin[3], uint32_t rank) {
+    if (level == e->k) {
+        const OrBox b = ex_box(e, node, level);
+        if (b.lo[0] >= b.hi[0]) return; /* empty */
+        if (e->count < e->capacity && e->prims) {
+            OrRtPrim* p = &e->prims[e->count];
+            p->node = node;
+            p->attr_base = rank;
+            p->cell = (uint64_t)(origin[0] >> e->k) | (uint64_t)(origin[1] >> e->k) << 21 |
+                      (uint64_t)(origin[2] >> e->k) << 42; /* rt.zig:16-18 packCell */
+            const float eps = 1.0f / 256.0f;
+            for (int i = 0; i < 3; i++) {
+                e->aabbs[e->count * 6 + i] = (float)(origin[i] + b.lo[i]) - eps;
+                e->aabbs[e->count * 6 + 3 + i] = (float)(origin[i] + b.hi[i]) + eps;
+            }
+        }
+        e->count++;
+        return;
+    }
+    const uint32_t mask = e->nodes[node] & 0xFF, child = 1u << (level - 1);
+    uint32_t r = rank, j = 0;
+    for (uint32_t idx = 0; idx < 8; idx++) {
+        if (!(mask >> idx & 1)) continue;
+        const uint32_t ref = e->nodes[node + 2 + j++];
+        const uint32_t o[3] = {origin[0] + (idx & 1) * child, origin[1] + (idx >> 1 & 1) * child,
+                               origin[2] + (idx >> 2) * child};
+        ex_visit(e, ref, level - 1, o, r);
+        r += e->nodes[ref + 1];
+    }
+}
+
+/* rt_prims.zig:127-137 extract */
+int64_t or_rt_prims(const uint32_t* nodes, const uint64_t* leaves, uint32_t root, uint32_t log2_size,
+                    uint32_t requested_log2, OrRtPrim* prims, float* aabbs, uint64_t capacity, uint32_t* out_k) {
+    const uint32_t k = requested_log2 < 3 ? 3 : requested_log2 > log2_size ? log2_size : requested_log2;
+    Extract e = {nodes, leaves, k, {calloc(1024, 8), malloc(1024 * sizeof(OrBox)), 1024, 0}, prims, aabbs, capacity,
+                 0, 0};
+    if (!e.memo.key || !e.memo.box) e.oom = 1;
+    const uint32_t origin[3] = {0, 0, 0};
+    if (!e.oom) ex_visit(&e, root, log2_size, origin, 0);
+    free(e.memo.key);
+    free(e.memo.box);
+    if (out_k) *out_k = k;
+    return e.oom ? -1 : (int64_t)e.count;
+}

@@ -36,6 +36,11 @@
  *   or_palette_pack     : decoding the packed words reproduces every attribute
  *                         (brute force), the 16-value boundary and the word
  *                         counts of context.zig:1352-1357
+ *   or_rt_prims         : known answer rt_prims.zig:139-160; brute force from
+ *                         the voxel set (cells = occupied 2^k cubes in Morton
+ *                         order, box = min/max of their voxels, attr_base =
+ *                         voxels before the cell in Morton order, node = a
+ *                         subtree whose lookups give exactly the cell's voxels)
  *   or_trace_touch      : word counts derived by hand on a one-voxel DAG (hit,
  *                         miss through the same leaf, ray outside the bounds),
  *                         per-ray distinctness, hit count equal to the tracer's
@@ -175,6 +180,17 @@ void     or_range_free(OrRangeAlloc* r);
  * layout only. */
 void or_palette_pack(const uint32_t* attrs, const uint32_t* first, uint32_t k, uint32_t* count,
                      uint64_t* word_offset, uint32_t* out);
+
+/* RT primitives (rt_prims.zig:47-137 extract; the RtPrim of types.zig:
+ * 518-525): every non-empty subtree of edge 2^k, k = clamp(requested, 3,
+ * log2_size), in DFS order: {node word, DFS rank of its first voxel, cell
+ * x | y << 21 | z << 42 in units of 2^k} and its voxel box (max exclusive)
+ * widened by 1/256, aabbs[6 i ..] = min xyz, max xyz. Writes the first
+ * `capacity` primitives (prims/aabbs may be NULL), returns the total count, -1
+ * when memory runs out; *out_k = k. */
+typedef struct OrRtPrim { uint32_t node, attr_base; uint64_t cell; } OrRtPrim;
+int64_t or_rt_prims(const uint32_t* nodes, const uint64_t* leaves, uint32_t root, uint32_t log2_size,
+                    uint32_t requested_log2, OrRtPrim* prims, float* aabbs, uint64_t capacity, uint32_t* out_k);
 
 #ifdef __cplusplus
 }

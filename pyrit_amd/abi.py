@@ -13,6 +13,7 @@ vp = C.c_void_p
 VOXEL = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("attribute", "<u4")])
 RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
+RT_PRIM = np.dtype([("node", "<u4"), ("attr_base", "<u4"), ("cell", "<u8")])  # PyrhipRtPrim
 
 PYR_OK = 0
 PYR_ERROR_INVALID_ARGUMENT = -1
@@ -245,6 +246,7 @@ FUNCTIONS = {
     "pyrhip_tuning_clear": (None, []),
     "pyrhip_path_edit": (i32, [P(PyrhipDag), u32, u32, u32, u32, u32, P(u32), u32, u64, vp, P(u32), P(u32), P(u32)]),
     "pyrhip_palette_pack": (i32, [u64, P(u32), u32, u64, u64, P(u64), P(u32), vp]),
+    "pyrhip_rt_prims": (i32, [P(PyrhipDag), u32, u32, u32, u64, u64, u32, vp, P(u32), P(u32)]),
     "pyrhip_downsample": (i32, [u64, u64, u32, u32, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
     "pyrhip_set_timing": (None, [i32]),
     "pyrhip_copy_probe": (i32, [u64, u64, u64, vp]),

@@ -3,7 +3,7 @@
 Compiles, in tree and for gfx950 only:
   pyrit_amd/libpyrit.so          build.hip, trace.hip, api.cpp, serve.cpp,
                                  world.cpp, devcache.cpp, dag.cpp, tune.cpp,
-                                 palette.hip, pathedit.hip
+                                 palette.hip, pathedit.hip, rtprims.hip
                                  (the C ABI of
                                  include/pyrit.h and include/pyrit_hip_build.h)
   pyrit_amd/libpyrit_terrain.so  terrain.hip (seeded workload generator)
@@ -36,7 +36,7 @@ COMMON = [
 ]
 
 LIBS = {
-    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp", "dag.cpp", "tune.cpp", "palette.hip", "pathedit.hip"],
+    "libpyrit.so": ["build.hip", "trace.hip", "api.cpp", "serve.cpp", "world.cpp", "devcache.cpp", "dag.cpp", "tune.cpp", "palette.hip", "pathedit.hip", "rtprims.hip"],
     "libpyrit_terrain.so": ["terrain.hip"],
 }
 

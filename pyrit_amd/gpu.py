@@ -190,6 +190,22 @@ def trace_shadow(dag: abi.PyrhipDag, rays: np.ndarray, hits: np.ndarray, sun, st
     return out.download(np.uint32, count)
 
 
+def rt_prims(dag: abi.PyrhipDag, node_words: int, leaf_count: int, requested_log2: int,
+             stream: int | None = None):
+    """Box primitives of a DAG (src/rt_prims.zig:127-137 extract):
+    (k, prims RT_PRIM[n], aabbs float32[n, 6])."""
+    L = lib()
+    count, k = C.c_uint32(0), C.c_uint32(0)
+    abi.check(L, L.pyrhip_rt_prims(C.byref(dag), node_words, leaf_count, requested_log2, 0, 0, 0, stream,
+                                   C.byref(count), C.byref(k)), "pyrhip_rt_prims")
+    n = count.value
+    prims = Buffer(max(n, 1) * abi.RT_PRIM.itemsize)
+    aabbs = Buffer(max(n, 1) * 24)
+    abi.check(L, L.pyrhip_rt_prims(C.byref(dag), node_words, leaf_count, requested_log2, prims.ptr, aabbs.ptr, n,
+                                   stream, C.byref(count), C.byref(k)), "pyrhip_rt_prims")
+    return k.value, prims.download(abi.RT_PRIM, n), aabbs.download(np.float32, 6 * n).reshape(n, 6)
+
+
 def set_timing(on: bool) -> None:
     lib().pyrhip_set_timing(1 if on else 0)
 

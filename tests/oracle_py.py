@@ -18,6 +18,7 @@ P = C.POINTER
 VOXEL = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("attribute", "<u4")])
 RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
+RT_PRIM = np.dtype([("node", "<u4"), ("attr_base", "<u4"), ("cell", "<u8")])
 
 
 class OrDag(C.Structure):
@@ -108,6 +109,8 @@ def lib() -> C.CDLL:
         L.or_range_release.argtypes = [P(OrRangeAlloc), u64, u64]
         L.or_range_free.argtypes = [P(OrRangeAlloc)]
         L.or_palette_pack.argtypes = [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p]
+        L.or_rt_prims.restype = C.c_int64
+        L.or_rt_prims.argtypes = [C.c_void_p, C.c_void_p, u32, u32, u32, C.c_void_p, C.c_void_p, u64, P(u32)]
         _lib = L
     return _lib
 
@@ -353,3 +356,18 @@ def palette_pack(attrs: np.ndarray, first: np.ndarray):
     out = np.zeros(max(int(offs[-1]), 1), np.uint32)
     L.or_palette_pack(_p(attrs), _p(first), k, _p(counts), _p(offs), _p(out))
     return counts, offs, out[:int(offs[-1])]
+
+
+def rt_prims(dag: Dag, requested_log2: int):
+    """(k, prims RT_PRIM[n], aabbs float32[n, 6]) of or_rt_prims"""
+    L = lib()
+    k = u32(0)
+    leaves = _p(dag.leaves)
+    n = L.or_rt_prims(_p(dag.nodes), leaves, dag.root, dag.log2_size, requested_log2, None, None, 0, C.byref(k))
+    if n < 0:
+        raise MemoryError("or_rt_prims")
+    prims = np.zeros(max(n, 1), RT_PRIM)
+    aabbs = np.zeros((max(n, 1), 6), np.float32)
+    L.or_rt_prims(_p(dag.nodes), leaves, dag.root, dag.log2_size, requested_log2, _p(prims), _p(aabbs.ravel()), n,
+                  C.byref(k))
+    return k.value, prims[:n], aabbs[:n]

@@ -514,3 +514,99 @@ def test_render_depth_closed_form():
     assert np.all(depth[~hit] == np.float32(3.402823466e38))
     # hits equal the plain trace of the camera rays
     assert np.array_equal(hits.view(np.uint32), O.trace_rays(s, O.camera_rays(cam)).view(np.uint32))
+
+
+def _morton(x, y, z):
+    return O.lib().or_morton(int(x), int(y), int(z))
+
+
+def _brute_prims(pts: np.ndarray, log2_size: int, k: int):
+    """occupied cubes of edge 2^k in Morton order: (cell, lo, hi, attr_base,
+    voxel count), from the voxel set alone"""
+    cells = {}
+    for p in pts:
+        x, y, z = int(p["x"]), int(p["y"]), int(p["z"])
+        c = (x >> k, y >> k, z >> k)
+        lo, hi, n = cells.get(c, ((1 << 30,) * 3, (0, 0, 0), 0))
+        cells[c] = (tuple(min(a, b) for a, b in zip(lo, (x, y, z))),
+                    tuple(max(a, b + 1) for a, b in zip(hi, (x, y, z))), n + 1)
+    out, rank = [], 0
+    for c in sorted(cells, key=lambda c: _morton(*c)):
+        lo, hi, n = cells[c]
+        out.append((c, lo, hi, rank, n))
+        rank += n
+    return out
+
+
+def _check_prims_brute_force(dag, pts, requested):
+    k, prims, aabbs = O.rt_prims(dag, requested)
+    assert k == min(max(requested, 3), dag.log2_size)
+    want = _brute_prims(pts, dag.log2_size, k)
+    assert len(prims) == len(want)
+    eps = np.float32(1.0 / 256.0)
+    by_cell = {}
+    for p in pts:
+        by_cell.setdefault((int(p["x"]) >> k, int(p["y"]) >> k, int(p["z"]) >> k), []).append(p)
+    L = O.lib()
+    for p, box, (c, lo, hi, rank, n) in zip(prims, aabbs, want):
+        assert int(p["cell"]) == c[0] | c[1] << 21 | c[2] << 42
+        assert int(p["attr_base"]) == rank
+        assert np.array_equal(box[:3], np.float32(lo) - eps) and np.array_equal(box[3:], np.float32(hi) + eps)
+        # the primitive's node is the subtree of exactly this cell's voxels:
+        # its count word and the lookups relative to attr_base
+        node = int(p["node"])
+        if k > 2:
+            assert int(dag.nodes[node + 1]) == n
+        sub = dag.attrs[rank:] if dag.attrs is not None else None
+        base = [v << k for v in c]
+        for v in by_cell[c]:
+            r = [int(v["x"]) - base[0], int(v["y"]) - base[1], int(v["z"]) - base[2]]
+            got = L.or_lookup(O._p(dag.nodes), O._p(dag.leaves), O._p(sub), node, k, *r)
+            assert got == int(v["attribute"])
+    return prims
+
+
+def test_rt_prims_known_answer():
+    # rt_prims.zig:139-160
+    pts = np.array([(1, 2, 3, 7), (30, 31, 17, 8), (31, 31, 17, 9)], O.VOXEL)
+    d = O.build_points(5, pts)
+    k, prims, aabbs = O.rt_prims(d, 4)
+    assert k == 4 and len(prims) == 2
+    assert abs(aabbs[0][0] - 1) < 0.01 and abs(aabbs[0][5] - 4) < 0.01
+    assert prims[0]["attr_base"] == 0 and prims[0]["cell"] == 0
+    assert prims[1]["cell"] == 1 | 1 << 21 | 1 << 42 and prims[1]["attr_base"] == 1
+    assert abs(aabbs[1][0] - 30) < 0.01 and abs(aabbs[1][3] - 32) < 0.01
+
+
+@pytest.mark.parametrize("log2,seed", [(5, 1), (6, 2), (7, 3)])
+def test_rt_prims_brute_force_random(log2, seed):
+    pts = final_points(random_points(3000, log2, seed, delete_frac=0.1), log2)
+    d = O.build_points(log2, pts)
+    for req in (0, 3, 4, log2 - 1, log2, log2 + 3):
+        _check_prims_brute_force(d, pts, req)
+
+
+def test_rt_prims_brute_force_dense_and_shared():
+    # a dense scene and a periodic pattern whose nodes are shared by many
+    # cells (the box memo per (level, node) of rt_prims.zig:61, 75-76)
+    grid = scene_grid(64)
+    d = O.build_dense(6, grid)
+    for req in (3, 4, 5):
+        _check_prims_brute_force(d, grid_to_points(grid), req)
+    n = 64
+    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    per = ((x % 8) + (y % 8) < (z % 8) + 2).astype(np.uint32)
+    d = O.build_dense(6, per)
+    prims = _check_prims_brute_force(d, grid_to_points(per), 3)
+    assert len(prims) == 512 and len(set(prims["node"].tolist())) == 1
+
+
+def test_rt_prims_empty_and_single():
+    d = O.build_points(5, np.zeros(0, O.VOXEL))
+    for req in (3, 5):
+        k, prims, _ = O.rt_prims(d, req)
+        assert len(prims) == 0
+    pts = np.array([(31, 0, 17, 5)], O.VOXEL)
+    d = O.build_points(5, pts)
+    for req in (3, 4, 5):
+        _check_prims_brute_force(d, pts, req)
