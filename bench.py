@@ -199,7 +199,6 @@ def trace_section(args, out):
     t0 = time.time()
     abi.check(L, L.pyrhip_build(log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build skin DAG")
     log(f"built {n} skin voxels in {time.time() - t0:.2f}s: {built.node_words} words, {built.leaf_count} leaves")
-    vox.free()
     dag = abi.PyrhipDag()
     dag.nodes, dag.leaves, dag.attrs = built.nodes, built.leaves, built.attrs
     dag.root, dag.log2_size, dag.default_attribute = built.root, log2, 1
@@ -239,6 +238,8 @@ def trace_section(args, out):
         "primary_plus_shadow_mrays_per_s": (nrays + hit_total * poses) / (both_ms * 1e-3) / 1e6,
         "last_frame_hit_fraction": hit_total / (W * H),
     }
+    out["traversal"]["render"] = render_leg(args, log2, vox, n, cams, W, H, hits)
+    vox.free()
     if args.cpu_trace:
         cb = cpu_trace_baseline(args, built, cams[0], W, H)
         out["traversal"]["parity_mismatches"] = cb["parity"]["mismatches"]
@@ -261,6 +262,46 @@ def trace_section(args, out):
     for b in (hits, rays, occ):
         b.free()
     return dag, w, cams
+
+
+def render_leg(args, log2, vox, n, cams, W, H, cam_hits):
+    """The same DAG through the reference's C ABI: pyr_geometry_build from the
+    resident records, one instance, pyr_commit, then pyr_render (hits + depth)
+    per pose. pyr_render returns synchronised, so the time is wall clock per
+    call. The last pose's hits must equal the camera kernel's bit for bit."""
+    from pyrit_amd import abi, lib
+    from pyrit_amd.device import Buffer
+
+    L = lib()
+    info = abi.PyrCreateInfo()
+    info.struct_size, info.version = C.sizeof(abi.PyrCreateInfo), abi.PYR_VERSION
+    info.node_pool_bytes = info.leaf_pool_bytes = 256 << 20
+    info.attribute_pool_bytes = max(256 << 20, 4 * n + (64 << 20))
+    ctx = C.c_void_p()
+    abi.check(L, L.pyr_create(C.byref(info), C.byref(ctx)), "create")
+    try:
+        geo, inst, view = C.c_void_p(), C.c_void_p(), C.c_void_p()
+        abi.check(L, L.pyr_geometry_build(ctx, log2, vox.ptr, n, 0, C.byref(geo)), "geometry build")
+        abi.check(L, L.pyr_instance_create(ctx, geo, C.byref(inst)), "instance")
+        abi.check(L, L.pyr_commit(ctx, None), "commit")
+        abi.check(L, L.pyr_view_create(ctx, C.byref(view)), "view")
+        hb, db = Buffer(W * H * 16), Buffer(W * H * 4)
+        t = abi.PyrTargets()
+        t.hits, t.depth = hb.ptr, db.ptr
+        for c in cams[:2]:
+            abi.check(L, L.pyr_render(ctx, view, C.byref(c), C.byref(t)), "render")
+        t0 = time.perf_counter()
+        for c in cams:
+            abi.check(L, L.pyr_render(ctx, view, C.byref(c), C.byref(t)), "render")
+        ms = (time.perf_counter() - t0) * 1e3 / len(cams)
+        same = bool(np.array_equal(hb.download(np.uint32, W * H * 4), cam_hits.download(np.uint32, W * H * 4)))
+        hb.free()
+        db.free()
+    finally:
+        L.pyr_destroy(ctx)
+    log(f"pyr_render {ms:.3f} ms/frame (hits + depth), hits equal to the camera kernel: {same}")
+    return {"ms_per_frame": ms, "grays_per_s": W * H / (ms * 1e-3) / 1e9, "hits_equal_camera_kernel": same,
+            "timing": "wall clock per synchronous pyr_render call (hits + depth targets)"}
 
 
 def _dag_of(built):

@@ -19,10 +19,12 @@ namespace pyr {
 
 thread_local char g_diag[512];
 
+// levels: the largest log2_size of a geometry the committed scene references
+// (the per-ray LDS stack is sized to it)
 PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t hits, uint32_t count,
-                             uint32_t ray_mask, uint32_t flags, hipStream_t s);
+                             uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s);
 PyrResult launch_render_scene(const PyrScene* scene_dev, const PyrCamera& cam, const float* w2v_dev, uint64_t hits,
-                              uint64_t depth, uint32_t ray_mask, uint32_t flags, hipStream_t s);
+                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s);
 
 namespace {
 
@@ -154,6 +156,7 @@ struct PyrContext {
     float* w2v_dev = nullptr;
     uint32_t history = 0;
     uint32_t views = 0;
+    uint32_t scene_levels = 3;  // largest geometry edge (log2) referenced by the committed instances
     uint32_t arena_users = 0;  // geometries with nodes in the path-edit arena (deferred ones included)
     std::vector<Deferred> deferred;
 };
@@ -996,9 +999,11 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
     std::vector<PyrGeometryData> geo(ctx->geos.size());
     for (size_t i = 0; i < geo.size(); i++) geo[i] = ctx->geos[i].data;
     std::vector<PyrInstanceData> ins(ctx->insts.size());
+    uint32_t levels = 3;
     for (size_t i = 0; i < ins.size(); i++) {
         ins[i] = ctx->insts[i].data;
         if (!ctx->insts[i].live) ins[i].flags = 0;
+        if (ins[i].geometry < geo.size()) levels = std::max(levels, geo[ins[i].geometry].log2_size);
     }
     ctx->time_prev = ctx->time;
     memcpy(ctx->origin_prev, ctx->origin, sizeof(ctx->origin));
@@ -1035,6 +1040,8 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
         r = hip_check(hipMemcpyAsync(ctx->scene_dev, &sc, sizeof(sc), hipMemcpyHostToDevice, s), "scene");
     // host staging above is stack memory: finish the copies before returning
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "commit");
+    // a failed commit may have replaced some tables: keep a stack that fits both scenes
+    ctx->scene_levels = r == PYR_OK ? levels : std::max(ctx->scene_levels, levels);
     // the new tables are in place and the work queued before them is done:
     // nothing reaches the deferred ranges any more
     if (r == PYR_OK) {
@@ -1054,7 +1061,7 @@ PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint3
     if (flags & (PYR_TRACE_EXTENDED | PYR_TRACE_SKIP_TRANSPARENT))
         return fail(PYR_ERROR_INVALID_ARGUMENT, "PYR_TRACE_EXTENDED / SKIP_TRANSPARENT are not supported");
     PyrResult r = launch_trace_scene(ctx->scene_dev, rays, hits, count, ray_mask ? ray_mask : 0xFFFFFFFFu, flags,
-                                     ctx->stream);
+                                     ctx->scene_levels, ctx->stream);
     if (r == PYR_OK && (ctx->flags & PYR_CREATE_DEBUG)) r = hip_check(hipStreamSynchronize(ctx->stream), "trace");
     return r;
 }
@@ -1087,7 +1094,7 @@ PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* cam
     if (r == PYR_OK)
         r = launch_render_scene(ctx->scene_dev, *camera, ctx->w2v_dev, t->hits, t->depth,
                                 t->ray_mask ? t->ray_mask : 0xFFFFFFFFu, t->flags & PYR_TRACE_NO_ATTRIBUTE,
-                                ctx->stream);
+                                ctx->scene_levels, ctx->stream);
     // w2v is host stack memory
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(ctx->stream), "render");
     return r;

@@ -1931,12 +1931,14 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
         return z.cap * 12 + z.nb * (4 + 12 + 8) + z.np2 * (20 + (uint64_t)kNodeStride * 4 + 4) + z.words * 4;
     };
     LevelSizes lz = level_sizes(nb);
-    // when the bounds would hold more than 8 bytes per record for the
-    // brick-sized buffers (sparse inputs with about one brick per record,
+    // when the bounds would hold more than 8 bytes per record plus 1 MiB for
+    // the brick-sized buffers (sparse inputs with about one brick per record,
     // batches whose chunk cells far exceed their bricks) the build reads the
-    // brick count back once after the unique step and sizes them exactly;
-    // graph captures and the static scratch keep the bounds
-    const bool exact_levels = !gb && !fixed && level_bytes(lz) > 8ull * nmax + (4ull << 20);
+    // brick count back once after the unique step and sizes them exactly, so
+    // a build never holds more than that beyond its real need; smaller
+    // builds skip the read-back; graph captures and the static scratch keep
+    // the bounds
+    const bool exact_levels = !gb && !fixed && level_bytes(lz) > 8ull * nmax + (1ull << 20);
 
     const auto host_t0 = std::chrono::steady_clock::now();
     Alloc A{{}, s};

@@ -6,7 +6,16 @@
 // the same hits and the same fp32 t bit for bit, so this file is compiled with
 // -ffp-contract=off and every float expression keeps the reference's order of
 // operations. The per-ray node stack (one word per level) lives in LDS,
-// strided by lane so the 64 lanes of a wave hit distinct banks.
+// strided by lane so the 64 lanes of a wave hit distinct banks, and is sized
+// by the launch to the DAG's edge (or the scene's largest), not to the
+// largest edge the format allows, which keeps more waves resident.
+//
+// A hit's attribute needs its DFS rank (a second walk down the stack, one
+// dependent load per earlier sibling). The walk computes it after the step
+// loop, once the whole wave has left the loop, so all lanes of a wave that
+// hit rank their voxels together; inside the loop every lane that hits in a
+// different iteration would make the wave run the rank walk again with
+// only that lane active.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -112,6 +121,7 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
     }
     stk[scale * kTraceBlock] = g.root;
     uint64_t brick = 0;
+    bool found = false;
 
     for (uint32_t step = 0; step < kMaxSteps; step++) {
         bool exists;
@@ -140,14 +150,12 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                     v[a] = ((oct >> a) & 1u) ? n - 1 - cell[a] : cell[a];
                     hit->voxel[a] = (int32_t)v[a];
                 }
-                uint32_t attribute = g.default_attribute;
-                if (want_attr && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
                 const bool mirrored = ((oct >> axis) & 1u) != 0;
                 hit->t = t;
                 hit->face = 2 * axis + (mirrored ? 0u : 1u);
-                hit->attribute = attribute;
                 hit->inside = starts_inside && !moved;
-                return true;
+                found = true;
+                break;
             }
             if (scale >= 3) stk[(scale - 1) * kTraceBlock] = child;
             else if (scale == 2) brick = g.leaves[child];
@@ -189,7 +197,15 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
             for (int a = 0; a < 3; a++) cell[a] &= m;
         }
     }
-    return false;
+    if (!found) return false;
+    uint32_t attribute = g.default_attribute;
+    if (want_attr && g.attrs) {
+        // the stack still holds the hit voxel's path and brick is its leaf
+        const uint32_t v[3] = {(uint32_t)hit->voxel[0], (uint32_t)hit->voxel[1], (uint32_t)hit->voxel[2]};
+        attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
+    }
+    hit->attribute = attribute;
+    return true;
 }
 
 __device__ __forceinline__ PyrHit to_hit(bool found, const DagHitD& h, uint32_t instance) {
@@ -226,7 +242,7 @@ __device__ __forceinline__ void xform_vector(const float* m, const float v[3], f
 }
 
 __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d[3], float tmin, float tmax,
-                              uint32_t ray_mask, uint32_t flags, uint32_t* stk) {
+                              uint32_t ray_mask, uint32_t flags, uint32_t* stk, uint32_t levels) {
     const PyrInstanceData* inst = (const PyrInstanceData*)(uintptr_t)s.instances;
     const PyrGeometryData* geo = (const PyrGeometryData*)(uintptr_t)s.geometries;
     const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
@@ -247,6 +263,7 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
         if (!(t0 <= t1)) continue;
         if (!(fmaxf(t0, tmin) < fminf(t1, best))) continue;
         const PyrGeometryData& g = geo[in.geometry];
+        if (g.log2_size > levels) continue;  // cannot happen: the stack is sized to the scene's largest edge
         DagView v;
         v.nodes = (const uint32_t*)(uintptr_t)s.nodes + g.node_offset;
         v.leaves = (const uint64_t*)(uintptr_t)s.leaves + g.leaf_offset;
@@ -272,18 +289,22 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
 }
 
 // ---- kernels -------------------------------------------------------------------
-__global__ void __launch_bounds__(kTraceBlock) k_trace_scene(const PyrScene* scene, const PyrRay* rays, PyrHit* hits,
-                                                              uint32_t count, uint32_t ray_mask, uint32_t flags) {
-    __shared__ uint32_t s_stk[kMaxLevels * kTraceBlock];
+// Every kernel takes its stack from dynamic LDS: stack_bytes(levels) per block.
+extern __shared__ uint32_t s_stk[];
+
+__host__ __device__ constexpr uint32_t stack_bytes(uint32_t levels) { return levels * kTraceBlock * 4u; }
+
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_scene(const PyrScene* scene, const PyrRay* rays, PyrHit* hits,
+                                                              uint32_t count, uint32_t ray_mask, uint32_t flags,
+                                                              uint32_t levels) {
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= count) return;
     const PyrRay r = rays[i];
-    hits[i] = trace_scene(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags, s_stk + threadIdx.x);
+    hits[i] = trace_scene(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags, s_stk + threadIdx.x, levels);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_dag(DagView g, const PyrRay* rays, PyrHit* hits,
                                                             uint32_t count, uint32_t flags) {
-    __shared__ uint32_t s_stk[kMaxLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= count) return;
     const PyrRay r = rays[i];
@@ -320,7 +341,6 @@ __device__ __forceinline__ void camera_ray(const PyrCamera& c, uint32_t x, uint3
 // 8x16 pixel tiles per workgroup keep a wave's rays coherent
 __global__ void __launch_bounds__(kTraceBlock) k_trace_camera(DagView g, PyrCamera cam, PyrHit* hits,
                                                                uint32_t flags) {
-    __shared__ uint32_t s_stk[kMaxLevels * kTraceBlock];
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 8 + (threadIdx.x >> 4);
     if (x >= cam.width || y >= cam.height) return;
     float o[3], d[3], tmin, tmax;
@@ -338,15 +358,14 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_rays(PyrCamera cam, PyrR
     rays[(uint64_t)y * cam.width + x] = r;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) k_render_scene(const PyrScene* scene, PyrCamera cam, const float* w2v,
+__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_render_scene(const PyrScene* scene, PyrCamera cam, const float* w2v,
                                                                PyrHit* hits, float* depth, uint32_t ray_mask,
-                                                               uint32_t flags) {
-    __shared__ uint32_t s_stk[kMaxLevels * kTraceBlock];
+                                                               uint32_t flags, uint32_t levels) {
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 8 + (threadIdx.x >> 4);
     if (x >= cam.width || y >= cam.height) return;
     float o[3], d[3], tmin, tmax;
     camera_ray(cam, x, y, o, d, &tmin, &tmax);
-    PyrHit h = trace_scene(*scene, o, d, tmin, tmax, ray_mask, flags, s_stk + threadIdx.x);
+    PyrHit h = trace_scene(*scene, o, d, tmin, tmax, ray_mask, flags, s_stk + threadIdx.x, levels);
     const uint64_t i = (uint64_t)y * cam.width + x;
     if (hits) hits[i] = h;
     if (depth) {
@@ -367,7 +386,6 @@ __global__ void __launch_bounds__(kTraceBlock) k_render_scene(const PyrScene* sc
 __global__ void __launch_bounds__(kTraceBlock) k_trace_shadow(DagView g, const PyrRay* rays, const PyrHit* prim,
                                                                float sx, float sy, float sz, uint32_t* out,
                                                                uint32_t count) {
-    __shared__ uint32_t s_stk[kMaxLevels * kTraceBlock];
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= count) return;
     const PyrHit h = prim[i];
@@ -403,19 +421,22 @@ DagView view_of(const PyrhipDag& d) {
 }
 
 PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t hits, uint32_t count,
-                             uint32_t ray_mask, uint32_t flags, hipStream_t s) {
+                             uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s) {
     if (count == 0) return PYR_OK;
-    hipLaunchKernelGGL(k_trace_scene, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0, s,
-                       scene_dev, (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits, count, ray_mask, flags);
+    if (levels < 3 || levels > kMaxLevels) return fail(PYR_ERROR_INVALID_ARGUMENT, "scene levels %u", levels);
+    hipLaunchKernelGGL(k_trace_scene, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock),
+                       stack_bytes(levels), s, scene_dev, (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits,
+                       count, ray_mask, flags, levels);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "trace launch: %s", hipGetErrorString(e));
 }
 
 PyrResult launch_render_scene(const PyrScene* scene_dev, const PyrCamera& cam, const float* w2v_dev, uint64_t hits,
-                              uint64_t depth, uint32_t ray_mask, uint32_t flags, hipStream_t s) {
+                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s) {
+    if (levels < 3 || levels > kMaxLevels) return fail(PYR_ERROR_INVALID_ARGUMENT, "scene levels %u", levels);
     dim3 grid((cam.width + 15) / 16, (cam.height + 7) / 8);
-    hipLaunchKernelGGL(k_render_scene, grid, dim3(kTraceBlock), 0, s, scene_dev, cam, w2v_dev,
-                       (PyrHit*)(uintptr_t)hits, (float*)(uintptr_t)depth, ray_mask, flags);
+    hipLaunchKernelGGL(k_render_scene, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev, cam, w2v_dev,
+                       (PyrHit*)(uintptr_t)hits, (float*)(uintptr_t)depth, ray_mask, flags, levels);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "render launch: %s", hipGetErrorString(e));
 }
@@ -432,8 +453,8 @@ PYR_API PyrResult pyrhip_trace_dag(const PyrhipDag* dag, uint64_t rays, uint64_t
     if (!dag || (count && (!rays || !hits))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
     if (dag->log2_size < 3 || dag->log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size");
     if (count == 0) return PYR_OK;
-    hipLaunchKernelGGL(k_trace_dag, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0,
-                       (hipStream_t)stream, view_of(*dag), (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits,
+    hipLaunchKernelGGL(k_trace_dag, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock),
+                       stack_bytes(dag->log2_size), (hipStream_t)stream, view_of(*dag), (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits,
                        count, flags);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "trace launch: %s", hipGetErrorString(e));
@@ -446,7 +467,8 @@ PYR_API PyrResult pyrhip_trace_camera(const PyrhipDag* dag, const PyrCamera* cam
     if (dag->log2_size < 3 || dag->log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size");
     if (camera->width == 0 || camera->height == 0) return PYR_OK;
     dim3 grid((camera->width + 15) / 16, (camera->height + 7) / 8);
-    hipLaunchKernelGGL(k_trace_camera, grid, dim3(kTraceBlock), 0, (hipStream_t)stream, view_of(*dag), *camera,
+    hipLaunchKernelGGL(k_trace_camera, grid, dim3(kTraceBlock), stack_bytes(dag->log2_size), (hipStream_t)stream,
+                       view_of(*dag), *camera,
                        (PyrHit*)(uintptr_t)hits, flags);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "trace launch: %s", hipGetErrorString(e));
@@ -469,8 +491,8 @@ PYR_API PyrResult pyrhip_trace_shadow(const PyrhipDag* dag, uint64_t rays, uint6
     if (!dag || !sun || (count && (!rays || !hits || !out))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
     if (dag->log2_size < 3 || dag->log2_size > 20) return fail(PYR_ERROR_INVALID_ARGUMENT, "log2_size");
     if (count == 0) return PYR_OK;
-    hipLaunchKernelGGL(k_trace_shadow, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock), 0,
-                       (hipStream_t)stream, view_of(*dag), (const PyrRay*)(uintptr_t)rays,
+    hipLaunchKernelGGL(k_trace_shadow, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock),
+                       stack_bytes(dag->log2_size), (hipStream_t)stream, view_of(*dag), (const PyrRay*)(uintptr_t)rays,
                        (const PyrHit*)(uintptr_t)hits, sun[0], sun[1], sun[2], (uint32_t*)(uintptr_t)out, count);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "trace launch: %s", hipGetErrorString(e));
