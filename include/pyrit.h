@@ -303,9 +303,60 @@ PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view);
 /* primary hits and depth of every pixel */
 PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* targets);
 
-/* arbitrary rays: rays and hits are device pointers (PyrRay[count], PyrHit[count]) */
+/* arbitrary rays: rays and hits are device pointers (PyrRay[count], PyrHit[count]);
+ * flags PYR_TRACE_ANY_HIT | PYR_TRACE_NO_ATTRIBUTE | PYR_TRACE_SKIP_TRANSPARENT */
 PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint32_t count, uint32_t ray_mask,
                             uint32_t flags);
+
+/* ---- materials (reference pyrit.h:162-228, 556-573) -----------------------
+ * A voxel attribute carries its material index in bits 0..7 and an sRGB
+ * colour 0xRRGGBB in bits 8..31. This backend shades nothing; of a material
+ * it uses the flags the traversal reads: with PYR_TRACE_SKIP_TRANSPARENT rays
+ * pass voxels of PYR_MATERIAL_TRANSPARENT materials, and every ray passes the
+ * holes of PYR_MATERIAL_CUTOUT materials (a fixed pattern, about 30 % of each
+ * voxel face) on instances whose voxel edge is at most about 1.4 world
+ * units. Both sets take effect at the next pyr_commit. The whole table is
+ * kept on the device for the caller's own kernels (PyrScene.materials). */
+#define PYR_MAX_MATERIALS 256u
+#define PYR_MATERIAL_VOXEL_COLOR 0x1u
+#define PYR_MATERIAL_REFRACT 0x2u
+#define PYR_MATERIAL_TRANSPARENT 0x4u
+#define PYR_MATERIAL_WAVES 0x8u
+#define PYR_MATERIAL_CUTOUT 0x10u
+#define PYR_MAX_TRANSPARENT_LAYERS 4u
+#define PYR_VOXEL(material, r, g, b) \
+    ((((uint32_t)(r) & 0xFFu) << 24) | (((uint32_t)(g) & 0xFFu) << 16) | (((uint32_t)(b) & 0xFFu) << 8) | \
+     ((uint32_t)(material) & 0xFFu))
+
+typedef struct PyrMaterial {
+    float base_color[3];
+    float roughness;
+    float emission[3];
+    float metallic;
+    uint32_t flags; /* PYR_MATERIAL_* */
+    float opacity;
+    float ior;
+    float density;
+    float wave_height, wave_length, wave_speed;
+    float clearcoat;
+    float clearcoat_roughness;
+    float subsurface;
+    float subsurface_color[3];
+    uint32_t texture;
+    uint32_t normal_texture;
+    float texture_scale;
+    float normal_strength;
+    float normal_scale;
+    uint32_t side_texture;
+    float side_color[3];
+    float variation[3];
+    float variation_scale[2];
+    float variation_warm;
+} PyrMaterial;
+
+PYR_API void pyr_material_default(PyrMaterial* material);
+PYR_API PyrResult pyr_material_set(PyrContext* ctx, uint32_t index, const PyrMaterial* material);
+PYR_API uint32_t pyr_voxel_attribute(uint32_t material, uint32_t r, uint32_t g, uint32_t b);
 
 /* ---- camera helpers (reference pyrit.h:917-921) -------------------------- */
 

@@ -831,9 +831,37 @@ static uint32_t attribute_rank(const OrDagView* g, const uint32_t* stk, const ui
     return rank;
 }
 
-/* walk with exit_mode = false and no skip/cut masks (dag.zig:160-351) */
-int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
-            OrDagHit* hit, uint32_t* steps_out) {
+/* skipped (dag.zig:153-157): bit (attribute & 0xFF) of a 256-bit material set */
+static int skipped(const uint64_t* set, uint32_t attribute) {
+    if (!set) return 0;
+    const uint32_t idx = attribute & 0xFF;
+    return (int)((set[idx >> 6] >> (idx & 63)) & 1);
+}
+
+/* cutHole (dag.zig:133-151): is the ray's entry point p on face `axis` of the
+ * voxel in a hole of the material's fixed 3x3 pattern? */
+static int cut_hole(const uint64_t* cut, const int32_t base[3], uint32_t attribute, const int32_t voxel[3],
+                    const float p[3], uint32_t axis) {
+    if (!skipped(cut, attribute)) return 0;
+    uint32_t sub[2] = {0, 0}, k = 0;
+    for (uint32_t a = 0; a < 3; a++) {
+        if (a == axis) continue;
+        const float f = fminf(fmaxf(p[a] - (float)voxel[a], 0.0f), 0.999f);
+        sub[k++] = (uint32_t)(f * 3.0f);
+    }
+    uint32_t h = (uint32_t)(voxel[0] + base[0]) * 0x8da6b343u + (uint32_t)(voxel[1] + base[1]) * 0xd8163841u +
+                 (uint32_t)(voxel[2] + base[2]) * 0xcb1ab31fu + (attribute >> 8) * 0x2545f491u + axis * 0x9e3779b9u +
+                 (sub[0] * 3 + sub[1]) * 0x632be5abu;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    return (h & 0xFF) < 77;
+}
+
+/* walk with exit_mode = false (dag.zig:160-351): skip / cut may be NULL */
+static int walk_impl(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
+                     const uint64_t* skip, const uint64_t* cut, const int32_t base[3], OrDagHit* hit,
+                     uint32_t* steps_out) {
     const uint32_t n = 1u << g->log2_size;
     const float nf = (float)n;
     uint32_t oct = 0;
@@ -898,10 +926,43 @@ int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, 
                     hit->voxel[a] = (int32_t)v[a];
                 }
                 uint32_t attribute = g->default_attribute;
-                if (want_attr && g->attrs) {
+                if ((want_attr || skip || cut) && g->attrs) {
                     const uint32_t r = attribute_rank(g, stk, v, brick);
                     touch(&g->attrs[g->attr_base + r], 2);
                     attribute = g->attrs[g->attr_base + r];
+                }
+                /* transparent material, or a hole of a cutout material: go on
+                 * as if the voxel were empty (a ray that starts inside a
+                 * cutout voxel passes it) */
+                int through = 0;
+                if (starts_inside && !moved) {
+                    through = skipped(cut, attribute);
+                } else if (cut) {
+                    float p[3];
+                    for (int a = 0; a < 3; a++) p[a] = o[a] + d[a] * t;
+                    through = cut_hole(cut, base, attribute, hit->voxel, p, axis);
+                }
+                if (skipped(skip, attribute) || through) {
+                    scale = 0;
+                    float tnext0 = ((float)(cell[0] + 1) - od[0]) * inv[0];
+                    axis = 0;
+                    for (int a = 1; a < 3; a++) {
+                        float ta = ((float)(cell[a] + 1) - od[a]) * inv[a];
+                        if (ta < tnext0) { tnext0 = ta; axis = (uint32_t)a; }
+                    }
+                    if (!(tnext0 < tend)) return 0;
+                    t = fmaxf(t, tnext0);
+                    moved = 1;
+                    uint32_t old0 = cell[axis], next0 = old0 + 1;
+                    if (next0 >= n) return 0;
+                    cell[axis] = next0;
+                    uint32_t h0 = 31 - (uint32_t)__builtin_clz(old0 ^ next0);
+                    if (h0 > scale) {
+                        scale = h0;
+                        uint32_t m0 = ~((1u << scale) - 1);
+                        for (int a = 0; a < 3; a++) cell[a] &= m0;
+                    }
+                    continue;
                 }
                 int mirrored = ((oct >> axis) & 1) != 0;
                 hit->t = t;
@@ -946,6 +1007,29 @@ int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, 
         }
     }
     return 0;
+}
+
+int or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
+            OrDagHit* hit, uint32_t* steps_out) {
+    const int32_t base[3] = {0, 0, 0};
+    return walk_impl(g, o, d, tmin, tmax, want_attr, NULL, NULL, base, hit, steps_out);
+}
+
+/* traceSkipping (dag.zig:113-115) */
+int or_walk_skipping(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
+                     const uint64_t skip[4], const uint64_t cut[4], const int32_t base[3], OrDagHit* hit) {
+    return walk_impl(g, o, d, tmin, tmax, want_attr, skip, cut, base, hit, NULL);
+}
+
+/* anyCutout / cutoutHere (trace.zig:161-173): cutouts only where a voxel
+ * edge is at most about 1.4 world units (row 0 of world_to_object) */
+static int any_cutout(const OrScene* s) {
+    return (s->cutout[0] | s->cutout[1] | s->cutout[2] | s->cutout[3]) != 0;
+}
+static int cutout_here(const OrInstance* in) {
+    float w2o = 0.0f;
+    for (int a = 0; a < 3; a++) w2o += in->world_to_object[a] * in->world_to_object[a];
+    return w2o > 0.5f;
 }
 
 /* safeRcp (trace.zig:56-58) */
@@ -996,7 +1080,13 @@ static void trace_one(const OrScene* s, const float o[3], const float d[3], floa
         xform_point(in->world_to_object, o, oo);
         xform_vector(in->world_to_object, d, dv);
         OrDagHit h;
-        if (or_walk(&v, oo, dv, tmin, best, want_attr, &h, NULL)) {
+        /* trace.zig:80, 93-98 */
+        const uint64_t* skip = (flags & 0x8u) ? s->transparent : NULL;
+        const uint64_t* cut = any_cutout(s) && cutout_here(in) ? s->cutout : NULL;
+        const int32_t base[3] = {0, 0, 0};
+        const int found = skip || cut ? walk_impl(&v, oo, dv, tmin, best, want_attr, skip, cut, base, &h, NULL)
+                                      : or_walk(&v, oo, dv, tmin, best, want_attr, &h, NULL);
+        if (found) {
             best = h.t;
             out->t = h.t; out->instance = i; out->attribute = h.attribute;
             out->meta = h.face | (h.inside ? 0x20u : 0u);

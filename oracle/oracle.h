@@ -19,6 +19,14 @@
  *                         lookup/attribute equalities of cpu_test.zig:758-850,
  *                         chunk batch == per-chunk single (cpu_test.zig:1001-1060)
  *   or_canon            : canon(or_build_points(P)) == or_build_points(P)
+ *   or_walk_skipping    : skipping a material == the same walk on the DAG
+ *                         built without that material's voxels (hits bit for
+ *                         bit) and == the f64 DDA with those voxels empty;
+ *                         empty sets == or_walk; cutouts: a cutout instance
+ *                         scaled past the voxel-edge limit == no cutout, holes
+ *                         on about 30 % of entry points (77/256); the hole
+ *                         hash constants themselves are parity unpinned (the
+ *                         reference has no test of them)
  *   or_walk / scene     : axis-ray KATs (cpu_test.zig:15-40), f64 DDA on a dense
  *                         grid (tests/common.zig:104-153, cpu_test.zig:42-103)
  *   or_camera_*         : closed forms (centre pixel looks at target; unit dirs)
@@ -116,6 +124,14 @@ typedef struct OrDagView {
 /* 1 = hit (dag.zig:160-351, exit_mode = false, no skip/cut) */
 int  or_walk(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
              OrDagHit* hit, uint32_t* steps_out);
+/* traceSkipping (dag.zig:113-115, 133-157, 236-289): voxels whose material
+ * (attribute & 0xFF) is in `skip` are passed; voxels whose material is in
+ * `cut` are passed where the entry point falls in a hole of the material's
+ * fixed pattern (about 30 % of each face), or when the ray starts inside
+ * them. Either set may be NULL. With a set given, hits carry the attribute
+ * even when want_attr is 0. */
+int  or_walk_skipping(const OrDagView* g, const float o[3], const float d[3], float tmin, float tmax, int want_attr,
+                      const uint64_t skip[4], const uint64_t cut[4], const int32_t base[3], OrDagHit* hit);
 
 /* Layouts identical to PyrRay / PyrHit / PyrGeometryData / PyrInstanceData */
 typedef struct OrRay { float origin[3]; float tmin; float direction[3]; float tmax; } OrRay;
@@ -131,8 +147,11 @@ typedef struct OrInstance {
 typedef struct OrScene {
     const uint32_t* nodes; const uint64_t* leaves; const uint32_t* attrs;
     const OrGeometry* geometries; const OrInstance* instances; uint32_t instance_count;
+    uint64_t transparent[4], cutout[4]; /* material sets (PyrScene transparent/cutout_materials) */
 } OrScene;
-/* pyr_trace semantics over a ray array (trace.zig:72-123) */
+/* pyr_trace semantics over a ray array (trace.zig:72-123); flags 0x8 =
+ * PYR_TRACE_SKIP_TRANSPARENT; cutout materials apply on instances whose voxel
+ * edge is at most about 1.4 world units */
 void or_trace_rays(const OrScene* s, const OrRay* rays, OrHit* hits, uint64_t count, uint32_t ray_mask,
                    uint32_t flags, uint32_t threads);
 

@@ -30,6 +30,7 @@ PYR_NO_HIT = 0xFFFFFFFF
 PYR_HIT_INSIDE = 0x20
 PYR_TRACE_ANY_HIT = 0x1
 PYR_TRACE_NO_ATTRIBUTE = 0x2
+PYR_TRACE_SKIP_TRANSPARENT = 0x8
 PYR_BUILD_HOST_INPUT = 0x1
 PYR_BUILD_EDITABLE = 0x2
 PYR_GEOMETRY_HAS_ATTRIBUTES = 0x1
@@ -72,6 +73,21 @@ class PyrCreateInfo(C.Structure):
 
 class PyrFrameInfo(C.Structure):
     _fields_ = [("time", f64), ("origin", f64 * 3)]
+
+
+class PyrMaterial(C.Structure):
+    _fields_ = [("base_color", f32 * 3), ("roughness", f32), ("emission", f32 * 3), ("metallic", f32),
+                ("flags", u32), ("opacity", f32), ("ior", f32), ("density", f32), ("wave_height", f32),
+                ("wave_length", f32), ("wave_speed", f32), ("clearcoat", f32), ("clearcoat_roughness", f32),
+                ("subsurface", f32), ("subsurface_color", f32 * 3), ("texture", u32), ("normal_texture", u32),
+                ("texture_scale", f32), ("normal_strength", f32), ("normal_scale", f32), ("side_texture", u32),
+                ("side_color", f32 * 3), ("variation", f32 * 3), ("variation_scale", f32 * 2), ("variation_warm", f32)]
+
+
+PYR_MAX_MATERIALS = 256
+PYR_MATERIAL_VOXEL_COLOR = 0x1
+PYR_MATERIAL_TRANSPARENT = 0x4
+PYR_MATERIAL_CUTOUT = 0x10
 
 
 class PyrTargets(C.Structure):
@@ -226,6 +242,9 @@ FUNCTIONS = {
     "pyr_view_destroy": (i32, [vp, vp]),
     "pyr_render": (i32, [vp, vp, P(PyrCamera), P(PyrTargets)]),
     "pyr_trace": (i32, [vp, u64, u64, u32, u32, u32]),
+    "pyr_material_default": (None, [P(PyrMaterial)]),
+    "pyr_material_set": (i32, [vp, u32, P(PyrMaterial)]),
+    "pyr_voxel_attribute": (u32, [u32, u32, u32, u32]),
     "pyr_camera_look_at": (None, [P(PyrCamera), P(f32), P(f32), P(f32)]),
     "pyr_camera_perspective": (None, [P(PyrCamera), f32, u32, u32, f32]),
     "pyrhip_build": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),

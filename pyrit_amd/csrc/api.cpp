@@ -20,11 +20,14 @@ namespace pyr {
 thread_local char g_diag[512];
 
 // levels: the largest log2_size of a geometry the committed scene references
-// (the per-ray LDS stack is sized to it)
+// (the per-ray LDS stack is sized to it); skipping: a material set may apply
+// to these rays (the skip flag with transparent materials, or cutout
+// materials), which selects the walk that checks them
 PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t hits, uint32_t count,
-                             uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s);
+                             uint32_t ray_mask, uint32_t flags, uint32_t levels, bool skipping, hipStream_t s);
 PyrResult launch_render_scene(const PyrScene* scene_dev, const PyrCamera& cam, const float* w2v_dev, uint64_t hits,
-                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s);
+                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, bool skipping,
+                              hipStream_t s);
 
 namespace {
 
@@ -157,6 +160,11 @@ struct PyrContext {
     uint32_t history = 0;
     uint32_t views = 0;
     uint32_t scene_levels = 3;  // largest geometry edge (log2) referenced by the committed instances
+    // material table (context.zig:1948-1964): host copy for the transparent /
+    // cutout sets of the next commit, device copy for the caller's kernels
+    std::vector<PyrMaterial> materials;
+    PyrMaterial* materials_dev = nullptr;
+    bool scene_transparent = false, scene_cutout = false;  // non-empty sets in the committed scene
     uint32_t arena_users = 0;  // geometries with nodes in the path-edit arena (deferred ones included)
     std::vector<Deferred> deferred;
 };
@@ -641,6 +649,16 @@ PYR_API PyrResult pyr_create(const PyrCreateInfo* info, PyrContext** out) {
         r = hip_check(hipMalloc(&ctx->inst_dev[q], (uint64_t)ctx->max_instances * sizeof(PyrInstanceData)), "tables");
     if (r == PYR_OK) r = hip_check(hipMalloc(&ctx->scene_dev, sizeof(PyrScene)), "scene");
     if (r == PYR_OK) r = hip_check(hipMalloc(&ctx->w2v_dev, 64), "camera");
+    if (r == PYR_OK) {
+        PyrMaterial m;
+        pyr_material_default(&m);
+        ctx->materials.assign(PYR_MAX_MATERIALS, m);
+        r = hip_check(hipMalloc(&ctx->materials_dev, PYR_MAX_MATERIALS * sizeof(PyrMaterial)), "materials");
+    }
+    if (r == PYR_OK)
+        r = hip_check(hipMemcpyAsync(ctx->materials_dev, ctx->materials.data(), PYR_MAX_MATERIALS * sizeof(PyrMaterial),
+                                     hipMemcpyHostToDevice, ctx->stream),
+                      "materials");
     if (r == PYR_OK) r = hip_check(hipMemsetAsync(ctx->scene_dev, 0, sizeof(PyrScene), ctx->stream), "scene");
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(ctx->stream), "create");
     if (r != PYR_OK) {
@@ -675,6 +693,7 @@ PYR_API void pyr_destroy(PyrContext* ctx) {
     (void)hipFree(ctx->inst_dev[1]);
     (void)hipFree(ctx->scene_dev);
     (void)hipFree(ctx->w2v_dev);
+    (void)hipFree(ctx->materials_dev);
     if (ctx->own_stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
 }
@@ -1027,6 +1046,12 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
     sc.time_prev = ctx->time_prev;
     memcpy(sc.origin, ctx->origin, sizeof(sc.origin));
     memcpy(sc.origin_prev, ctx->origin_prev, sizeof(sc.origin_prev));
+    // material sets of the traversal (context.zig:766-767, 1948-1958)
+    sc.materials = (uint64_t)(uintptr_t)ctx->materials_dev;
+    for (uint32_t i = 0; i < (uint32_t)ctx->materials.size(); i++) {
+        if (ctx->materials[i].flags & PYR_MATERIAL_TRANSPARENT) sc.transparent_materials[i >> 6] |= 1ull << (i & 63);
+        if (ctx->materials[i].flags & PYR_MATERIAL_CUTOUT) sc.cutout_materials[i >> 6] |= 1ull << (i & 63);
+    }
     PyrResult r = PYR_OK;
     if (!geo.empty())
         r = hip_check(hipMemcpyAsync(ctx->geo_dev, geo.data(), geo.size() * sizeof(PyrGeometryData),
@@ -1042,6 +1067,12 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(s), "commit");
     // a failed commit may have replaced some tables: keep a stack that fits both scenes
     ctx->scene_levels = r == PYR_OK ? levels : std::max(ctx->scene_levels, levels);
+    const bool tr = (sc.transparent_materials[0] | sc.transparent_materials[1] | sc.transparent_materials[2] |
+                     sc.transparent_materials[3]) != 0;
+    const bool cu = (sc.cutout_materials[0] | sc.cutout_materials[1] | sc.cutout_materials[2] |
+                     sc.cutout_materials[3]) != 0;
+    ctx->scene_transparent = r == PYR_OK ? tr : (ctx->scene_transparent || tr);
+    ctx->scene_cutout = r == PYR_OK ? cu : (ctx->scene_cutout || cu);
     // the new tables are in place and the work queued before them is done:
     // nothing reaches the deferred ranges any more
     if (r == PYR_OK) {
@@ -1058,10 +1089,10 @@ PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint3
                             uint32_t flags) {
     diag_clear();
     if (!ctx || (count && (!rays || !hits))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
-    if (flags & (PYR_TRACE_EXTENDED | PYR_TRACE_SKIP_TRANSPARENT))
-        return fail(PYR_ERROR_INVALID_ARGUMENT, "PYR_TRACE_EXTENDED / SKIP_TRANSPARENT are not supported");
+    if (flags & PYR_TRACE_EXTENDED) return fail(PYR_ERROR_INVALID_ARGUMENT, "PYR_TRACE_EXTENDED is not supported");
+    const bool skipping = ((flags & PYR_TRACE_SKIP_TRANSPARENT) && ctx->scene_transparent) || ctx->scene_cutout;
     PyrResult r = launch_trace_scene(ctx->scene_dev, rays, hits, count, ray_mask ? ray_mask : 0xFFFFFFFFu, flags,
-                                     ctx->scene_levels, ctx->stream);
+                                     ctx->scene_levels, skipping, ctx->stream);
     if (r == PYR_OK && (ctx->flags & PYR_CREATE_DEBUG)) r = hip_check(hipStreamSynchronize(ctx->stream), "trace");
     return r;
 }
@@ -1079,6 +1110,41 @@ PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view) {
     return PYR_OK;
 }
 
+// materials (root.zig:363-370, context.zig:1902-1926, 1960-1964)
+PYR_API void pyr_material_default(PyrMaterial* m) {
+    if (!m) return;
+    memset(m, 0, sizeof(*m));
+    for (int a = 0; a < 3; a++) {
+        m->base_color[a] = 1.0f;
+        m->subsurface_color[a] = 1.0f;
+    }
+    m->roughness = 0.8f;
+    m->flags = PYR_MATERIAL_VOXEL_COLOR;
+    m->ior = 1.5f;
+    m->wave_height = 0.15f;
+    m->wave_length = 12.0f;
+    m->wave_speed = 0.35f;
+    m->clearcoat_roughness = 0.1f;
+    m->texture_scale = 1.0f;
+    m->normal_scale = 1.0f;
+}
+
+PYR_API PyrResult pyr_material_set(PyrContext* ctx, uint32_t index, const PyrMaterial* m) {
+    diag_clear();
+    if (!ctx || !m) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    if (index >= PYR_MAX_MATERIALS)
+        return fail(PYR_ERROR_INVALID_ARGUMENT, "material index %u >= %u", index, PYR_MAX_MATERIALS);
+    ctx->materials[index] = *m;
+    // from the host copy, which stays put: the copy may finish after the return
+    return hip_check(hipMemcpyAsync(ctx->materials_dev + index, &ctx->materials[index], sizeof(PyrMaterial),
+                                    hipMemcpyHostToDevice, ctx->stream),
+                     "material upload");
+}
+
+PYR_API uint32_t pyr_voxel_attribute(uint32_t material, uint32_t r, uint32_t g, uint32_t b) {
+    return PYR_VOXEL(material, r, g, b);
+}
+
 // render (context.zig:1766-1867), primary hits and linear depth only
 PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* t) {
     diag_clear();
@@ -1094,7 +1160,7 @@ PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* cam
     if (r == PYR_OK)
         r = launch_render_scene(ctx->scene_dev, *camera, ctx->w2v_dev, t->hits, t->depth,
                                 t->ray_mask ? t->ray_mask : 0xFFFFFFFFu, t->flags & PYR_TRACE_NO_ATTRIBUTE,
-                                ctx->scene_levels, ctx->stream);
+                                ctx->scene_levels, ctx->scene_cutout, ctx->stream);
     // w2v is host stack memory
     if (r == PYR_OK) r = hip_check(hipStreamSynchronize(ctx->stream), "render");
     return r;

@@ -75,10 +75,44 @@ __device__ uint32_t attribute_rank(const DagView& g, const uint32_t* stk, const 
     return rank + (uint32_t)__popcll(brick & ((1ull << bit) - 1));
 }
 
-// walk (dag.zig:160-351) without skip/cut masks; stk points at this lane's
-// column of the block's LDS stack
+// skipped (dag.zig:153-157): bit (attribute & 0xFF) of a 256-bit material set
+__device__ __forceinline__ bool skipped(const uint64_t* set, uint32_t attribute) {
+    if (!set) return false;
+    const uint32_t idx = attribute & 0xFFu;
+    return ((set[idx >> 6] >> (idx & 63u)) & 1ull) != 0;
+}
+
+// cutHole (dag.zig:133-151) for a DAG at the origin of its object space: is
+// the entry point p on face `axis` of the voxel in a hole of the material's
+// fixed 3 x 3 pattern (about 30 % of the face)?
+__device__ __forceinline__ bool cut_hole(const uint64_t* cut, uint32_t attribute, const int32_t voxel[3],
+                                         const float p[3], uint32_t axis) {
+    if (!skipped(cut, attribute)) return false;
+    uint32_t sub[2] = {0, 0}, k = 0;
+#pragma unroll
+    for (uint32_t a = 0; a < 3; a++) {
+        if (a == axis) continue;
+        const float f = fminf(fmaxf(p[a] - (float)voxel[a], 0.0f), 0.999f);
+        sub[k++] = (uint32_t)(f * 3.0f);
+    }
+    uint32_t h = (uint32_t)voxel[0] * 0x8da6b343u + (uint32_t)voxel[1] * 0xd8163841u + (uint32_t)voxel[2] * 0xcb1ab31fu +
+                 (attribute >> 8) * 0x2545f491u + axis * 0x9e3779b9u + (sub[0] * 3 + sub[1]) * 0x632be5abu;
+    h ^= h >> 15;
+    h *= 0x2c1b3c6du;
+    h ^= h >> 12;
+    return (h & 0xFFu) < 77u;
+}
+
+// walk (dag.zig:160-351); stk points at this lane's column of the block's
+// LDS stack. SKIPPING (traceSkipping, dag.zig:113-115, 236-289): voxels of a
+// material in `skip` are passed, and voxels of a material in `cut` where the
+// entry point lies in a hole of the material's pattern or the ray starts
+// inside them; such a walk needs each candidate voxel's attribute inside the
+// loop, so it ranks there, and its hits carry the attribute even when
+// want_attr is false (as the reference's do).
+template <bool SKIPPING>
 __device__ bool walk(const DagView& g, const float o[3], const float d[3], float tmin, float tmax, bool want_attr,
-                     uint32_t* stk, DagHitD* hit) {
+                     uint32_t* stk, DagHitD* hit, const uint64_t* skip = nullptr, const uint64_t* cut = nullptr) {
     const uint32_t n = 1u << g.log2_size;
     const float nf = (float)n;
     uint32_t oct = 0;
@@ -150,6 +184,51 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                     v[a] = ((oct >> a) & 1u) ? n - 1 - cell[a] : cell[a];
                     hit->voxel[a] = (int32_t)v[a];
                 }
+                if constexpr (SKIPPING) {
+                    uint32_t attribute = g.default_attribute;
+                    if ((want_attr || skip || cut) && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
+                    bool through = false;
+                    if (starts_inside && !moved) {
+                        through = skipped(cut, attribute);
+                    } else if (cut) {
+                        float p[3];
+#pragma unroll
+                        for (int a = 0; a < 3; a++) p[a] = o[a] + d[a] * t;
+                        through = cut_hole(cut, attribute, hit->voxel, p, axis);
+                    }
+                    if (skipped(skip, attribute) || through) {
+                        // as if the voxel were empty: on to the next cell of edge 1
+                        scale = 0;
+                        float tnext0 = ((float)(cell[0] + 1) - od[0]) * inv[0];
+                        axis = 0;
+#pragma unroll
+                        for (int a = 1; a < 3; a++) {
+                            const float ta = ((float)(cell[a] + 1) - od[a]) * inv[a];
+                            if (ta < tnext0) {
+                                tnext0 = ta;
+                                axis = (uint32_t)a;
+                            }
+                        }
+                        if (!(tnext0 < tend)) return false;
+                        t = fmaxf(t, tnext0);
+                        moved = true;
+                        const uint32_t old0 = axis == 0 ? cell[0] : axis == 1 ? cell[1] : cell[2];
+                        const uint32_t next0 = old0 + 1;
+                        if (next0 >= n) return false;
+                        if (axis == 0) cell[0] = next0;
+                        else if (axis == 1) cell[1] = next0;
+                        else cell[2] = next0;
+                        const uint32_t h0 = 31 - (uint32_t)__clz(old0 ^ next0);
+                        if (h0 > scale) {
+                            scale = h0;
+                            const uint32_t m0 = ~((1u << scale) - 1);
+#pragma unroll
+                            for (int a = 0; a < 3; a++) cell[a] &= m0;
+                        }
+                        continue;
+                    }
+                    hit->attribute = attribute;
+                }
                 const bool mirrored = ((oct >> axis) & 1u) != 0;
                 hit->t = t;
                 hit->face = 2 * axis + (mirrored ? 0u : 1u);
@@ -198,13 +277,15 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
         }
     }
     if (!found) return false;
-    uint32_t attribute = g.default_attribute;
-    if (want_attr && g.attrs) {
-        // the stack still holds the hit voxel's path and brick is its leaf
-        const uint32_t v[3] = {(uint32_t)hit->voxel[0], (uint32_t)hit->voxel[1], (uint32_t)hit->voxel[2]};
-        attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
+    if constexpr (!SKIPPING) {
+        uint32_t attribute = g.default_attribute;
+        if (want_attr && g.attrs) {
+            // the stack still holds the hit voxel's path and brick is its leaf
+            const uint32_t v[3] = {(uint32_t)hit->voxel[0], (uint32_t)hit->voxel[1], (uint32_t)hit->voxel[2]};
+            attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
+        }
+        hit->attribute = attribute;
     }
-    hit->attribute = attribute;
     return true;
 }
 
@@ -241,6 +322,22 @@ __device__ __forceinline__ void xform_vector(const float* m, const float v[3], f
     r[2] = m[8] * v[0] + m[9] * v[1] + m[10] * v[2];
 }
 
+// anyCutout / cutoutHere (trace.zig:161-173): cutouts only where a voxel
+// edge is at most about 1.4 world units (row 0 of world_to_object)
+__device__ __forceinline__ bool any_cutout(const PyrScene& s) {
+    return (s.cutout_materials[0] | s.cutout_materials[1] | s.cutout_materials[2] | s.cutout_materials[3]) != 0;
+}
+__device__ __forceinline__ bool cutout_here(const PyrInstanceData& in) {
+    float w2o = 0.0f;
+#pragma unroll
+    for (int a = 0; a < 3; a++) w2o += in.world_to_object[a] * in.world_to_object[a];
+    return w2o > 0.5f;
+}
+
+// SKIPPING: the launch found a material set that may apply (skip flag with
+// transparent materials, or cutout materials); each variant holds one walk,
+// which keeps the registers of the plain one at its own count
+template <bool SKIPPING>
 __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d[3], float tmin, float tmax,
                               uint32_t ray_mask, uint32_t flags, uint32_t* stk, uint32_t levels) {
     const PyrInstanceData* inst = (const PyrInstanceData*)(uintptr_t)s.instances;
@@ -249,6 +346,9 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
     const bool want_attr = (flags & PYR_TRACE_NO_ATTRIBUTE) == 0;
     float best = tmax;
     PyrHit result = to_hit(false, DagHitD{}, 0);
+    // material sets of the traversal (trace.zig:80, 93-98)
+    const uint64_t* skip = SKIPPING && (flags & PYR_TRACE_SKIP_TRANSPARENT) ? s.transparent_materials : nullptr;
+    const uint64_t* cut = SKIPPING && any_cutout(s) ? s.cutout_materials : nullptr;
     for (uint32_t i = 0; i < s.instance_count; i++) {
         const PyrInstanceData& in = inst[i];
         if (!(in.flags & PYR_INSTANCE_ACTIVE) || !(in.mask & ray_mask)) continue;
@@ -279,7 +379,9 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
         xform_point(in.world_to_object, o, oo);
         xform_vector(in.world_to_object, d, dv);
         DagHitD h;
-        if (walk(v, oo, dv, tmin, best, want_attr, stk, &h)) {
+        // with both sets null the skipping walk returns what the plain one does
+        const uint64_t* icut = cut && cutout_here(in) ? cut : nullptr;
+        if (walk<SKIPPING>(v, oo, dv, tmin, best, want_attr, stk, &h, skip, icut)) {
             best = h.t;
             result = to_hit(true, h, i);
             if (flags & PYR_TRACE_ANY_HIT) return result;
@@ -294,13 +396,15 @@ extern __shared__ uint32_t s_stk[];
 
 __host__ __device__ constexpr uint32_t stack_bytes(uint32_t levels) { return levels * kTraceBlock * 4u; }
 
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_trace_scene(const PyrScene* scene, const PyrRay* rays, PyrHit* hits,
+template <bool SKIPPING>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_scene(const PyrScene* scene, const PyrRay* rays, PyrHit* hits,
                                                               uint32_t count, uint32_t ray_mask, uint32_t flags,
                                                               uint32_t levels) {
     const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
     if (i >= count) return;
     const PyrRay r = rays[i];
-    hits[i] = trace_scene(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags, s_stk + threadIdx.x, levels);
+    hits[i] = trace_scene<SKIPPING>(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags, s_stk + threadIdx.x,
+                                    levels);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_dag(DagView g, const PyrRay* rays, PyrHit* hits,
@@ -309,7 +413,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_dag(DagView g, const PyrR
     if (i >= count) return;
     const PyrRay r = rays[i];
     DagHitD h;
-    bool f = walk(g, r.origin, r.direction, r.tmin, r.tmax, (flags & PYR_TRACE_NO_ATTRIBUTE) == 0,
+    bool f = walk<false>(g, r.origin, r.direction, r.tmin, r.tmax, (flags & PYR_TRACE_NO_ATTRIBUTE) == 0,
                   s_stk + threadIdx.x, &h);
     hits[i] = to_hit(f, h, 0);
 }
@@ -346,7 +450,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_camera(DagView g, PyrCame
     float o[3], d[3], tmin, tmax;
     camera_ray(cam, x, y, o, d, &tmin, &tmax);
     DagHitD h;
-    bool f = walk(g, o, d, tmin, tmax, (flags & PYR_TRACE_NO_ATTRIBUTE) == 0, s_stk + threadIdx.x, &h);
+    bool f = walk<false>(g, o, d, tmin, tmax, (flags & PYR_TRACE_NO_ATTRIBUTE) == 0, s_stk + threadIdx.x, &h);
     hits[(uint64_t)y * cam.width + x] = to_hit(f, h, 0);
 }
 
@@ -358,14 +462,15 @@ __global__ void __launch_bounds__(kTraceBlock) k_camera_rays(PyrCamera cam, PyrR
     rays[(uint64_t)y * cam.width + x] = r;
 }
 
-__global__ void __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(7))) k_render_scene(const PyrScene* scene, PyrCamera cam, const float* w2v,
+template <bool SKIPPING>
+__global__ void __launch_bounds__(kTraceBlock) k_render_scene(const PyrScene* scene, PyrCamera cam, const float* w2v,
                                                                PyrHit* hits, float* depth, uint32_t ray_mask,
                                                                uint32_t flags, uint32_t levels) {
     const uint32_t x = blockIdx.x * 16 + (threadIdx.x & 15), y = blockIdx.y * 8 + (threadIdx.x >> 4);
     if (x >= cam.width || y >= cam.height) return;
     float o[3], d[3], tmin, tmax;
     camera_ray(cam, x, y, o, d, &tmin, &tmax);
-    PyrHit h = trace_scene(*scene, o, d, tmin, tmax, ray_mask, flags, s_stk + threadIdx.x, levels);
+    PyrHit h = trace_scene<SKIPPING>(*scene, o, d, tmin, tmax, ray_mask, flags, s_stk + threadIdx.x, levels);
     const uint64_t i = (uint64_t)y * cam.width + x;
     if (hits) hits[i] = h;
     if (depth) {
@@ -404,7 +509,7 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_shadow(DagView g, const P
         o[a] = r.origin[a] + r.direction[a] * h.t + nrm * 1e-3f;
     }
     DagHitD sh;
-    out[i] = walk(g, o, d, 0.0f, PYR_FLT_MAX, false, s_stk + threadIdx.x, &sh) ? 1u : 0u;
+    out[i] = walk<false>(g, o, d, 0.0f, PYR_FLT_MAX, false, s_stk + threadIdx.x, &sh) ? 1u : 0u;
 }
 
 DagView view_of(const PyrhipDag& d) {
@@ -421,22 +526,31 @@ DagView view_of(const PyrhipDag& d) {
 }
 
 PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t hits, uint32_t count,
-                             uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s) {
+                             uint32_t ray_mask, uint32_t flags, uint32_t levels, bool skipping, hipStream_t s) {
     if (count == 0) return PYR_OK;
     if (levels < 3 || levels > kMaxLevels) return fail(PYR_ERROR_INVALID_ARGUMENT, "scene levels %u", levels);
-    hipLaunchKernelGGL(k_trace_scene, dim3((count + kTraceBlock - 1) / kTraceBlock), dim3(kTraceBlock),
-                       stack_bytes(levels), s, scene_dev, (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits,
-                       count, ray_mask, flags, levels);
+    const dim3 grid((count + kTraceBlock - 1) / kTraceBlock);
+    if (skipping)
+        hipLaunchKernelGGL(k_trace_scene<true>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev,
+                           (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits, count, ray_mask, flags, levels);
+    else
+        hipLaunchKernelGGL(k_trace_scene<false>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev,
+                           (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits, count, ray_mask, flags, levels);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "trace launch: %s", hipGetErrorString(e));
 }
 
 PyrResult launch_render_scene(const PyrScene* scene_dev, const PyrCamera& cam, const float* w2v_dev, uint64_t hits,
-                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, hipStream_t s) {
+                              uint64_t depth, uint32_t ray_mask, uint32_t flags, uint32_t levels, bool skipping,
+                              hipStream_t s) {
     if (levels < 3 || levels > kMaxLevels) return fail(PYR_ERROR_INVALID_ARGUMENT, "scene levels %u", levels);
     dim3 grid((cam.width + 15) / 16, (cam.height + 7) / 8);
-    hipLaunchKernelGGL(k_render_scene, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev, cam, w2v_dev,
-                       (PyrHit*)(uintptr_t)hits, (float*)(uintptr_t)depth, ray_mask, flags, levels);
+    if (skipping)
+        hipLaunchKernelGGL(k_render_scene<true>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev, cam,
+                           w2v_dev, (PyrHit*)(uintptr_t)hits, (float*)(uintptr_t)depth, ray_mask, flags, levels);
+    else
+        hipLaunchKernelGGL(k_render_scene<false>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev, cam,
+                           w2v_dev, (PyrHit*)(uintptr_t)hits, (float*)(uintptr_t)depth, ray_mask, flags, levels);
     hipError_t e = hipGetLastError();
     return e == hipSuccess ? PYR_OK : fail(PYR_ERROR_CUDA, "render launch: %s", hipGetErrorString(e));
 }

@@ -53,7 +53,8 @@ class OrInstance(C.Structure):
 
 class OrScene(C.Structure):
     _fields_ = [("nodes", P(u32)), ("leaves", P(u64)), ("attrs", P(u32)), ("geometries", P(OrGeometry)),
-                ("instances", P(OrInstance)), ("instance_count", u32)]
+                ("instances", P(OrInstance)), ("instance_count", u32), ("transparent", u64 * 4),
+                ("cutout", u64 * 4)]
 
 
 class OrCamera(C.Structure):
@@ -94,6 +95,8 @@ def lib() -> C.CDLL:
         L.or_built_free.argtypes = [P(OrBuilt)]
         L.or_canon.argtypes = [C.c_void_p, C.c_void_p, u32, u32, P(OrDag)]
         L.or_walk.argtypes = [P(OrDagView), P(f32), P(f32), f32, f32, i32, P(OrDagHit), P(u32)]
+        L.or_walk_skipping.argtypes = [P(OrDagView), P(f32), P(f32), f32, f32, i32, C.c_void_p, C.c_void_p,
+                                       P(i32), P(OrDagHit)]
         L.or_trace_rays.argtypes = [P(OrScene), C.c_void_p, C.c_void_p, u64, u32, u32, u32]
         L.or_trace_touch.argtypes = [P(OrScene), C.c_void_p, u64, u32, u32, P(u64)]
         L.or_camera_look_at.argtypes = [P(OrCamera), P(f32), P(f32), P(f32)]

@@ -60,7 +60,8 @@ int main(void) {
   S(PyrRay) S(PyrHit) S(PyrCamera) S(PyrGeometryData) S(PyrInstanceData) S(PyrScene) S(PyrCreateInfo)
   S(PyrVoxel) S(PyrFrameInfo) S(PyrTargets) S(PyrhipBuilt) S(PyrhipChunkBatch) S(PyrhipDag)
   S(PyrChunkKey) S(PyrWorldGenParams) S(PyrWorldInfo) S(PyrWorldStats) S(PyrWorldEdit) S(PyrDagInfo)
-  S(PyrhipBuildTuning) S(PyrhipTuneReport)
+  S(PyrhipBuildTuning) S(PyrhipTuneReport) S(PyrMaterial)
+  O(PyrMaterial, flags) O(PyrMaterial, texture) O(PyrMaterial, side_color) O(PyrMaterial, variation_warm)
   O(PyrDagInfo, nodes) O(PyrDagInfo, leaves) O(PyrDagInfo, attributes) O(PyrDagInfo, attribute_count)
   O(PyrWorldInfo, memory_budget) O(PyrWorldInfo, y_min) O(PyrWorldInfo, flags) O(PyrWorldInfo, generate)
   O(PyrWorldInfo, user) O(PyrWorldStats, voxel_pixels) O(PyrWorldStats, bytes) O(PyrWorldGenParams, user)
@@ -81,11 +82,14 @@ int main(void) {
 REFERENCE_SIZES = {"PyrRay": 32, "PyrHit": 16, "PyrCamera": 92, "PyrGeometryData": 32, "PyrInstanceData": 144,
                    "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80,
                    "PyrChunkKey": 16, "PyrWorldGenParams": 48, "PyrWorldInfo": 88, "PyrWorldStats": 48,
-                   "PyrWorldEdit": 32, "PyrDagInfo": 64}
+                   "PyrWorldEdit": 32, "PyrDagInfo": 64, "PyrMaterial": 144}
 # field offsets the reference's world ABI fixes (reference include/pyrit.h:815-875)
 REFERENCE_OFFSETS = {"PyrWorldInfo.memory_budget": 24, "PyrWorldInfo.y_min": 32, "PyrWorldInfo.flags": 68,
                      "PyrWorldInfo.generate": 72, "PyrWorldInfo.user": 80, "PyrWorldStats.voxel_pixels": 24,
-                     "PyrWorldStats.bytes": 40, "PyrWorldGenParams.user": 40}
+                     "PyrWorldStats.bytes": 40, "PyrWorldGenParams.user": 40,
+                     # reference include/pyrit.h:190-228 (field list of PyrMaterial)
+                     "PyrMaterial.flags": 32, "PyrMaterial.texture": 84, "PyrMaterial.side_color": 108,
+                     "PyrMaterial.variation_warm": 140}
 
 
 def test_struct_layouts_match_reference_and_mirrors():
@@ -111,7 +115,8 @@ def test_struct_layouts_match_reference_and_mirrors():
                "PyrhipChunkBatch": abi.PyrhipChunkBatch, "PyrhipDag": abi.PyrhipDag, "PyrChunkKey": abi.PyrChunkKey,
                "PyrWorldGenParams": abi.PyrWorldGenParams, "PyrWorldInfo": abi.PyrWorldInfo,
                "PyrWorldStats": abi.PyrWorldStats, "PyrWorldEdit": abi.PyrWorldEdit, "PyrDagInfo": abi.PyrDagInfo,
-               "PyrhipBuildTuning": abi.PyrhipBuildTuning, "PyrhipTuneReport": abi.PyrhipTuneReport}
+               "PyrhipBuildTuning": abi.PyrhipBuildTuning, "PyrhipTuneReport": abi.PyrhipTuneReport,
+               "PyrMaterial": abi.PyrMaterial}
     for name, cls in mirrors.items():
         assert C.sizeof(cls) == vals[name], name
     for key, v in vals.items():

@@ -610,3 +610,109 @@ def test_rt_prims_empty_and_single():
     d = O.build_points(5, pts)
     for req in (3, 4, 5):
         _check_prims_brute_force(d, pts, req)
+
+
+SKIP = 0x8
+
+
+def _material_set(materials):
+    m = [0, 0, 0, 0]
+    for i in materials:
+        m[i >> 6] |= 1 << (i & 63)
+    return m
+
+
+def _probe_rays(n, count, seed):
+    """random rays through the cube plus axis-aligned ones (the DDA test's mix)"""
+    rng = np.random.default_rng(seed)
+    rays = np.zeros(count, O.RAY)
+    for i in range(count):
+        o = n * 0.5 + (rng.random(3) * 2 - 1) * n * 1.5
+        dv = n * 0.5 + (rng.random(3) * 2 - 1) * n * 0.45 - o
+        if i % 8 == 0:
+            dv[i % 3] = 0
+        rays[i]["origin"], rays[i]["direction"] = o, dv
+    rays["tmin"], rays["tmax"] = 0.0, np.float32(3.0e38)
+    return rays
+
+
+def test_skip_transparent_equals_removing_the_voxels():
+    """traceSkipping (dag.zig:236-289) with transparent materials: every hit
+    (t, instance, face, inside, attribute) equals the plain walk of the DAG
+    built without those voxels; an empty set equals the plain walk."""
+    n = 64
+    grid = scene_grid(n)
+    trans = {m for m in range(256) if m % 5 == 0}
+    mat = grid & 0xFF
+    removed = np.where(np.isin(mat, list(trans)), 0, grid).astype(np.uint32)
+    assert (grid != 0).sum() > (removed != 0).sum() > 0
+    full, rem = O.build_dense(6, grid), O.build_dense(6, removed)
+    s_full, s_rem = O.single_scene(full), O.single_scene(rem)
+    s_full.transparent[:] = _material_set(trans)
+    rays = np.concatenate([_probe_rays(n, 6000, 3),
+                           O.camera_rays(O.camera((-32, 96, -32), (32, 32, 32), (0, 1, 0), 1.0, 64, 64))])
+    # rays starting inside voxels too: origins on voxel centres
+    inside = _probe_rays(n, 2000, 4)
+    z, y, x = np.nonzero(grid)
+    pick = np.random.default_rng(5).integers(0, len(x), len(inside))
+    inside["origin"] = np.stack([x[pick], y[pick], z[pick]], 1) + 0.5
+    rays = np.concatenate([rays, inside])
+    a = O.trace_rays(s_full, rays, flags=SKIP)
+    b = O.trace_rays(s_rem, rays)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert (a["instance"] == 0).sum() > len(rays) // 4
+    plain = O.trace_rays(s_full, rays)
+    assert not np.array_equal(plain.view(np.uint32), a.view(np.uint32))
+    # without the flag the set is ignored; with an empty set the flag changes nothing
+    s_full.transparent[:] = [0, 0, 0, 0]
+    assert np.array_equal(O.trace_rays(s_full, rays, flags=SKIP).view(np.uint32), plain.view(np.uint32))
+
+
+def test_cutout_holes():
+    """Cutout materials (dag.zig:133-151, trace.zig:161-173): rays pass about
+    30 % (77/256) of the entry points on cutout voxels, a ray passing a hole
+    goes on to the same hit as with that voxel removed or further, a ray
+    starting inside a cutout voxel passes it, and an instance whose voxel edge
+    exceeds the limit traces as if there were no cutouts."""
+    n = 64
+    grid = scene_grid(n)
+    cut = {m for m in range(256) if m % 3 == 0}
+    d = O.build_dense(6, grid)
+    s = O.single_scene(d)
+    rays = np.concatenate([_probe_rays(n, 8000, 6),
+                           O.camera_rays(O.camera((-32, 96, -32), (32, 32, 32), (0, 1, 0), 1.0, 96, 96))])
+    plain = O.trace_rays(s, rays)
+    s.cutout[:] = _material_set(cut)
+    got = O.trace_rays(s, rays)
+    on_cut = (plain["instance"] == 0) & np.isin(plain["attribute"] & 0xFF, list(cut))
+    passed = on_cut & (got.view(np.uint32).reshape(-1, 4) != plain.view(np.uint32).reshape(-1, 4)).any(1)
+    frac = passed.sum() / on_cut.sum()
+    assert on_cut.sum() > 2000 and 0.25 < frac < 0.36, (on_cut.sum(), frac)
+    # rays that did not first meet a cutout voxel are unchanged
+    same = ~on_cut
+    assert np.array_equal(got[same].view(np.uint32), plain[same].view(np.uint32))
+    # a passed ray hits later (or misses)
+    later = passed & (got["instance"] == 0)
+    assert np.all(got["t"][later] >= plain["t"][later])
+    # starting inside a cutout voxel: the plain walk reports an inside hit at
+    # tmin, the cutout walk never stops in that voxel
+    z, y, x = np.nonzero(np.isin(grid & 0xFF, list(cut)) & (grid != 0))
+    start = _probe_rays(n, 500, 7)
+    start["origin"] = np.stack([x[:500], y[:500], z[:500]], 1) + 0.5
+    p0 = O.trace_rays(O.single_scene(d), start)
+    assert np.all(p0["meta"] & 0x20)
+    g0 = O.trace_rays(s, start)
+    assert not np.any(g0["meta"] & 0x20)
+    # voxel edge 2 world units: w2o row 0 = (0.5, 0, 0), squared norm 0.25 <= 0.5
+    s2 = O.single_scene(d)
+    m = [2, 0, 0, 0, 0, 2, 0, 0, 0, 0, 2, 0]
+    w = [0.5, 0, 0, 0, 0, 0.5, 0, 0, 0, 0, 0.5, 0]
+    s2._keep[1][0].object_to_world[:] = m
+    s2._keep[1][0].world_to_object[:] = w
+    O.lib().or_transformed_box(s2._keep[1][0].object_to_world, float(n), s2._keep[1][0].bounds_min,
+                               s2._keep[1][0].bounds_max)
+    big = rays.copy()
+    big["origin"] *= 2
+    a = O.trace_rays(s2, big)
+    s2.cutout[:] = _material_set(cut)
+    assert np.array_equal(O.trace_rays(s2, big).view(np.uint32), a.view(np.uint32))
