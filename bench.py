@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="smaller traversal/batch sections")
-    ap.add_argument("--only", default="", help="comma list of sections: build,trace,batch,cpu,probe")
+    ap.add_argument("--only", default="", help="comma list of sections: build,rtprims,trace,batch,serve,cpu,probe")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="build with the built-in defaults instead of the setting the autotuner picks for the "
@@ -146,19 +146,19 @@ def build_section(args, out):
                        "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
                        "traffic": measured_traffic(kname, n), "bytes_per_launch": pass_bytes,
                        "avg_launch_ms": pass_ms}
-    rt_prims_section(args, out, vox, n)
     vox.free()
     return n, ms
 
 
-def rt_prims_section(args, out, vox, n):
-    """Box primitives of the built grid (rt_prims.extract as its own
+def rt_prims_section(args, out):
+    """Box primitives of the built 1024^3 grid (rt_prims.extract as its own
     operator): wall time of the counting call plus the writing call, each of
     which ends synchronised, at the reference's default edge 2^7 and at 2^3."""
-    from pyrit_amd import abi, gpu, lib
+    from pyrit_amd import abi, gpu, lib, workloads
     from pyrit_amd.device import Buffer
 
     L = lib()
+    vox, _, n = workloads.heightfield_grid(log2_size=args.log2).generate_device()
     built = abi.PyrhipBuilt()
     abi.check(L, L.pyrhip_build(args.log2, 0, 0, 0, vox.ptr, n, None, None, C.byref(built)), "build")
     b = gpu.Built(built)
@@ -182,6 +182,7 @@ def rt_prims_section(args, out, vox, n):
         pb.free()
         ab.free()
     b.free()
+    vox.free()
     out["rt_prims"] = res
     log(f"rt prims: {res}")
 
@@ -680,7 +681,7 @@ def main() -> int:
     if not device.gpu_available():
         print(json.dumps({"error": "no HIP device visible"}), flush=True)
         return 1
-    only = set(args.only.split(",")) if args.only else {"build", "trace", "batch", "serve", "cpu", "probe"}
+    only = set(args.only.split(",")) if args.only else {"build", "rtprims", "trace", "batch", "serve", "cpu", "probe"}
     args.cpu_trace = "cpu" in only
     out = {}
     n, ms = 0, float("nan")
@@ -691,6 +692,8 @@ def main() -> int:
         n, ms = build_section(args, out)
         log(f"build {out['build']['gvoxels_per_s']:.2f} Gvox/s, sort pass {out['roofline']['achieved']:.0f} GB/s, "
             f"finish {out['build']['finish']}, phases {out['build']['phase_ms']}")
+    if "rtprims" in only:
+        rt_prims_section(args, out)
     if "batch" in only:
         batch_section(args, out)
         log(f"batched {out['batched']['batched_gvoxels_per_s']:.2f} Gvox/s, "

@@ -434,9 +434,10 @@ struct SortArgs {
 constexpr uint32_t kSortPeel = 4;
 
 // predecessor status words a sort tile's digit thread loads in the first
-// look-back round and in every later one
-constexpr int kPollFirst = 16;
-constexpr int kPoll = 16;
+// look-back round and in every later one (1024^3 heightfield, both passes:
+// 1/16 4.99 ms, 2/8 4.64, 4/16 4.94, 8/8 4.58, 16/16 4.70)
+constexpr int kPollFirst = 8;
+constexpr int kPoll = 8;
 
 // PACKED: one u64 word per record, key << 32 | attribute (keys of at most 32
 // bits); the digit is taken at a.shift >= 32 and there is no value stream.

@@ -225,19 +225,53 @@ __global__ void __launch_bounds__(kBlock) k_expand_write(DagRef g, Frontier in, 
 
 // ---- phase 2: distinct nodes per level -------------------------------------
 
+// Positions and children repeat a few nodes many times over (flat or
+// periodic regions), so a block first merges its nodes in an LDS table of
+// twice as many slots: only the first lane of each distinct node inserts it
+// into the level's table, whose hot slots then see one access per block
+// instead of one per occurrence.
+constexpr uint32_t kLdsSlots = 2 * kTile;  // a block merges at most kTile nodes
+
+__device__ __forceinline__ bool first_in_block(uint32_t* s_key, uint32_t key) {
+    uint32_t h = set_hash(key) & (kLdsSlots - 1);
+    for (uint32_t probe = 0; probe < kLdsSlots; probe++) {  // the table is never more than half full
+        const uint32_t cur = atomicCAS(&s_key[h], 0u, key);
+        if (cur == 0) return true;
+        if (cur == key) return false;
+        h = (h + 1) & (kLdsSlots - 1);
+    }
+    return true;
+}
+
 __global__ void __launch_bounds__(kBlock) k_set_items(const uint32_t* node, uint32_t n, NodeSet s, uint32_t* err) {
-    const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
-    if (j < n) set_insert(s, node[j], err);
+    __shared__ uint32_t s_key[kLdsSlots];
+    for (uint32_t i = threadIdx.x; i < kLdsSlots; i += kBlock) s_key[i] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+        const uint64_t j = (uint64_t)blockIdx.x * kTile + (uint64_t)q * kBlock + threadIdx.x;
+        if (j < n && first_in_block(s_key, node[j] + 1)) set_insert(s, node[j], err);
+    }
 }
 
 // children of the parent level's distinct nodes (all nodes: parent level > 3)
+// (128 parents per block: at most 1024 children, as many as k_set_items merges)
+constexpr uint32_t kParentsPerBlock = kTile / 8;
+
 __global__ void __launch_bounds__(kBlock) k_set_children(DagRef g, NodeSet parent, uint32_t n, NodeSet s,
                                                          uint32_t* err) {
-    const uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+    __shared__ uint32_t s_key[kLdsSlots];
+    for (uint32_t i = threadIdx.x; i < kLdsSlots; i += kBlock) s_key[i] = 0;
+    __syncthreads();
+    // two lanes per parent, four child slots each
+    const uint64_t j = (uint64_t)blockIdx.x * kParentsPerBlock + threadIdx.x / 2;
     if (j >= n) return;
     const uint32_t node = parent.list[j];
     const uint32_t c = child_count(g, node, err);
-    for (uint32_t q = 0; q < c; q++) set_insert(s, g.nodes[node + 2 + q], err);
+    for (uint32_t q = (threadIdx.x & 1) * 4; q < min(c, (threadIdx.x & 1) * 4 + 4); q++) {
+        const uint32_t child = g.nodes[node + 2 + q];
+        if (first_in_block(s_key, child + 1)) set_insert(s, child, err);
+    }
 }
 
 // ---- phase 3: boxes bottom-up ----------------------------------------------
@@ -495,7 +529,7 @@ PYR_API PyrResult pyrhip_rt_prims(const PyrhipDag* dag, uint32_t node_words, uin
     };
     const uint64_t node_bound = node_words / 3 + 1;  // every node holds at least 3 words
     if (!make_set(std::min<uint64_t>(n, node_bound), &sets[k])) return fail(PYR_ERROR_OUT_OF_MEMORY, "rt prims set");
-    hipLaunchKernelGGL(k_set_items, dim3(blocks_of(n)), dim3(kBlock), 0, s, cur.node, (uint32_t)n, sets[k], err);
+    hipLaunchKernelGGL(k_set_items, dim3(tiles_of(n)), dim3(kBlock), 0, s, cur.node, (uint32_t)n, sets[k], err);
     for (uint32_t level = k;; level--) {
         uint32_t c = 0;
         if (hipMemcpyAsync(&c, sets[level].count, 4, hipMemcpyDeviceToHost, s) != hipSuccess)
@@ -506,8 +540,8 @@ PYR_API PyrResult pyrhip_rt_prims(const PyrhipDag* dag, uint32_t node_words, uin
         if (!make_set(std::min<uint64_t>(8ull * c, node_bound), &sets[level - 1]))
             return fail(PYR_ERROR_OUT_OF_MEMORY, "rt prims set");
         if (c)
-            hipLaunchKernelGGL(k_set_children, dim3(blocks_of(c)), dim3(kBlock), 0, s, g, sets[level], c,
-                               sets[level - 1], err);
+            hipLaunchKernelGGL(k_set_children, dim3((c + kParentsPerBlock - 1) / kParentsPerBlock), dim3(kBlock), 0,
+                               s, g, sets[level], c, sets[level - 1], err);
     }
 
     // phase 3: boxes, level 3 first
