@@ -141,3 +141,55 @@ def test_error_strings_without_device():
     assert L.pyrhip_build(2, 0, 0, 0, 0, 0, None, None, C.byref(out)) == -1
     assert b"log2_size" in L.pyr_error_message()
     assert L.pyrhip_build(5, 0, 0, 0, 0, 10, None, None, C.byref(out)) == -1
+
+
+REFERENCE_INCLUDE = os.path.join(os.environ.get("PYRIT_REFERENCE", "/root/reference"), "include")
+SHARED_STRUCTS = ["PyrRay", "PyrHit", "PyrCamera", "PyrGeometryData", "PyrInstanceData", "PyrScene", "PyrCreateInfo",
+                  "PyrVoxel", "PyrFrameInfo", "PyrTargets", "PyrMaterial", "PyrChunkKey", "PyrWorldGenParams",
+                  "PyrWorldInfo", "PyrWorldStats", "PyrWorldEdit", "PyrDagInfo"]
+
+
+def _fields(name):
+    from pyrit_amd import abi
+
+    dtypes = {"PyrRay": abi.RAY, "PyrHit": abi.HIT, "PyrVoxel": abi.VOXEL}
+    if name in dtypes:
+        return list(dtypes[name].names)
+    return [f[0] for f in getattr(abi, name)._fields_]
+
+
+def _layout(include_dir, structs, macros):
+    lines = ["#include <stdio.h>", "#include <stddef.h>", "#include <stdint.h>", '#include "pyrit.h"', "int main(void) {"]
+    for s in structs:
+        lines.append(f'  printf("{s} %zu\\n", sizeof({s}));')
+        for f in _fields(s):
+            lines.append(f'  printf("{s}.{f} %zu\\n", offsetof({s}, {f}));')
+    for m in macros:
+        lines.append(f'  printf("{m} %lld\\n", (long long)({m}));')
+    lines.append("  return 0;\n}")
+    with tempfile.TemporaryDirectory() as d:
+        src, exe = os.path.join(d, "l.c"), os.path.join(d, "l")
+        open(src, "w").write("\n".join(lines))
+        r = subprocess.run(["gcc", "-std=c11", f"-I{include_dir}", src, "-o", exe], capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr[-2000:]
+        return subprocess.run([exe], capture_output=True, text=True, check=True).stdout.splitlines()
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REFERENCE_INCLUDE, "pyrit.h")),
+                    reason="the reference's include/pyrit.h is not available here")
+def test_layouts_and_constants_equal_the_reference_header():
+    """Every field offset and size of the structs both headers declare, and
+    every integer constant of ours that the reference header also defines,
+    compiled against both headers (the check of tests/abi_test.zig:25-55,
+    field by field)."""
+    ref_text = open(os.path.join(REFERENCE_INCLUDE, "pyrit.h")).read()
+    ours = open(os.path.join(ROOT, "include", "pyrit.h")).read()
+    macros = [m for m in re.findall(r"^#define\s+(PYR_[A-Z0-9_]+)\s+\S", ours, flags=re.M)
+              if re.search(r"^#define\s+" + m + r"\s", ref_text, flags=re.M) and m not in ("PYR_API",)]
+    macros = [m for m in macros if not m.endswith("_H") and "(" not in m]
+    assert len(macros) > 20
+    a = _layout(os.path.join(ROOT, "include"), SHARED_STRUCTS, macros)
+    b = _layout(REFERENCE_INCLUDE, SHARED_STRUCTS, macros)
+    diff = [(x, y) for x, y in zip(a, b) if x != y]
+    assert not diff, diff[:10]
+    assert len(a) == len(b) and len(a) > 200
