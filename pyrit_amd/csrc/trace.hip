@@ -178,13 +178,14 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
         }
         if (exists) {
             if (scale == 0) {
-                uint32_t v[3];
-#pragma unroll
-                for (int a = 0; a < 3; a++) {
-                    v[a] = ((oct >> a) & 1u) ? n - 1 - cell[a] : cell[a];
-                    hit->voxel[a] = (int32_t)v[a];
-                }
                 if constexpr (SKIPPING) {
+                    uint32_t v[3];
+                    int32_t vox[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) {
+                        v[q] = ((oct >> q) & 1u) ? n - 1 - cell[q] : cell[q];
+                        vox[q] = (int32_t)v[q];
+                    }
                     uint32_t attribute = g.default_attribute;
                     if ((want_attr || skip || cut) && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
                     bool through = false;
@@ -193,8 +194,8 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                     } else if (cut) {
                         float p[3];
 #pragma unroll
-                        for (int a = 0; a < 3; a++) p[a] = o[a] + d[a] * t;
-                        through = cut_hole(cut, attribute, hit->voxel, p, axis);
+                        for (int q = 0; q < 3; q++) p[q] = o[q] + d[q] * t;
+                        through = cut_hole(cut, attribute, vox, p, axis);
                     }
                     if (skipped(skip, attribute) || through) {
                         // as if the voxel were empty: on to the next cell of edge 1
@@ -202,19 +203,19 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                         float tnext0 = ((float)(cell[0] + 1) - od[0]) * inv[0];
                         axis = 0;
 #pragma unroll
-                        for (int a = 1; a < 3; a++) {
-                            const float ta = ((float)(cell[a] + 1) - od[a]) * inv[a];
+                        for (int q = 1; q < 3; q++) {
+                            const float ta = ((float)(cell[q] + 1) - od[q]) * inv[q];
                             if (ta < tnext0) {
                                 tnext0 = ta;
-                                axis = (uint32_t)a;
+                                axis = (uint32_t)q;
                             }
                         }
-                        if (!(tnext0 < tend)) return false;
+                        if (!(tnext0 < tend)) break;
                         t = fmaxf(t, tnext0);
                         moved = true;
                         const uint32_t old0 = axis == 0 ? cell[0] : axis == 1 ? cell[1] : cell[2];
                         const uint32_t next0 = old0 + 1;
-                        if (next0 >= n) return false;
+                        if (next0 >= n) break;
                         if (axis == 0) cell[0] = next0;
                         else if (axis == 1) cell[1] = next0;
                         else cell[2] = next0;
@@ -223,16 +224,12 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                             scale = h0;
                             const uint32_t m0 = ~((1u << scale) - 1);
 #pragma unroll
-                            for (int a = 0; a < 3; a++) cell[a] &= m0;
+                            for (int q = 0; q < 3; q++) cell[q] &= m0;
                         }
                         continue;
                     }
                     hit->attribute = attribute;
                 }
-                const bool mirrored = ((oct >> axis) & 1u) != 0;
-                hit->t = t;
-                hit->face = 2 * axis + (mirrored ? 0u : 1u);
-                hit->inside = starts_inside && !moved;
                 found = true;
                 break;
             }
@@ -259,12 +256,12 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
                 axis = (uint32_t)a;
             }
         }
-        if (!(tnext < tend)) return false;
+        if (!(tnext < tend)) break;
         t = fmaxf(t, tnext);
         moved = true;
         const uint32_t old = axis == 0 ? cell[0] : axis == 1 ? cell[1] : cell[2];
         const uint32_t next = old + size;
-        if (next >= n) return false;
+        if (next >= n) break;
         if (axis == 0) cell[0] = next;
         else if (axis == 1) cell[1] = next;
         else cell[2] = next;
@@ -277,13 +274,21 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
         }
     }
     if (!found) return false;
+    // the hit: every value below is final when the loop leaves
+    uint32_t v[3];
+#pragma unroll
+    for (int a = 0; a < 3; a++) {
+        v[a] = ((oct >> a) & 1u) ? n - 1 - cell[a] : cell[a];
+        hit->voxel[a] = (int32_t)v[a];
+    }
+    const bool mirrored = ((oct >> axis) & 1u) != 0;
+    hit->t = t;
+    hit->face = 2 * axis + (mirrored ? 0u : 1u);
+    hit->inside = starts_inside && !moved;
     if constexpr (!SKIPPING) {
         uint32_t attribute = g.default_attribute;
-        if (want_attr && g.attrs) {
-            // the stack still holds the hit voxel's path and brick is its leaf
-            const uint32_t v[3] = {(uint32_t)hit->voxel[0], (uint32_t)hit->voxel[1], (uint32_t)hit->voxel[2]};
-            attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
-        }
+        // the stack still holds the hit voxel's path and brick is its leaf
+        if (want_attr && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
         hit->attribute = attribute;
     }
     return true;
