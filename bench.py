@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--quick", action="store_true", help="smaller traversal/batch sections")
-    ap.add_argument("--only", default="", help="comma list of sections: build,rtprims,trace,batch,serve,cpu,probe")
+    ap.add_argument("--only", default="", help="comma list of sections: build,rtprims,trace,scenes,batch,serve,cpu,probe")
     ap.add_argument("--radix-bits", type=int, default=0)
     ap.add_argument("--no-autotune", action="store_true",
                     help="build with the built-in defaults instead of the setting the autotuner picks for the "
@@ -303,6 +303,77 @@ def render_leg(args, log2, vox, n, cams, W, H, cam_hits):
     log(f"pyr_render {ms:.3f} ms/frame (hits + depth), hits equal to the camera kernel: {same}")
     return {"ms_per_frame": ms, "grays_per_s": W * H / (ms * 1e-3) / 1e9, "hits_equal_camera_kernel": same,
             "timing": "wall clock per synchronous pyr_render call (hits + depth targets)"}
+
+
+# published frame times of the reference's CUDA traversal for the same scenes
+# (RTX 5070 Laptop, 1920x1080 primary rays + depth + motion vectors,
+# pyrit-plan.md:37-41): instances -> ms per frame
+PUBLISHED_INSTANCE_MS = {3: 0.83, 64: 4.14, 1000: 31.6}
+
+
+def scenes_section(args, out):
+    """The reference's instance benchmark (tests/gpu_test.zig:394-470): the
+    128^3 test scene as one geometry, 3, 64 and 1000 instances on a grid,
+    pyr_commit + pyr_render (hits + depth) per frame at 1920x1080; GPU time
+    by HIP events on the context's stream over 40 frames after 5."""
+    from types import SimpleNamespace
+
+    from pyrit_amd import abi, lib, workloads
+    from pyrit_amd.device import Buffer, Events
+
+    L = lib()
+    grid = workloads.scene_grid(128)
+    res = {}
+    for count in (3, 64, 1000):
+        info = abi.PyrCreateInfo()
+        info.struct_size, info.version = C.sizeof(abi.PyrCreateInfo), abi.PYR_VERSION
+        info.max_instances, info.max_geometries = 4096, 16
+        info.node_pool_bytes = info.leaf_pool_bytes = info.attribute_pool_bytes = 64 << 20
+        ctx = C.c_void_p()
+        abi.check(L, L.pyr_create(C.byref(info), C.byref(ctx)), "create")
+        try:
+            dag, geo, view = C.c_void_p(), C.c_void_p(), C.c_void_p()
+            abi.check(L, L.pyr_dag_build_dense(7, grid.ctypes.data_as(C.c_void_p), 0, C.byref(dag)), "dag")
+            abi.check(L, L.pyr_geometry_create(ctx, dag, C.byref(geo)), "geometry")
+            L.pyr_dag_destroy(dag)
+            xf, eye, target = workloads.instance_grid(count)
+            for m in xf:
+                inst = C.c_void_p()
+                abi.check(L, L.pyr_instance_create(ctx, geo, C.byref(inst)), "instance")
+                abi.check(L, L.pyr_instance_set_transform(ctx, inst, (C.c_float * 12)(*m)), "transform")
+            abi.check(L, L.pyr_view_create(ctx, C.byref(view)), "view")
+            cam = abi.PyrCamera()
+            L.pyr_camera_look_at(C.byref(cam), (C.c_float * 3)(*eye), (C.c_float * 3)(*target),
+                                 (C.c_float * 3)(0.0, 1.0, 0.0))
+            L.pyr_camera_perspective(C.byref(cam), 0.9, 1920, 1080, 0.1)
+            hb, db = Buffer(1920 * 1080 * 16), Buffer(1920 * 1080 * 4)
+            t = abi.PyrTargets()
+            t.hits, t.depth = hb.ptr, db.ptr
+            abi.check(L, L.pyr_commit(ctx, None), "commit")
+            ev = Events(SimpleNamespace(handle=L.pyr_cuda_stream(ctx)))
+            frames = 40
+            for f in range(frames + 5):
+                if f == 5:
+                    ev.start()
+                abi.check(L, L.pyr_commit(ctx, None), "commit")
+                abi.check(L, L.pyr_render(ctx, view, C.byref(cam), C.byref(t)), "render")
+            ms = ev.stop() / frames
+            hits = hb.download(abi.HIT, 1920 * 1080)
+            ev.destroy()
+            hb.free()
+            db.free()
+        finally:
+            L.pyr_destroy(ctx)
+        res[str(count)] = {"ms_per_frame": ms, "grays_per_s": 1920 * 1080 / (ms * 1e-3) / 1e9,
+                           "hit_fraction": float((hits["instance"] != 0xFFFFFFFF).mean()),
+                           "published_ms_rtx5070_laptop": PUBLISHED_INSTANCE_MS[count],
+                           "speedup_vs_published": PUBLISHED_INSTANCE_MS[count] / ms}
+        log(f"scene of {count} instances: {ms:.3f} ms/frame ({res[str(count)]['hit_fraction']:.2f} hit), "
+            f"published {PUBLISHED_INSTANCE_MS[count]} ms")
+    out["scenes"] = {"workload": "128^3 test scene, instance grids of tests/gpu_test.zig:394-470, 1920x1080",
+                     "timing": "HIP events over 40 frames of pyr_commit + pyr_render (hits + depth)",
+                     "published": "RTX 5070 Laptop, CUDA software traversal, primary rays + depth + motion vectors",
+                     "instances": res}
 
 
 def _dag_of(built):
@@ -692,7 +763,8 @@ def main() -> int:
     if not device.gpu_available():
         print(json.dumps({"error": "no HIP device visible"}), flush=True)
         return 1
-    only = set(args.only.split(",")) if args.only else {"build", "rtprims", "trace", "batch", "serve", "cpu", "probe"}
+    only = set(args.only.split(",")) if args.only else {"build", "rtprims", "trace", "scenes", "batch", "serve", "cpu",
+                                                         "probe"}
     args.cpu_trace = "cpu" in only
     out = {}
     n, ms = 0, float("nan")
@@ -719,6 +791,8 @@ def main() -> int:
     if "trace" in only:
         trace_section(args, out)
         log(f"primary {out['traversal']['primary_grays_per_s']:.2f} Grays/s")
+    if "scenes" in only:
+        scenes_section(args, out)
     if "cpu" in only and rank == 0:
         cpu_section(args, out)
     res = {
@@ -728,7 +802,8 @@ def main() -> int:
         "config": {"workload": "1024^3 heightfield grid (4 octaves, amplitude 256, 2000 caves), full-depth build",
                    "voxels": n, "log2_size": args.log2, "parallelism": "single GPU"},
     }
-    for k in ("roofline", "cpu_baseline", "build", "rt_prims", "traversal", "batched", "serving", "copy_probe"):
+    for k in ("roofline", "cpu_baseline", "build", "rt_prims", "traversal", "scenes", "batched", "serving",
+              "copy_probe"):
         if k in out:
             res[k] = out[k]
     print(json.dumps(res), flush=True)

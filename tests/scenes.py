@@ -7,34 +7,7 @@ import numpy as np
 from oracle_py import VOXEL
 
 
-def attr_of(x, y, z):
-    """attrOf (tests/common.zig:14-19), vectorised, u32 wrap-around"""
-    x = np.asarray(x, np.int64).astype(np.uint32)
-    y = np.asarray(y, np.int64).astype(np.uint32)
-    z = np.asarray(z, np.int64).astype(np.uint32)
-    with np.errstate(over="ignore"):
-        h = (x * np.uint32(73856093)) ^ (y * np.uint32(19349663)) ^ (z * np.uint32(83492791))
-        h = h ^ (h >> np.uint32(13))
-        h = h * np.uint32(0x5BD1E995)
-        return (h ^ (h >> np.uint32(15))) | np.uint32(1)
-
-
-def scene_grid(n: int) -> np.ndarray:
-    """sceneVoxel (tests/common.zig:23-35): sphere, floor, thin pillars and
-    scattered single voxels; dense u32 grid indexed [z, y, x]."""
-    z, y, x = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
-    nf = np.float32(n)
-    c = np.float32(nf * np.float32(0.5))
-    dx = (x.astype(np.float32) + np.float32(0.5)) - np.float32(c * np.float32(0.8))
-    dy = (y.astype(np.float32) + np.float32(0.5)) - c
-    dz = (z.astype(np.float32) + np.float32(0.5)) - c
-    r = np.float32(nf * np.float32(0.3))
-    inside = (dx * dx + dy * dy + dz * dz) < r * r
-    inside |= y < 3
-    inside |= (x % 16 == 3) & (z % 16 == 5)
-    a = attr_of(x, y, z)
-    inside |= a % 97 == 0
-    return np.where(inside, a, 0).astype(np.uint32)
+from pyrit_amd.workloads import attr_of, scene_grid  # noqa: E402,F401  (sceneVoxel, tests/common.zig:14-35)
 
 
 def grid_to_points(grid: np.ndarray) -> np.ndarray:
