@@ -57,8 +57,19 @@ typedef struct PyrHit {
 
 #define PYR_TRACE_ANY_HIT 0x1u
 #define PYR_TRACE_NO_ATTRIBUTE 0x2u
-#define PYR_TRACE_EXTENDED 0x4u
+#define PYR_TRACE_EXTENDED 0x4u /* pyr_trace writes PyrHitEx (voxel, position, normal), e.g. for picking */
 #define PYR_TRACE_SKIP_TRANSPARENT 0x8u
+
+/* extended hit (reference pyrit.h:80-88) */
+typedef struct PyrHitEx {
+    PyrHit hit;
+    int32_t voxel[3];   /* voxel coordinate in the geometry's object space */
+    uint32_t reserved0;
+    float position[3];  /* hit point in world space (relative to the render origin) */
+    uint32_t reserved1;
+    float normal[3];    /* world normal */
+    uint32_t reserved2;
+} PyrHitEx;
 
 /* ---- camera (reference pyrit.h:96-113) ----------------------------------- */
 
@@ -303,8 +314,9 @@ PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view);
 /* primary hits and depth of every pixel */
 PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* targets);
 
-/* arbitrary rays: rays and hits are device pointers (PyrRay[count], PyrHit[count]);
- * flags PYR_TRACE_ANY_HIT | PYR_TRACE_NO_ATTRIBUTE | PYR_TRACE_SKIP_TRANSPARENT */
+/* arbitrary rays: rays and hits are device pointers (PyrRay[count], PyrHit[count],
+ * PyrHitEx[count] with PYR_TRACE_EXTENDED); flags PYR_TRACE_ANY_HIT |
+ * PYR_TRACE_NO_ATTRIBUTE | PYR_TRACE_SKIP_TRANSPARENT | PYR_TRACE_EXTENDED */
 PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint32_t count, uint32_t ray_mask,
                             uint32_t flags);
 

@@ -1050,9 +1050,10 @@ static void xform_vector(const float* m, const float v[3], float r[3]) {
     r[2] = m[8] * v[0] + m[9] * v[1] + m[10] * v[2];
 }
 
-/* traceScene + toHit (trace.zig:72-118) */
+/* traceScene + toHit (trace.zig:72-118); p_object (may be NULL) gets the
+ * hit point in the hit instance's object space (trace.zig:107) */
 static void trace_one(const OrScene* s, const float o[3], const float d[3], float tmin, float tmax,
-                      uint32_t ray_mask, uint32_t flags, OrHit* out) {
+                      uint32_t ray_mask, uint32_t flags, OrHit* out, float* p_object) {
     float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
     int want_attr = (flags & 0x2u) == 0;
     float best = tmax;
@@ -1090,6 +1091,8 @@ static void trace_one(const OrScene* s, const float o[3], const float d[3], floa
             best = h.t;
             out->t = h.t; out->instance = i; out->attribute = h.attribute;
             out->meta = h.face | (h.inside ? 0x20u : 0u);
+            if (p_object)
+                for (int a = 0; a < 3; a++) p_object[a] = oo[a] + dv[a] * h.t;
             if (flags & 0x1u) return;
         }
     }
@@ -1103,7 +1106,7 @@ static void* trace_worker(void* arg) {
     TraceJob* j = (TraceJob*)arg;
     for (uint64_t i = j->begin; i < j->end; i++) {
         const OrRay* r = &j->rays[i];
-        trace_one(j->s, r->origin, r->direction, r->tmin, r->tmax, j->mask, j->flags, &j->hits[i]);
+        trace_one(j->s, r->origin, r->direction, r->tmin, r->tmax, j->mask, j->flags, &j->hits[i], NULL);
     }
     return NULL;
 }
@@ -1126,6 +1129,68 @@ void or_trace_rays(const OrScene* s, const OrRay* rays, OrHit* hits, uint64_t co
         for (uint32_t k = 0; k < threads; k++) pthread_join(th[k], NULL);
 }
 
+/* extendedHit and hitNormal (trace.zig:125-157): the voxel from the hit
+ * point moved half a voxel inward on the hit face's axis (unless the ray
+ * started inside), the world hit point o + d t, and the world normal: the
+ * face normal through the transpose of world_to_object (vec.zig:43-49),
+ * normalised as v * (1 / sqrt((x x + y y) + z z)) (vec.zig:13-23). A miss
+ * is all zero but for the miss hit. */
+static void extended_hit(const OrScene* s, const float o[3], const float d[3], const OrHit* h, const float po[3],
+                         OrHitEx* r) {
+    memset(r, 0, sizeof(*r));
+    r->hit = *h;
+    if (h->instance == 0xFFFFFFFFu) return;
+    const uint32_t face = h->meta & 7u, axis = face >> 1;
+    const int inside = (h->meta & 0x20u) != 0;
+    for (uint32_t a = 0; a < 3; a++) {
+        float p = po[a];
+        if (!inside && a == axis) p += (face & 1u) ? 0.5f : -0.5f;
+        r->voxel[a] = (int32_t)floorf(p);
+    }
+    for (int a = 0; a < 3; a++) r->position[a] = o[a] + d[a] * h->t;
+    const float sign = (face & 1u) ? -1.0f : 1.0f;
+    const float n[3] = {axis == 0 ? sign : 0.0f, axis == 1 ? sign : 0.0f, axis == 2 ? sign : 0.0f};
+    const float* m = s->instances[h->instance].world_to_object;
+    const float v[3] = {m[0] * n[0] + m[4] * n[1] + m[8] * n[2], m[1] * n[0] + m[5] * n[1] + m[9] * n[2],
+                        m[2] * n[0] + m[6] * n[1] + m[10] * n[2]};
+    const float il = 1.0f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int a = 0; a < 3; a++) r->normal[a] = v[a] * il;
+}
+
+typedef struct {
+    const OrScene* s; const OrRay* rays; OrHitEx* hits; uint64_t begin, end; uint32_t mask, flags;
+} TraceExJob;
+
+static void* trace_ex_worker(void* arg) {
+    TraceExJob* j = (TraceExJob*)arg;
+    for (uint64_t i = j->begin; i < j->end; i++) {
+        const OrRay* r = &j->rays[i];
+        OrHit h;
+        float po[3] = {0, 0, 0};
+        trace_one(j->s, r->origin, r->direction, r->tmin, r->tmax, j->mask, j->flags, &h, po);
+        extended_hit(j->s, r->origin, r->direction, &h, po, &j->hits[i]);
+    }
+    return NULL;
+}
+
+void or_trace_rays_ex(const OrScene* s, const OrRay* rays, OrHitEx* hits, uint64_t count, uint32_t ray_mask,
+                      uint32_t flags, uint32_t threads) {
+    if (ray_mask == 0) ray_mask = 0xFFFFFFFFu;
+    if (threads < 1) threads = 1;
+    if (threads > 256) threads = 256;
+    pthread_t th[256];
+    TraceExJob jobs[256];
+    for (uint32_t k = 0; k < threads; k++) {
+        jobs[k].s = s; jobs[k].rays = rays; jobs[k].hits = hits;
+        jobs[k].begin = count * k / threads; jobs[k].end = count * (k + 1) / threads;
+        jobs[k].mask = ray_mask; jobs[k].flags = flags;
+        if (threads == 1) trace_ex_worker(&jobs[k]);
+        else pthread_create(&th[k], NULL, trace_ex_worker, &jobs[k]);
+    }
+    if (threads > 1)
+        for (uint32_t k = 0; k < threads; k++) pthread_join(th[k], NULL);
+}
+
 void or_trace_touch(const OrScene* s, const OrRay* rays, uint64_t count, uint32_t ray_mask, uint32_t flags,
                     uint64_t out[4]) {
     if (ray_mask == 0) ray_mask = 0xFFFFFFFFu;
@@ -1136,7 +1201,7 @@ void or_trace_touch(const OrScene* s, const OrRay* rays, uint64_t count, uint32_
     for (uint64_t i = 0; i < count; i++) {
         t->cur++; /* a new ray: every slot reads as empty */
         OrHit h;
-        trace_one(s, rays[i].origin, rays[i].direction, rays[i].tmin, rays[i].tmax, ray_mask, flags, &h);
+        trace_one(s, rays[i].origin, rays[i].direction, rays[i].tmin, rays[i].tmax, ray_mask, flags, &h, NULL);
         out[3] += h.instance != 0xFFFFFFFFu;
     }
     g_touch = NULL;

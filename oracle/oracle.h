@@ -49,6 +49,13 @@
  *                         order, box = min/max of their voxels, attr_base =
  *                         voxels before the cell in Morton order, node = a
  *                         subtree whose lookups give exactly the cell's voxels)
+ *   or_trace_rays_ex    : the voxel equals the walk's own hit voxel (the
+ *                         start voxel for inside hits) but at float grazing
+ *                         edges; hits equal or_trace_rays; the normal of a
+ *                         similarity instance equals object_to_world times
+ *                         the face normal, normalised in f64 (within 1e-6),
+ *                         and faces the ray (dot(normal, d) < 0) for hits
+ *                         from outside; misses are zero but for the hit
  *   or_trace_touch      : word counts derived by hand on a one-voxel DAG (hit,
  *                         miss through the same leaf, ray outside the bounds),
  *                         per-ray distinctness, hit count equal to the tracer's
@@ -154,6 +161,14 @@ typedef struct OrScene {
  * edge is at most about 1.4 world units */
 void or_trace_rays(const OrScene* s, const OrRay* rays, OrHit* hits, uint64_t count, uint32_t ray_mask,
                    uint32_t flags, uint32_t threads);
+
+/* PYR_TRACE_EXTENDED hits (trace.zig:125-157; the layout of PyrHitEx) */
+typedef struct OrHitEx {
+    OrHit hit; int32_t voxel[3]; uint32_t reserved0; float position[3]; uint32_t reserved1;
+    float normal[3]; uint32_t reserved2;
+} OrHitEx;
+void or_trace_rays_ex(const OrScene* s, const OrRay* rays, OrHitEx* hits, uint64_t count, uint32_t ray_mask,
+                      uint32_t flags, uint32_t threads);
 
 /* Instrumented trace (one thread): sums over the rays of the distinct node
  * words, leaves and attributes each ray's traversal reads, and the hit count:

@@ -13,6 +13,8 @@ vp = C.c_void_p
 VOXEL = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("attribute", "<u4")])
 RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
+HIT_EX = np.dtype([("hit", HIT), ("voxel", "<i4", 3), ("reserved0", "<u4"), ("position", "<f4", 3),
+                   ("reserved1", "<u4"), ("normal", "<f4", 3), ("reserved2", "<u4")])  # PyrHitEx
 RT_PRIM = np.dtype([("node", "<u4"), ("attr_base", "<u4"), ("cell", "<u8")])  # PyrhipRtPrim
 
 PYR_OK = 0
@@ -30,6 +32,7 @@ PYR_NO_HIT = 0xFFFFFFFF
 PYR_HIT_INSIDE = 0x20
 PYR_TRACE_ANY_HIT = 0x1
 PYR_TRACE_NO_ATTRIBUTE = 0x2
+PYR_TRACE_EXTENDED = 0x4
 PYR_TRACE_SKIP_TRANSPARENT = 0x8
 PYR_BUILD_HOST_INPUT = 0x1
 PYR_BUILD_EDITABLE = 0x2

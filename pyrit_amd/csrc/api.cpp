@@ -1084,12 +1084,12 @@ PYR_API PyrResult pyr_commit(PyrContext* ctx, const PyrFrameInfo* info) {
 
 PYR_API uint64_t pyr_scene_device(PyrContext* ctx) { return ctx ? (uint64_t)(uintptr_t)ctx->scene_dev : 0; }
 
-// trace (context.zig:1869-1885); ray_mask 0 = all instances
+// trace (context.zig:1869-1885); ray_mask 0 = all instances; PYR_TRACE_EXTENDED
+// writes PyrHitEx (gpu_kernels.zig:74-85)
 PYR_API PyrResult pyr_trace(PyrContext* ctx, uint64_t rays, uint64_t hits, uint32_t count, uint32_t ray_mask,
                             uint32_t flags) {
     diag_clear();
     if (!ctx || (count && (!rays || !hits))) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
-    if (flags & PYR_TRACE_EXTENDED) return fail(PYR_ERROR_INVALID_ARGUMENT, "PYR_TRACE_EXTENDED is not supported");
     const bool skipping = ((flags & PYR_TRACE_SKIP_TRANSPARENT) && ctx->scene_transparent) || ctx->scene_cutout;
     PyrResult r = launch_trace_scene(ctx->scene_dev, rays, hits, count, ray_mask ? ray_mask : 0xFFFFFFFFu, flags,
                                      ctx->scene_levels, skipping, ctx->stream);

@@ -341,10 +341,12 @@ __device__ __forceinline__ bool cutout_here(const PyrInstanceData& in) {
 
 // SKIPPING: the launch found a material set that may apply (skip flag with
 // transparent materials, or cutout materials); each variant holds one walk,
-// which keeps the registers of the plain one at its own count
-template <bool SKIPPING>
+// which keeps the registers of the plain one at its own count. EXT: the best
+// hit's point in its instance's object space goes to p_object (trace.zig:107).
+template <bool SKIPPING, bool EXT = false>
 __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d[3], float tmin, float tmax,
-                              uint32_t ray_mask, uint32_t flags, uint32_t* stk, uint32_t levels) {
+                              uint32_t ray_mask, uint32_t flags, uint32_t* stk, uint32_t levels,
+                              float* p_object = nullptr) {
     const PyrInstanceData* inst = (const PyrInstanceData*)(uintptr_t)s.instances;
     const PyrGeometryData* geo = (const PyrGeometryData*)(uintptr_t)s.geometries;
     const float inv[3] = {safe_rcp(d[0]), safe_rcp(d[1]), safe_rcp(d[2])};
@@ -389,6 +391,9 @@ __device__ PyrHit trace_scene(const PyrScene& s, const float o[3], const float d
         if (walk<SKIPPING>(v, oo, dv, tmin, best, want_attr, stk, &h, skip, icut)) {
             best = h.t;
             result = to_hit(true, h, i);
+            if (EXT)
+#pragma unroll
+                for (int a = 0; a < 3; a++) p_object[a] = oo[a] + dv[a] * h.t;
             if (flags & PYR_TRACE_ANY_HIT) return result;
         }
     }
@@ -410,6 +415,57 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_scene(const PyrScene* sce
     const PyrRay r = rays[i];
     hits[i] = trace_scene<SKIPPING>(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags, s_stk + threadIdx.x,
                                     levels);
+}
+
+// extendedHit and hitNormal (trace.zig:125-157): the voxel from the object
+// space hit point moved half a voxel inward on the hit face's axis (unless the
+// ray started inside), the hit point o + d t in the rays' space, and the face
+// normal through the transpose of world_to_object (vec.zig:43-49), normalised
+// as v * (1 / sqrt((x x + y y) + z z)) (vec.zig:13-23). A miss is all zero
+// but for the miss hit.
+__device__ __forceinline__ void store_extended(const PyrScene& s, const float o[3], const float d[3], const PyrHit& h,
+                                               const float po[3], PyrHitEx* out) {
+    uint4 w[4] = {make_uint4(__float_as_uint(h.t), h.instance, h.attribute, h.meta), make_uint4(0, 0, 0, 0),
+                  make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    if (h.instance != PYR_NO_HIT) {
+        const uint32_t face = h.meta & PYR_HIT_FACE_MASK, axis = face >> 1;
+        const bool inside = (h.meta & PYR_HIT_INSIDE) != 0;
+        int32_t vox[3];
+        float n[3], pw[3];
+#pragma unroll
+        for (uint32_t a = 0; a < 3; a++) {
+            float p = po[a];
+            if (!inside && a == axis) p += (face & 1u) ? 0.5f : -0.5f;
+            vox[a] = (int32_t)floorf(p);
+            pw[a] = o[a] + d[a] * h.t;
+            n[a] = a == axis ? ((face & 1u) ? -1.0f : 1.0f) : 0.0f;
+        }
+        const float* m = ((const PyrInstanceData*)(uintptr_t)s.instances)[h.instance].world_to_object;
+        const float v0 = m[0] * n[0] + m[4] * n[1] + m[8] * n[2];
+        const float v1 = m[1] * n[0] + m[5] * n[1] + m[9] * n[2];
+        const float v2 = m[2] * n[0] + m[6] * n[1] + m[10] * n[2];
+        const float il = 1.0f / sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
+        w[1] = make_uint4((uint32_t)vox[0], (uint32_t)vox[1], (uint32_t)vox[2], 0u);
+        w[2] = make_uint4(__float_as_uint(pw[0]), __float_as_uint(pw[1]), __float_as_uint(pw[2]), 0u);
+        w[3] = make_uint4(__float_as_uint(v0 * il), __float_as_uint(v1 * il), __float_as_uint(v2 * il), 0u);
+    }
+    uint4* dst = (uint4*)out;
+#pragma unroll
+    for (int k = 0; k < 4; k++) dst[k] = w[k];
+}
+
+// pyr_trace with PYR_TRACE_EXTENDED (gpu_kernels.zig:79-81): one PyrHitEx per ray
+template <bool SKIPPING>
+__global__ void __launch_bounds__(kTraceBlock) k_trace_scene_ex(const PyrScene* scene, const PyrRay* rays,
+                                                                 PyrHitEx* hits, uint32_t count, uint32_t ray_mask,
+                                                                 uint32_t flags, uint32_t levels) {
+    const uint32_t i = blockIdx.x * kTraceBlock + threadIdx.x;
+    if (i >= count) return;
+    const PyrRay r = rays[i];
+    float po[3] = {0.0f, 0.0f, 0.0f};
+    const PyrHit h = trace_scene<SKIPPING, true>(*scene, r.origin, r.direction, r.tmin, r.tmax, ray_mask, flags,
+                                                  s_stk + threadIdx.x, levels, po);
+    store_extended(*scene, r.origin, r.direction, h, po, hits + i);
 }
 
 __global__ void __launch_bounds__(kTraceBlock) k_trace_dag(DagView g, const PyrRay* rays, PyrHit* hits,
@@ -535,7 +591,16 @@ PyrResult launch_trace_scene(const PyrScene* scene_dev, uint64_t rays, uint64_t 
     if (count == 0) return PYR_OK;
     if (levels < 3 || levels > kMaxLevels) return fail(PYR_ERROR_INVALID_ARGUMENT, "scene levels %u", levels);
     const dim3 grid((count + kTraceBlock - 1) / kTraceBlock);
-    if (skipping)
+    if (flags & PYR_TRACE_EXTENDED) {
+        if (skipping)
+            hipLaunchKernelGGL(k_trace_scene_ex<true>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev,
+                               (const PyrRay*)(uintptr_t)rays, (PyrHitEx*)(uintptr_t)hits, count, ray_mask, flags,
+                               levels);
+        else
+            hipLaunchKernelGGL(k_trace_scene_ex<false>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev,
+                               (const PyrRay*)(uintptr_t)rays, (PyrHitEx*)(uintptr_t)hits, count, ray_mask, flags,
+                               levels);
+    } else if (skipping)
         hipLaunchKernelGGL(k_trace_scene<true>, grid, dim3(kTraceBlock), stack_bytes(levels), s, scene_dev,
                            (const PyrRay*)(uintptr_t)rays, (PyrHit*)(uintptr_t)hits, count, ray_mask, flags, levels);
     else

@@ -18,6 +18,8 @@ P = C.POINTER
 VOXEL = np.dtype([("x", "<i4"), ("y", "<i4"), ("z", "<i4"), ("attribute", "<u4")])
 RAY = np.dtype([("origin", "<f4", 3), ("tmin", "<f4"), ("direction", "<f4", 3), ("tmax", "<f4")])
 HIT = np.dtype([("t", "<f4"), ("instance", "<u4"), ("attribute", "<u4"), ("meta", "<u4")])
+HIT_EX = np.dtype([("hit", HIT), ("voxel", "<i4", 3), ("reserved0", "<u4"), ("position", "<f4", 3),
+                   ("reserved1", "<u4"), ("normal", "<f4", 3), ("reserved2", "<u4")])
 RT_PRIM = np.dtype([("node", "<u4"), ("attr_base", "<u4"), ("cell", "<u8")])
 
 
@@ -98,6 +100,7 @@ def lib() -> C.CDLL:
         L.or_walk_skipping.argtypes = [P(OrDagView), P(f32), P(f32), f32, f32, i32, C.c_void_p, C.c_void_p,
                                        P(i32), P(OrDagHit)]
         L.or_trace_rays.argtypes = [P(OrScene), C.c_void_p, C.c_void_p, u64, u32, u32, u32]
+        L.or_trace_rays_ex.argtypes = [P(OrScene), C.c_void_p, C.c_void_p, u64, u32, u32, u32]
         L.or_trace_touch.argtypes = [P(OrScene), C.c_void_p, u64, u32, u32, P(u64)]
         L.or_camera_look_at.argtypes = [P(OrCamera), P(f32), P(f32), P(f32)]
         L.or_camera_perspective.argtypes = [P(OrCamera), f32, u32, u32, f32]
@@ -276,6 +279,14 @@ def trace_rays(scene: OrScene, rays: np.ndarray, ray_mask=0, flags=0, threads=8)
     rays = np.ascontiguousarray(rays, dtype=RAY)
     hits = np.zeros(len(rays), HIT)
     lib().or_trace_rays(C.byref(scene), _p(rays), _p(hits), len(rays), ray_mask, flags, threads)
+    return hits
+
+
+def trace_rays_ex(scene: OrScene, rays: np.ndarray, ray_mask=0, flags=0, threads=8) -> np.ndarray:
+    """PYR_TRACE_EXTENDED hits (trace.zig:125-157)"""
+    rays = np.ascontiguousarray(rays, dtype=RAY)
+    hits = np.zeros(len(rays), HIT_EX)
+    lib().or_trace_rays_ex(C.byref(scene), _p(rays), _p(hits), len(rays), ray_mask, flags, threads)
     return hits
 
 

@@ -60,7 +60,8 @@ int main(void) {
   S(PyrRay) S(PyrHit) S(PyrCamera) S(PyrGeometryData) S(PyrInstanceData) S(PyrScene) S(PyrCreateInfo)
   S(PyrVoxel) S(PyrFrameInfo) S(PyrTargets) S(PyrhipBuilt) S(PyrhipChunkBatch) S(PyrhipDag)
   S(PyrChunkKey) S(PyrWorldGenParams) S(PyrWorldInfo) S(PyrWorldStats) S(PyrWorldEdit) S(PyrDagInfo)
-  S(PyrhipBuildTuning) S(PyrhipTuneReport) S(PyrMaterial)
+  S(PyrhipBuildTuning) S(PyrhipTuneReport) S(PyrMaterial) S(PyrHitEx)
+  O(PyrHitEx, voxel) O(PyrHitEx, position) O(PyrHitEx, normal) O(PyrHitEx, reserved2)
   O(PyrMaterial, flags) O(PyrMaterial, texture) O(PyrMaterial, side_color) O(PyrMaterial, variation_warm)
   O(PyrDagInfo, nodes) O(PyrDagInfo, leaves) O(PyrDagInfo, attributes) O(PyrDagInfo, attribute_count)
   O(PyrWorldInfo, memory_budget) O(PyrWorldInfo, y_min) O(PyrWorldInfo, flags) O(PyrWorldInfo, generate)
@@ -82,7 +83,7 @@ int main(void) {
 REFERENCE_SIZES = {"PyrRay": 32, "PyrHit": 16, "PyrCamera": 92, "PyrGeometryData": 32, "PyrInstanceData": 144,
                    "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80,
                    "PyrChunkKey": 16, "PyrWorldGenParams": 48, "PyrWorldInfo": 88, "PyrWorldStats": 48,
-                   "PyrWorldEdit": 32, "PyrDagInfo": 64, "PyrMaterial": 144}
+                   "PyrWorldEdit": 32, "PyrDagInfo": 64, "PyrMaterial": 144, "PyrHitEx": 64}
 # field offsets the reference's world ABI fixes (reference include/pyrit.h:815-875)
 REFERENCE_OFFSETS = {"PyrWorldInfo.memory_budget": 24, "PyrWorldInfo.y_min": 32, "PyrWorldInfo.flags": 68,
                      "PyrWorldInfo.generate": 72, "PyrWorldInfo.user": 80, "PyrWorldStats.voxel_pixels": 24,
@@ -126,6 +127,9 @@ def test_struct_layouts_match_reference_and_mirrors():
                 assert getattr(mirrors[sname], field).offset == v, key
     assert abi.VOXEL.itemsize == vals["PyrVoxel"] and abi.RAY.itemsize == vals["PyrRay"]
     assert abi.HIT.itemsize == vals["PyrHit"]
+    assert abi.HIT_EX.itemsize == vals["PyrHitEx"]
+    for f in ("voxel", "position", "normal", "reserved2"):
+        assert abi.HIT_EX.fields[f][1] == vals["PyrHitEx." + f], f
 
 
 def test_error_strings_without_device():
@@ -146,13 +150,13 @@ def test_error_strings_without_device():
 REFERENCE_INCLUDE = os.path.join(os.environ.get("PYRIT_REFERENCE", "/root/reference"), "include")
 SHARED_STRUCTS = ["PyrRay", "PyrHit", "PyrCamera", "PyrGeometryData", "PyrInstanceData", "PyrScene", "PyrCreateInfo",
                   "PyrVoxel", "PyrFrameInfo", "PyrTargets", "PyrMaterial", "PyrChunkKey", "PyrWorldGenParams",
-                  "PyrWorldInfo", "PyrWorldStats", "PyrWorldEdit", "PyrDagInfo"]
+                  "PyrWorldInfo", "PyrWorldStats", "PyrWorldEdit", "PyrDagInfo", "PyrHitEx"]
 
 
 def _fields(name):
     from pyrit_amd import abi
 
-    dtypes = {"PyrRay": abi.RAY, "PyrHit": abi.HIT, "PyrVoxel": abi.VOXEL}
+    dtypes = {"PyrRay": abi.RAY, "PyrHit": abi.HIT, "PyrVoxel": abi.VOXEL, "PyrHitEx": abi.HIT_EX}
     if name in dtypes:
         return list(dtypes[name].names)
     return [f[0] for f in getattr(abi, name)._fields_]

@@ -716,3 +716,62 @@ def test_cutout_holes():
     a = O.trace_rays(s2, big)
     s2.cutout[:] = _material_set(cut)
     assert np.array_equal(O.trace_rays(s2, big).view(np.uint32), a.view(np.uint32))
+
+
+def test_extended_hits_against_f64_dda():
+    """extendedHit (trace.zig:125-157) on three transformed instances of the
+    64^3 scene: the hit part equals the plain trace, the voxel equals the f64
+    DDA's voxel in the hit instance's object space (the start voxel for rays
+    starting inside), the position lies on the f64 ray at the DDA's t, the
+    normal equals object_to_world times the face normal for these similarity
+    transforms and faces the ray, and misses are zero but for the hit."""
+    n = 64
+    grid = scene_grid(n)
+    dag = O.build_dense(6, grid)
+    xf = [make_transform((0, 1, 0), 0.0, 1.0, (-80, 0, 0)),
+          make_transform((1, 2, 3), 0.7, 0.5, (20, 10, -5)),
+          make_transform((-1, 0.3, 0.2), 2.1, 1.7, (0, -40, 60))]
+    s, inst = transformed_scene(dag, n, xf, [0xFF, 0xFF, 0xFF])
+    rays_n = 12000
+    rng = np.random.default_rng(21)
+    o = (rng.random((rays_n, 3)) * 2 - 1) * 200
+    d = (rng.random((rays_n, 3)) * 2 - 1) * 70 - o
+    # a tenth of the rays start inside a filled voxel of instance 0
+    z, y, x = np.nonzero(grid)
+    pick = rng.integers(0, len(x), rays_n // 10)
+    o[: rays_n // 10] = np.stack([x[pick], y[pick], z[pick]], 1) + 0.5 + np.array([-80, 0, 0])
+    rays = np.zeros(rays_n, O.RAY)
+    rays["origin"], rays["direction"] = o.astype(np.float32), d.astype(np.float32)
+    rays["tmin"], rays["tmax"] = 0.0, np.float32(3.402823466e38)
+    ex = O.trace_rays_ex(s, rays)
+    plain = O.trace_rays(s, rays)
+    assert np.array_equal(ex.view(np.uint32).reshape(-1, 16)[:, :4], plain.view(np.uint32).reshape(-1, 4))
+    miss = plain["instance"] == 0xFFFFFFFF
+    rest = ex[miss].view(np.uint32).reshape(-1, 16)[:, 4:]
+    assert not rest.any()
+    w2o = [np.array(inst[k].world_to_object[:], np.float64) for k in range(3)]
+    o2w = [np.array(inst[k].object_to_world[:], np.float64).reshape(3, 4)[:, :3] for k in range(3)]
+    off = nhits = inside = 0
+    for i in np.nonzero(~miss)[0]:
+        h, e = plain[i], ex[i]
+        k = int(h["instance"])
+        of, df = rays[i]["origin"].astype(np.float64), rays[i]["direction"].astype(np.float64)
+        oo, dd = apply_f64(w2o[k], of, 1), apply_f64(w2o[k], df, 0)
+        face, ax = int(h["meta"] & 7), int(h["meta"] & 7) >> 1
+        nobj = np.zeros(3)
+        nobj[ax] = -1.0 if face & 1 else 1.0
+        nw = o2w[k] @ nobj
+        assert np.abs(e["normal"] - nw / np.sqrt(nw @ nw)).max() < 1e-6
+        pw = of + df * float(h["t"])
+        assert np.abs(e["position"] - pw).max() <= 1e-5 * max(1.0, np.abs(pw).max())
+        if h["meta"] & 0x20:
+            inside += 1
+            assert tuple(e["voxel"]) == tuple(np.floor(oo).astype(int))
+            continue
+        nhits += 1
+        assert float(e["normal"] @ df) < 0
+        r = O.ref_dda(grid, n, oo, dd, 0.0, 1e30)
+        if r is None or tuple(e["voxel"]) != r[1]:
+            off += 1
+    assert nhits > rays_n // 20 and inside >= rays_n // 10 - 5
+    assert off * 2000 <= nhits, (off, nhits)
