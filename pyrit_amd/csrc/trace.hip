@@ -425,19 +425,18 @@ __global__ void __launch_bounds__(kTraceBlock) k_trace_scene(const PyrScene* sce
 // but for the miss hit.
 __device__ __forceinline__ void store_extended(const PyrScene& s, const float o[3], const float d[3], const PyrHit& h,
                                                const float po[3], PyrHitEx* out) {
-    uint4 w[4] = {make_uint4(__float_as_uint(h.t), h.instance, h.attribute, h.meta), make_uint4(0, 0, 0, 0),
-                  make_uint4(0, 0, 0, 0), make_uint4(0, 0, 0, 0)};
+    PyrHitEx r = {};
+    r.hit = h;
     if (h.instance != PYR_NO_HIT) {
         const uint32_t face = h.meta & PYR_HIT_FACE_MASK, axis = face >> 1;
         const bool inside = (h.meta & PYR_HIT_INSIDE) != 0;
-        int32_t vox[3];
-        float n[3], pw[3];
+        float n[3];
 #pragma unroll
         for (uint32_t a = 0; a < 3; a++) {
             float p = po[a];
             if (!inside && a == axis) p += (face & 1u) ? 0.5f : -0.5f;
-            vox[a] = (int32_t)floorf(p);
-            pw[a] = o[a] + d[a] * h.t;
+            r.voxel[a] = (int32_t)floorf(p);
+            r.position[a] = o[a] + d[a] * h.t;
             n[a] = a == axis ? ((face & 1u) ? -1.0f : 1.0f) : 0.0f;
         }
         const float* m = ((const PyrInstanceData*)(uintptr_t)s.instances)[h.instance].world_to_object;
@@ -445,13 +444,11 @@ __device__ __forceinline__ void store_extended(const PyrScene& s, const float o[
         const float v1 = m[1] * n[0] + m[5] * n[1] + m[9] * n[2];
         const float v2 = m[2] * n[0] + m[6] * n[1] + m[10] * n[2];
         const float il = 1.0f / sqrtf((v0 * v0 + v1 * v1) + v2 * v2);
-        w[1] = make_uint4((uint32_t)vox[0], (uint32_t)vox[1], (uint32_t)vox[2], 0u);
-        w[2] = make_uint4(__float_as_uint(pw[0]), __float_as_uint(pw[1]), __float_as_uint(pw[2]), 0u);
-        w[3] = make_uint4(__float_as_uint(v0 * il), __float_as_uint(v1 * il), __float_as_uint(v2 * il), 0u);
+        r.normal[0] = v0 * il;
+        r.normal[1] = v1 * il;
+        r.normal[2] = v2 * il;
     }
-    uint4* dst = (uint4*)out;
-#pragma unroll
-    for (int k = 0; k < 4; k++) dst[k] = w[k];
+    *out = r;
 }
 
 // pyr_trace with PYR_TRACE_EXTENDED (gpu_kernels.zig:79-81): one PyrHitEx per ray
