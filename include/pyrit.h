@@ -277,6 +277,10 @@ PYR_API PyrResult pyr_instance_create(PyrContext* ctx, PyrGeometry geometry, Pyr
 PYR_API PyrResult pyr_instance_destroy(PyrContext* ctx, PyrInstance instance);
 PYR_API PyrResult pyr_instance_set_transform(PyrContext* ctx, PyrInstance instance, const float object_to_world[12]);
 PYR_API PyrResult pyr_instance_set_mask(PyrContext* ctx, PyrInstance instance, uint32_t mask); /* default 0xFF */
+/* another geometry for the instance (bounds from its size, a new history);
+ * like every instance change it reaches the device at the next commit */
+PYR_API PyrResult pyr_instance_set_geometry(PyrContext* ctx, PyrInstance instance, PyrGeometry geometry);
+PYR_API PyrResult pyr_instance_set_user(PyrContext* ctx, PyrInstance instance, uint32_t user); /* InstanceData.user */
 PYR_API uint32_t pyr_instance_index(PyrInstance instance); /* = PyrHit.instance */
 
 /* ---- frames (reference pyrit.h:504-515) ---------------------------------- */
@@ -375,6 +379,8 @@ PYR_API uint32_t pyr_voxel_attribute(uint32_t material, uint32_t r, uint32_t g, 
 PYR_API void pyr_camera_look_at(PyrCamera* camera, const float eye[3], const float target[3], const float up[3]);
 PYR_API void pyr_camera_perspective(PyrCamera* camera, float fov_y_radians, uint32_t width, uint32_t height,
                                     float near_plane);
+/* parallel rays; the view spans +-half_height world units vertically */
+PYR_API void pyr_camera_orthographic(PyrCamera* camera, float half_height, uint32_t width, uint32_t height);
 
 /* ---- streamed worlds (reference pyrit.h:789-910) --------------------------
  *

@@ -30,6 +30,8 @@ PYR_ERROR_VERSION = -9
 PYR_FLT_MAX = np.float32(3.402823466e38)
 PYR_NO_HIT = 0xFFFFFFFF
 PYR_HIT_INSIDE = 0x20
+PYR_PROJECTION_PERSPECTIVE = 0
+PYR_PROJECTION_ORTHOGRAPHIC = 1
 PYR_TRACE_ANY_HIT = 0x1
 PYR_TRACE_NO_ATTRIBUTE = 0x2
 PYR_TRACE_EXTENDED = 0x4
@@ -238,6 +240,8 @@ FUNCTIONS = {
     "pyr_instance_destroy": (i32, [vp, vp]),
     "pyr_instance_set_transform": (i32, [vp, vp, P(f32)]),
     "pyr_instance_set_mask": (i32, [vp, vp, u32]),
+    "pyr_instance_set_geometry": (i32, [vp, vp, vp]),
+    "pyr_instance_set_user": (i32, [vp, vp, u32]),
     "pyr_instance_index": (u32, [vp]),
     "pyr_commit": (i32, [vp, P(PyrFrameInfo)]),
     "pyr_scene_device": (u64, [vp]),
@@ -250,6 +254,7 @@ FUNCTIONS = {
     "pyr_voxel_attribute": (u32, [u32, u32, u32, u32]),
     "pyr_camera_look_at": (None, [P(PyrCamera), P(f32), P(f32), P(f32)]),
     "pyr_camera_perspective": (None, [P(PyrCamera), f32, u32, u32, f32]),
+    "pyr_camera_orthographic": (None, [P(PyrCamera), f32, u32, u32]),
     "pyrhip_build": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuilt)]),
     "pyrhip_build_static": (i32, [u32, u64, u64, u32, u64, u32, P(PyrhipBuildTuning), vp, P(PyrhipBuildOutput),
                                   P(PyrhipBuilt)]),

@@ -1004,6 +1004,33 @@ PYR_API PyrResult pyr_instance_set_mask(PyrContext* ctx, PyrInstance instance, u
     return PYR_OK;
 }
 
+// instanceSetGeometry / instanceSetUser (context.zig:1619-1646)
+PYR_API PyrResult pyr_instance_set_geometry(PyrContext* ctx, PyrInstance instance, PyrGeometry geometry) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    Geometry* g = geo_of(ctx, geometry);
+    if (!g) return fail(PYR_ERROR_INVALID_HANDLE, "geometry");
+    Geometry& old = ctx->geos[in->data.geometry];
+    if (old.users) old.users--;
+    g->users++;
+    in->data.geometry = handle_index((uint64_t)(uintptr_t)geometry);
+    in->data.history = ++ctx->history;  // another shape: no history
+    transformed_box(in->data.object_to_world, (float)(1u << g->data.log2_size), in->data.bounds_min,
+                    in->data.bounds_max);
+    return PYR_OK;
+}
+
+PYR_API PyrResult pyr_instance_set_user(PyrContext* ctx, PyrInstance instance, uint32_t user) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    in->data.user = user;
+    return PYR_OK;
+}
+
 PYR_API uint32_t pyr_instance_index(PyrInstance instance) {
     return instance ? handle_index((uint64_t)(uintptr_t)instance) : 0xFFFFFFFFu;
 }
@@ -1191,6 +1218,16 @@ PYR_API void pyr_camera_look_at(PyrCamera* c, const float eye[3], const float ta
     const float m[12] = {right[0], tu[0], back[0], eye[0], right[1], tu[1], back[1], eye[1],
                          right[2], tu[2], back[2], eye[2]};
     memcpy(c->view_to_world, m, sizeof(m));
+}
+
+// root.zig:483-489
+PYR_API void pyr_camera_orthographic(PyrCamera* c, float half_height, uint32_t width, uint32_t height) {
+    if (!c) return;
+    c->projection = PYR_PROJECTION_ORTHOGRAPHIC;
+    c->width = width;
+    c->height = height;
+    c->scale[0] = half_height * (float)width / (float)height;
+    c->scale[1] = half_height;
 }
 
 PYR_API void pyr_camera_perspective(PyrCamera* c, float fov_y, uint32_t width, uint32_t height, float near_plane) {

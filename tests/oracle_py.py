@@ -275,6 +275,36 @@ def single_scene(dag: Dag, log2_size: int | None = None):
     return s
 
 
+def multi_scene(dags: list, instances: list) -> OrScene:
+    """OrScene over several DAGs (pools concatenated, geometry offsets into
+    them) and instances given as (object_to_world[12], geometry index, mask);
+    inverse and bounds as the product computes them (xform.zig:8-58)."""
+    OL = lib()
+    nodes = np.concatenate([d.nodes for d in dags]).astype(np.uint32)
+    leaves = np.concatenate([d.leaves for d in dags] + [np.zeros(1, np.uint64)]).astype(np.uint64)
+    attrs = np.concatenate([d.attrs for d in dags] + [np.zeros(1, np.uint32)]).astype(np.uint32)
+    geo = (OrGeometry * len(dags))()
+    no = lo = ao = 0
+    for i, d in enumerate(dags):
+        geo[i].node_offset, geo[i].leaf_offset, geo[i].attribute_offset = no, lo, ao
+        geo[i].root, geo[i].log2_size, geo[i].flags, geo[i].default_attribute = d.root, d.log2_size, 1, 1
+        no, lo, ao = no + len(d.nodes), lo + len(d.leaves), ao + len(d.attrs)
+    inst = (OrInstance * len(instances))()
+    for k, (m, g, mask) in enumerate(instances):
+        inst[k].object_to_world[:] = [float(v) for v in m]
+        OL.or_inverse(inst[k].object_to_world, inst[k].world_to_object)
+        OL.or_transformed_box(inst[k].object_to_world, float(1 << dags[g].log2_size), inst[k].bounds_min,
+                              inst[k].bounds_max)
+        inst[k].geometry, inst[k].mask, inst[k].flags = g, mask, 1
+    s = OrScene()
+    s.nodes = nodes.ctypes.data_as(P(u32))
+    s.leaves = leaves.ctypes.data_as(P(u64))
+    s.attrs = attrs.ctypes.data_as(P(u32))
+    s.geometries, s.instances, s.instance_count = geo, inst, len(instances)
+    s._keep = (nodes, leaves, attrs, geo, inst, dags)
+    return s
+
+
 def trace_rays(scene: OrScene, rays: np.ndarray, ray_mask=0, flags=0, threads=8) -> np.ndarray:
     rays = np.ascontiguousarray(rays, dtype=RAY)
     hits = np.zeros(len(rays), HIT)

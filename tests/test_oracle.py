@@ -324,6 +324,46 @@ def test_camera_closed_forms():
     assert np.allclose(rays["origin"], (-32, 96, -32))
 
 
+def test_camera_shift_planes_and_orthographic_closed_forms():
+    """cameraRay (render.zig:16-32) in closed form, in f64: a perspective ray
+    through pixel (x, y) points along right vx + up vy - back with
+    vx = (nx + shift x) scale x, nx from the jittered pixel centre, and its
+    tmin/tmax are the near/far planes times |(vx, vy, -1)|; an orthographic
+    ray starts at eye + right vx + up vy, points along -back, and its planes
+    are not scaled."""
+    eye, tgt = np.array([-32.0, 96.0, -32.0]), np.array([32.0, 32.0, 32.0])
+    back = (eye - tgt) / np.linalg.norm(eye - tgt)
+    right = np.cross([0.0, 1.0, 0.0], back)
+    right /= np.linalg.norm(right)
+    up = np.cross(back, right)
+    w, h = 40, 30
+    for ortho in (False, True):
+        c = O.camera(tuple(eye), tuple(tgt), (0, 1, 0), 0.9, w, h, 3.0)
+        if ortho:
+            c.projection, c.scale[0], c.scale[1] = 1, 20.0 * w / h, 20.0
+        c.shift[:] = [0.1, -0.2]
+        c.jitter[:] = [0.25, -0.375]
+        c.far_plane = 150.0
+        rays = O.camera_rays(c)
+        y, x = np.divmod(np.arange(w * h), w)
+        nx = 2 * (x + 0.5 + 0.25) / w - 1
+        ny = 1 - 2 * (y + 0.5 - 0.375) / h
+        vx, vy = (nx + 0.1) * c.scale[0], (ny - 0.2) * c.scale[1]
+        if ortho:
+            o = eye + vx[:, None] * right + vy[:, None] * up
+            d = np.broadcast_to(-back, o.shape)
+            ln = np.ones(w * h)
+        else:
+            o = np.broadcast_to(eye, (w * h, 3))
+            dv = vx[:, None] * right + vy[:, None] * up - back
+            ln = np.linalg.norm(dv, axis=1)
+            d = dv / ln[:, None]
+        assert np.abs(rays["origin"] - o).max() < 1e-4
+        assert np.abs(rays["direction"] - d).max() < 1e-5
+        assert np.allclose(rays["tmin"], 3.0 * ln, rtol=1e-5)
+        assert np.allclose(rays["tmax"], 150.0 * ln, rtol=1e-5)
+
+
 def test_trace_scene_equals_walk_for_identity_instance():
     d = O.build_dense(6, scene_grid(64))
     c = O.camera((-32, 96, -32), (32, 32, 32), (0, 1, 0), np.pi / 3, 64, 64)
