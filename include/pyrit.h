@@ -197,6 +197,20 @@ PYR_API void* pyr_cuda_stream(PyrContext* ctx); /* the context's hipStream_t */
 PYR_API uint32_t pyr_features(PyrContext* ctx); /* 0: no RT cores */
 PYR_API PyrResult pyr_synchronize(PyrContext* ctx);
 
+/* pool use in bytes (the path-edit arena is not part of the capacity), live
+ * geometries and instances, committed frames (reference pyrit.h:712-722) */
+typedef struct PyrStats {
+    uint64_t node_pool_used, node_pool_capacity;
+    uint64_t leaf_pool_used, leaf_pool_capacity;
+    uint64_t attribute_pool_used, attribute_pool_capacity;
+    uint32_t geometries, instances;
+    uint64_t frame;
+    uint32_t features;
+    uint32_t reserved;
+} PyrStats;
+
+PYR_API PyrResult pyr_get_stats(PyrContext* ctx, PyrStats* out);
+
 /* ---- voxels (reference pyrit.h:413-416) ---------------------------------- */
 
 typedef struct PyrVoxel {

@@ -210,6 +210,12 @@ class PyrTerrainChunk(C.Structure):
                 ("grid_dim", u32), ("grid_first", u32)]
 
 
+class PyrStats(C.Structure):
+    _fields_ = [("node_pool_used", u64), ("node_pool_capacity", u64), ("leaf_pool_used", u64),
+                ("leaf_pool_capacity", u64), ("attribute_pool_used", u64), ("attribute_pool_capacity", u64),
+                ("geometries", u32), ("instances", u32), ("frame", u64), ("features", u32), ("reserved", u32)]
+
+
 class PyrDagInfo(C.Structure):
     _fields_ = [("log2_size", u32), ("root", u32), ("voxel_count", u64), ("nodes", C.POINTER(u32)),
                 ("node_words", u64), ("leaves", C.POINTER(u64)), ("leaf_count", u64),
@@ -241,6 +247,7 @@ FUNCTIONS = {
     "pyr_instance_set_transform": (i32, [vp, vp, P(f32)]),
     "pyr_instance_set_mask": (i32, [vp, vp, u32]),
     "pyr_instance_set_geometry": (i32, [vp, vp, vp]),
+    "pyr_get_stats": (i32, [vp, P(PyrStats)]),
     "pyr_instance_set_user": (i32, [vp, vp, u32]),
     "pyr_instance_index": (u32, [vp]),
     "pyr_commit": (i32, [vp, P(PyrFrameInfo)]),

@@ -705,6 +705,25 @@ PYR_API uint32_t pyr_features(PyrContext* ctx) {
     return 0;
 }
 
+// stats (context.zig:2999-3017)
+PYR_API PyrResult pyr_get_stats(PyrContext* ctx, PyrStats* out) {
+    diag_clear();
+    if (!ctx || !out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
+    PyrStats st{};
+    st.node_pool_used = ctx->nodes.ra.used() * ctx->nodes.elem;
+    st.node_pool_capacity = ctx->nodes.ra.capacity() * ctx->nodes.elem;
+    st.leaf_pool_used = ctx->leaves.ra.used() * ctx->leaves.elem;
+    st.leaf_pool_capacity = ctx->leaves.ra.capacity() * ctx->leaves.elem;
+    st.attribute_pool_used = ctx->attrs.ra.used() * ctx->attrs.elem;
+    st.attribute_pool_capacity = ctx->attrs.ra.capacity() * ctx->attrs.elem;
+    for (const Geometry& g : ctx->geos) st.geometries += g.live ? 1u : 0u;
+    for (const Instance& in : ctx->insts) st.instances += in.live ? 1u : 0u;
+    st.frame = ctx->frame;
+    st.features = 0;
+    *out = st;
+    return PYR_OK;
+}
+
 PYR_API PyrResult pyr_synchronize(PyrContext* ctx) {
     diag_clear();
     if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");

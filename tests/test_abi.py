@@ -60,7 +60,7 @@ int main(void) {
   S(PyrRay) S(PyrHit) S(PyrCamera) S(PyrGeometryData) S(PyrInstanceData) S(PyrScene) S(PyrCreateInfo)
   S(PyrVoxel) S(PyrFrameInfo) S(PyrTargets) S(PyrhipBuilt) S(PyrhipChunkBatch) S(PyrhipDag)
   S(PyrChunkKey) S(PyrWorldGenParams) S(PyrWorldInfo) S(PyrWorldStats) S(PyrWorldEdit) S(PyrDagInfo)
-  S(PyrhipBuildTuning) S(PyrhipTuneReport) S(PyrMaterial) S(PyrHitEx)
+  S(PyrhipBuildTuning) S(PyrhipTuneReport) S(PyrMaterial) S(PyrHitEx) S(PyrStats)
   O(PyrHitEx, voxel) O(PyrHitEx, position) O(PyrHitEx, normal) O(PyrHitEx, reserved2)
   O(PyrMaterial, flags) O(PyrMaterial, texture) O(PyrMaterial, side_color) O(PyrMaterial, variation_warm)
   O(PyrDagInfo, nodes) O(PyrDagInfo, leaves) O(PyrDagInfo, attributes) O(PyrDagInfo, attribute_count)
@@ -83,7 +83,8 @@ int main(void) {
 REFERENCE_SIZES = {"PyrRay": 32, "PyrHit": 16, "PyrCamera": 92, "PyrGeometryData": 32, "PyrInstanceData": 144,
                    "PyrScene": 224, "PyrCreateInfo": 104, "PyrVoxel": 16, "PyrFrameInfo": 32, "PyrTargets": 80,
                    "PyrChunkKey": 16, "PyrWorldGenParams": 48, "PyrWorldInfo": 88, "PyrWorldStats": 48,
-                   "PyrWorldEdit": 32, "PyrDagInfo": 64, "PyrMaterial": 144, "PyrHitEx": 64}
+                   "PyrWorldEdit": 32, "PyrDagInfo": 64, "PyrMaterial": 144, "PyrHitEx": 64,
+                   "PyrStats": 72}
 # field offsets the reference's world ABI fixes (reference include/pyrit.h:815-875)
 REFERENCE_OFFSETS = {"PyrWorldInfo.memory_budget": 24, "PyrWorldInfo.y_min": 32, "PyrWorldInfo.flags": 68,
                      "PyrWorldInfo.generate": 72, "PyrWorldInfo.user": 80, "PyrWorldStats.voxel_pixels": 24,
@@ -117,7 +118,7 @@ def test_struct_layouts_match_reference_and_mirrors():
                "PyrWorldGenParams": abi.PyrWorldGenParams, "PyrWorldInfo": abi.PyrWorldInfo,
                "PyrWorldStats": abi.PyrWorldStats, "PyrWorldEdit": abi.PyrWorldEdit, "PyrDagInfo": abi.PyrDagInfo,
                "PyrhipBuildTuning": abi.PyrhipBuildTuning, "PyrhipTuneReport": abi.PyrhipTuneReport,
-               "PyrMaterial": abi.PyrMaterial}
+               "PyrMaterial": abi.PyrMaterial, "PyrStats": abi.PyrStats}
     for name, cls in mirrors.items():
         assert C.sizeof(cls) == vals[name], name
     for key, v in vals.items():
@@ -150,7 +151,7 @@ def test_error_strings_without_device():
 REFERENCE_INCLUDE = os.path.join(os.environ.get("PYRIT_REFERENCE", "/root/reference"), "include")
 SHARED_STRUCTS = ["PyrRay", "PyrHit", "PyrCamera", "PyrGeometryData", "PyrInstanceData", "PyrScene", "PyrCreateInfo",
                   "PyrVoxel", "PyrFrameInfo", "PyrTargets", "PyrMaterial", "PyrChunkKey", "PyrWorldGenParams",
-                  "PyrWorldInfo", "PyrWorldStats", "PyrWorldEdit", "PyrDagInfo", "PyrHitEx"]
+                  "PyrWorldInfo", "PyrWorldStats", "PyrWorldEdit", "PyrDagInfo", "PyrHitEx", "PyrStats"]
 
 
 def _fields(name):

@@ -121,8 +121,28 @@ def _instance_table(c, count):
     return [abi.PyrInstanceData.from_buffer_copy(raw, k * C.sizeof(abi.PyrInstanceData)) for k in range(count)]
 
 
+def _stats(c):
+    st = c.abi.PyrStats()
+    c.check(c.L.pyr_get_stats(c.ctx, C.byref(st)), "stats")
+    return st
+
+
+def _words(c, geo):
+    """node words and leaves of a geometry, from its download"""
+    d = C.c_void_p()
+    c.check(c.L.pyr_geometry_download(c.ctx, geo, C.byref(d)), "download")
+    info = c.abi.PyrDagInfo()
+    c.L.pyr_dag_get_info(d, C.byref(info))
+    r = (info.node_words, info.leaf_count)
+    c.L.pyr_dag_destroy(d)
+    return r
+
+
 def test_instance_set_geometry_and_user(hip_device):
-    print("case: pyr_instance_set_geometry / set_user", flush=True)
+    """Also pyr_get_stats (context.zig:2999-3017): pool use in bytes follows
+    the geometries' node words and leaves, and a destroyed geometry's ranges
+    stay counted until the next commit releases them."""
+    print("case: pyr_instance_set_geometry / set_user / get_stats", flush=True)
     from pyrit_amd import abi
     from pyrit_amd.device import Buffer, to_device
 
@@ -130,13 +150,22 @@ def test_instance_set_geometry_and_user(hip_device):
     try:
         pa = grid_to_points(scene_grid(64))
         pb = grid_to_points(sphere_noise_grid(32, seed=77))
+        st0 = _stats(c)
+        assert (st0.node_pool_used, st0.leaf_pool_used, st0.geometries, st0.instances, st0.features) == (0, 0, 0, 0, 0)
+        assert st0.node_pool_capacity > 0 and st0.leaf_pool_capacity > 0 and st0.attribute_pool_capacity > 0
         ga, gb = c.geometry(6, pa), c.geometry(5, pb)
         da, db = O.build_points(6, pa), O.build_points(5, pb)
+        (wa, la), (wb, lb) = _words(c, ga), _words(c, gb)
+        st1 = _stats(c)
+        assert (st1.node_pool_used, st1.leaf_pool_used) == (4 * (wa + wb), 8 * (la + lb))
+        assert st1.attribute_pool_used > 0 and st1.geometries == 2
         insts = [c.instance(ga, m) for m in XF]
         for k, i in enumerate(insts):
             c.check(c.L.pyr_instance_set_user(c.ctx, i, 1000 + k), "user")
         c.check(c.L.pyr_commit(c.ctx, None), "commit")
         assert [d.user for d in _instance_table(c, 3)] == [1000, 1001, 1002]
+        st2 = _stats(c)
+        assert (st2.instances, st2.frame) == (3, st0.frame + 1)
 
         rays = O.camera_rays(O.camera((-60, 90, -70), (30, 10, 30), (0, 1, 0), 1.1, 160, 120))
         rng = np.random.default_rng(5)
@@ -172,7 +201,12 @@ def test_instance_set_geometry_and_user(hip_device):
         for i in (insts[0], insts[2]):
             c.check(c.L.pyr_instance_set_geometry(c.ctx, i, gb), "set geometry")
         c.check(c.L.pyr_geometry_destroy(c.ctx, ga), "destroy")
+        st3 = _stats(c)
+        assert st3.geometries == 1 and st3.node_pool_used == st1.node_pool_used  # released by the commit
         c.check(c.L.pyr_commit(c.ctx, None), "commit")
+        st4 = _stats(c)
+        assert (st4.node_pool_used, st4.leaf_pool_used) == (4 * wb, 8 * lb)
+        assert st4.attribute_pool_used < st1.attribute_pool_used
         all_b = O.trace_rays(O.multi_scene([db], [(m, 0, 0xFF) for m in XF]), rays)
         assert np.array_equal(_bits(trace()), _bits(all_b))
         assert len(set(all_b["instance"].tolist())) == 4  # three instances and misses
