@@ -295,6 +295,8 @@ PYR_API PyrResult pyr_instance_set_mask(PyrContext* ctx, PyrInstance instance, u
  * like every instance change it reaches the device at the next commit */
 PYR_API PyrResult pyr_instance_set_geometry(PyrContext* ctx, PyrInstance instance, PyrGeometry geometry);
 PYR_API PyrResult pyr_instance_set_user(PyrContext* ctx, PyrInstance instance, uint32_t user); /* InstanceData.user */
+/* a new InstanceData.history (a teleport: the next frame has no past) */
+PYR_API PyrResult pyr_instance_reset_history(PyrContext* ctx, PyrInstance instance);
 PYR_API uint32_t pyr_instance_index(PyrInstance instance); /* = PyrHit.instance */
 
 /* ---- frames (reference pyrit.h:504-515) ---------------------------------- */
@@ -329,6 +331,8 @@ typedef struct PyrTargets {
 
 PYR_API PyrResult pyr_view_create(PyrContext* ctx, PyrView* out);
 PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view);
+/* views keep no frame history here (no temporal passes): checks the handle */
+PYR_API PyrResult pyr_view_reset_history(PyrContext* ctx, PyrView view);
 /* primary hits and depth of every pixel */
 PYR_API PyrResult pyr_render(PyrContext* ctx, PyrView view, const PyrCamera* camera, const PyrTargets* targets);
 
@@ -395,6 +399,11 @@ PYR_API void pyr_camera_perspective(PyrCamera* camera, float fov_y_radians, uint
                                     float near_plane);
 /* parallel rays; the view spans +-half_height world units vertically */
 PYR_API void pyr_camera_orthographic(PyrCamera* camera, float half_height, uint32_t width, uint32_t height);
+/* sub-pixel offsets in [-0.5, 0.5) for PyrCamera.jitter: Halton (2, 3) over
+ * 1024 frames, or a 4x4 grid visited in 16 frames, rows in bit-reversed
+ * order and every other row backwards (reference pyrit.h:697-706) */
+PYR_API void pyr_jitter_halton(uint32_t frame, float out[2]);
+PYR_API void pyr_jitter_ordered(uint32_t frame, float out[2]);
 
 /* ---- streamed worlds (reference pyrit.h:789-910) --------------------------
  *

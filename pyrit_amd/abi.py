@@ -254,6 +254,10 @@ FUNCTIONS = {
     "pyr_scene_device": (u64, [vp]),
     "pyr_view_create": (i32, [vp, P(vp)]),
     "pyr_view_destroy": (i32, [vp, vp]),
+    "pyr_view_reset_history": (i32, [vp, vp]),
+    "pyr_instance_reset_history": (i32, [vp, vp]),
+    "pyr_jitter_halton": (None, [u32, P(f32)]),
+    "pyr_jitter_ordered": (None, [u32, P(f32)]),
     "pyr_render": (i32, [vp, vp, P(PyrCamera), P(PyrTargets)]),
     "pyr_trace": (i32, [vp, u64, u64, u32, u32, u32]),
     "pyr_material_default": (None, [P(PyrMaterial)]),

@@ -1041,6 +1041,15 @@ PYR_API PyrResult pyr_instance_set_geometry(PyrContext* ctx, PyrInstance instanc
     return PYR_OK;
 }
 
+PYR_API PyrResult pyr_instance_reset_history(PyrContext* ctx, PyrInstance instance) {
+    diag_clear();
+    if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
+    Instance* in = inst_of(ctx, instance);
+    if (!in) return fail(PYR_ERROR_INVALID_HANDLE, "instance");
+    in->data.history = ++ctx->history;
+    return PYR_OK;
+}
+
 PYR_API PyrResult pyr_instance_set_user(PyrContext* ctx, PyrInstance instance, uint32_t user) {
     diag_clear();
     if (!ctx) return fail(PYR_ERROR_INVALID_ARGUMENT, "null context");
@@ -1154,6 +1163,42 @@ PYR_API PyrResult pyr_view_destroy(PyrContext* ctx, PyrView view) {
     diag_clear();
     if (!ctx || !view) return fail(PYR_ERROR_INVALID_HANDLE, "view");
     return PYR_OK;
+}
+
+PYR_API PyrResult pyr_view_reset_history(PyrContext* ctx, PyrView view) {
+    diag_clear();
+    if (!ctx || !view) return fail(PYR_ERROR_INVALID_HANDLE, "view");
+    return PYR_OK;
+}
+
+// jitter sequences (root.zig:405-443)
+namespace {
+// radical inverse of index in base b: its base-b digits mirrored behind the point
+float radical_inverse(uint32_t index, uint32_t b) {
+    float scale = 1.0f, r = 0.0f;
+    for (uint32_t i = index; i > 0; i /= b) {
+        scale /= (float)b;
+        r += scale * (float)(i % b);
+    }
+    return r;
+}
+}  // namespace
+
+PYR_API void pyr_jitter_halton(uint32_t frame, float out[2]) {
+    if (!out) return;
+    const uint32_t i = frame % 1024u + 1u;
+    out[0] = radical_inverse(i, 2) - 0.5f;
+    out[1] = radical_inverse(i, 3) - 0.5f;
+}
+
+PYR_API void pyr_jitter_ordered(uint32_t frame, float out[2]) {
+    if (!out) return;
+    const uint32_t k = frame % 16u;
+    const uint32_t row = ((k & 1u) << 1) | ((k >> 1) & 1u);  // 0, 2, 1, 3
+    const uint32_t step = (k >> 2) & 3u;
+    const uint32_t col = (row & 1u) ? 3u - step : step;
+    out[0] = ((float)col + 0.5f) / 4.0f - 0.5f;
+    out[1] = ((float)row + 0.5f) / 4.0f - 0.5f;
 }
 
 // materials (root.zig:363-370, context.zig:1902-1926, 1960-1964)

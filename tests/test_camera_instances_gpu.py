@@ -195,6 +195,15 @@ def test_instance_set_geometry_and_user(hip_device):
         table = _instance_table(c, 3)
         assert [d.geometry for d in table] == [0, 1, 0] and [d.user for d in table] == [1000, 1001, 1002]
         assert table[1].history > table[0].history  # a new shape starts a new history
+        c.check(c.L.pyr_instance_reset_history(c.ctx, insts[2]), "reset history")
+        c.check(c.L.pyr_commit(c.ctx, None), "commit")
+        t2 = _instance_table(c, 3)
+        assert t2[2].history > table[1].history and t2[0].history == table[0].history
+        assert np.array_equal(_bits(trace()), _bits(mixed))
+        view = C.c_void_p()
+        c.check(c.L.pyr_view_create(c.ctx, C.byref(view)), "view")
+        c.check(c.L.pyr_view_reset_history(c.ctx, view), "view reset")
+        assert c.L.pyr_view_reset_history(c.ctx, None) == abi.PYR_ERROR_INVALID_HANDLE
 
         # the first geometry stays in use until no instance refers to it
         assert c.L.pyr_geometry_destroy(c.ctx, ga) == abi.PYR_ERROR_IN_USE
