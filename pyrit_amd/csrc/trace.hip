@@ -157,6 +157,12 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
     uint64_t brick = 0;
     bool found = false;
 
+    // One iteration per cell: look the cell up; a filled leaf voxel ends the
+    // walk, a filled larger cell is entered (descend), an empty one is left
+    // along the ray (advance, ascending to the highest changed coordinate
+    // bit). Both successors are formed in straight-line code and the lookup's
+    // result picks one, so a wave whose lanes split between entering and
+    // leaving cells runs one short path instead of two branches.
     for (uint32_t step = 0; step < kMaxSteps; step++) {
         bool exists;
         uint32_t child = 0;
@@ -173,104 +179,86 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
             uint32_t l[3];
 #pragma unroll
             for (int a = 0; a < 3; a++) l[a] = (((oct >> a) & 1u) ? n - size - cell[a] : cell[a]) & 3u;
-            if (scale == 1) exists = (brick & (0x00330033ull << brick_bit(l[0], l[1], l[2]))) != 0;
-            else exists = ((brick >> brick_bit(l[0], l[1], l[2])) & 1ull) != 0;
+            // scale 1: any voxel of the 2 x 2 x 2 block
+            const uint64_t sel = scale ? 0x00330033ull : 1ull;
+            exists = ((brick >> brick_bit(l[0], l[1], l[2])) & sel) != 0;
         }
-        if (exists) {
-            if (scale == 0) {
-                if constexpr (SKIPPING) {
-                    uint32_t v[3];
-                    int32_t vox[3];
+        if (exists && scale == 0) {
+            if constexpr (SKIPPING) {
+                uint32_t v[3];
+                int32_t vox[3];
 #pragma unroll
-                    for (int q = 0; q < 3; q++) {
-                        v[q] = ((oct >> q) & 1u) ? n - 1 - cell[q] : cell[q];
-                        vox[q] = (int32_t)v[q];
-                    }
-                    uint32_t attribute = g.default_attribute;
-                    if ((want_attr || skip || cut) && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
-                    bool through = false;
-                    if (starts_inside && !moved) {
-                        through = skipped(cut, attribute);
-                    } else if (cut) {
-                        float p[3];
-#pragma unroll
-                        for (int q = 0; q < 3; q++) p[q] = o[q] + d[q] * t;
-                        through = cut_hole(cut, attribute, vox, p, axis);
-                    }
-                    if (skipped(skip, attribute) || through) {
-                        // as if the voxel were empty: on to the next cell of edge 1
-                        scale = 0;
-                        float tnext0 = ((float)(cell[0] + 1) - od[0]) * inv[0];
-                        axis = 0;
-#pragma unroll
-                        for (int q = 1; q < 3; q++) {
-                            const float ta = ((float)(cell[q] + 1) - od[q]) * inv[q];
-                            if (ta < tnext0) {
-                                tnext0 = ta;
-                                axis = (uint32_t)q;
-                            }
-                        }
-                        if (!(tnext0 < tend)) break;
-                        t = fmaxf(t, tnext0);
-                        moved = true;
-                        const uint32_t old0 = axis == 0 ? cell[0] : axis == 1 ? cell[1] : cell[2];
-                        const uint32_t next0 = old0 + 1;
-                        if (next0 >= n) break;
-                        if (axis == 0) cell[0] = next0;
-                        else if (axis == 1) cell[1] = next0;
-                        else cell[2] = next0;
-                        const uint32_t h0 = 31 - (uint32_t)__clz(old0 ^ next0);
-                        if (h0 > scale) {
-                            scale = h0;
-                            const uint32_t m0 = ~((1u << scale) - 1);
-#pragma unroll
-                            for (int q = 0; q < 3; q++) cell[q] &= m0;
-                        }
-                        continue;
-                    }
-                    hit->attribute = attribute;
+                for (int q = 0; q < 3; q++) {
+                    v[q] = ((oct >> q) & 1u) ? n - 1 - cell[q] : cell[q];
+                    vox[q] = (int32_t)v[q];
                 }
+                uint32_t attribute = g.default_attribute;
+                if ((want_attr || skip || cut) && g.attrs) attribute = attribute_at(g, attribute_rank(g, stk, v, brick));
+                bool through = false;
+                if (starts_inside && !moved) {
+                    through = skipped(cut, attribute);
+                } else if (cut) {
+                    float p[3];
+#pragma unroll
+                    for (int q = 0; q < 3; q++) p[q] = o[q] + d[q] * t;
+                    through = cut_hole(cut, attribute, vox, p, axis);
+                }
+                // a passed voxel is left like an empty one
+                if (skipped(skip, attribute) || through) {
+                    exists = false;
+                } else {
+                    hit->attribute = attribute;
+                    found = true;
+                    break;
+                }
+            } else {
                 found = true;
                 break;
             }
+        }
+        if (exists) {
             if (scale >= 3) stk[(scale - 1) * kTraceBlock] = child;
             else if (scale == 2) brick = g.leaves[child];
-            scale -= 1;
-            const uint32_t half = 1u << scale;
+        }
+        // leaving the cell: the nearest of its three far faces
+        const uint32_t size = 1u << scale;
+        float tnext = ((float)(cell[0] + size) - od[0]) * inv[0];
+        uint32_t ax = 0;
+#pragma unroll
+        for (int a = 1; a < 3; a++) {
+            const float ta = ((float)(cell[a] + size) - od[a]) * inv[a];
+            if (ta < tnext) {
+                tnext = ta;
+                ax = (uint32_t)a;
+            }
+        }
+        const uint32_t old = ax == 0 ? cell[0] : ax == 1 ? cell[1] : cell[2];
+        const uint32_t next = old + size;
+        if (!exists && (!(tnext < tend) || next >= n)) break;
+        if (exists) {
+            // entering it: the child cell that holds the point at t
+            const uint32_t half = size >> 1;
             const float halff = (float)half;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 const float p = od[a] + dd[a] * t;
                 if (p - (float)cell[a] >= halff) cell[a] += half;
             }
-            continue;
-        }
-        const uint32_t size = 1u << scale;
-        float tnext = ((float)(cell[0] + size) - od[0]) * inv[0];
-        axis = 0;
+            scale -= 1;
+        } else {
+            t = fmaxf(t, tnext);
+            moved = true;
+            axis = ax;
+            if (ax == 0) cell[0] = next;
+            else if (ax == 1) cell[1] = next;
+            else cell[2] = next;
+            const uint32_t h = 31 - (uint32_t)__clz(old ^ next);
+            if (h > scale) {
+                scale = h;
+                const uint32_t m = ~((1u << scale) - 1);
 #pragma unroll
-        for (int a = 1; a < 3; a++) {
-            const float ta = ((float)(cell[a] + size) - od[a]) * inv[a];
-            if (ta < tnext) {
-                tnext = ta;
-                axis = (uint32_t)a;
+                for (int a = 0; a < 3; a++) cell[a] &= m;
             }
-        }
-        if (!(tnext < tend)) break;
-        t = fmaxf(t, tnext);
-        moved = true;
-        const uint32_t old = axis == 0 ? cell[0] : axis == 1 ? cell[1] : cell[2];
-        const uint32_t next = old + size;
-        if (next >= n) break;
-        if (axis == 0) cell[0] = next;
-        else if (axis == 1) cell[1] = next;
-        else cell[2] = next;
-        const uint32_t h = 31 - (uint32_t)__clz(old ^ next);
-        if (h > scale) {
-            scale = h;
-            const uint32_t m = ~((1u << scale) - 1);
-#pragma unroll
-            for (int a = 0; a < 3; a++) cell[a] &= m;
         }
     }
     if (!found) return false;
