@@ -82,6 +82,9 @@ typedef struct PyrhipBuildTuning {
     uint32_t counted_pass;  /* 1 (default): with 8-bit digits the first radix pass takes every tile's output
                                offsets from per-tile digit counts the encode step writes (later passes use the
                                decoupled look-back); 2: decoupled look-back in every pass */
+    uint32_t levels;        /* node levels: 2 one workgroup builds every level in one launch, 1 four launches
+                               per level over the whole GPU, 0 (default): 2 when the build holds at most
+                               32768 bricks (its brick bound), else 1; the output is the same */
 } PyrhipBuildTuning;
 
 /* gpu_build.build (src/gpu_build.zig:94-96): DAG of n_old sorted unique

@@ -113,7 +113,8 @@ class PyrhipChunkBatch(C.Structure):
 
 class PyrhipBuildTuning(C.Structure):
     _fields_ = [("radix_bits", u32), ("sort_items", u32), ("pair_layout", u32), ("scan_items", u32),
-                ("hash_load_pct", u32), ("status_words", u32), ("finish", u32), ("counted_pass", u32)]
+                ("hash_load_pct", u32), ("status_words", u32), ("finish", u32), ("counted_pass", u32),
+                ("levels", u32)]
 
 
 class PyrhipTuneReport(C.Structure):

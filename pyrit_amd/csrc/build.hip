@@ -1397,6 +1397,91 @@ struct LevelArgs {
     uint32_t load_pct;        // maximum load of the tuple hash in percent
 };
 
+// Child idx of a level (key k) heads a parent when the parent key k >> 3
+// changes there (gbuild.zig:475-480); a chunk whose edge is 2^lv ends at this
+// level instead: its only element is the chunk's root, recorded in roots.
+__device__ __forceinline__ uint32_t level_head(const LevelArgs& a, uint64_t idx, uint64_t k) {
+    if (a.chunk_log2) {
+        const uint32_t c = (uint32_t)(k >> a.top_shift);
+        if (c < a.chunk_count && a.chunk_log2[c] == a.lv) {
+            a.roots[c] = a.ch.pos[a.ch.rep[idx]];
+            return 0;
+        }
+    }
+    return idx == 0 || (a.ch.key[idx - 1] >> 3) != (k >> 3);
+}
+
+// node_build (gbuild.zig:482-520): the head at idx gathers its (at most 8)
+// children into parent p's tuple [mask, count, child positions]; their keys,
+// then reps and counts, then positions are each loaded together
+__device__ __forceinline__ void level_tuple(const LevelArgs& a, uint64_t idx, uint32_t n, uint64_t k, uint32_t p) {
+    const uint64_t pk = k >> 3;
+    uint32_t w[kNodeStride];
+    uint32_t mask = 0, count = 0, len = 2;
+    uint64_t ck[8];
+    uint32_t nkids = 0;
+#pragma unroll
+    for (int q = 0; q < 8; q++) ck[q] = idx + q < n ? a.ch.key[idx + q] : ~0ull;
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+        if (nkids == (uint32_t)q && (ck[q] >> 3) == pk) nkids++;
+    uint32_t rep[8], cnt[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        rep[q] = (uint32_t)q < nkids ? a.ch.rep[idx + q] : 0u;
+        cnt[q] = (uint32_t)q < nkids ? a.ch.count[idx + q] : 0u;
+    }
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+        if ((uint32_t)q < nkids) {
+            mask |= 1u << (uint32_t)(ck[q] & 7);
+            w[len++] = a.ch.pos[rep[q]];
+            count = sat_add(count, cnt[q]);
+        }
+    }
+    w[0] = mask;
+    w[1] = count;
+    uint32_t* dst = a.node_tmp + (uint64_t)p * kNodeStride;
+    for (uint32_t q = 0; q < len; q++) dst[q] = w[q];
+    a.node_len[p] = len;
+    a.par.key[p] = pk;
+    a.par.count[p] = count;
+}
+
+// node_insert/node_owner (gbuild.zig:527-554) of parent p's tuple w (len
+// words): the level's hash holds index + 1; a collision compares the stored
+// tuple word by word; the owner of a slot is the smallest index. Returns the
+// slot.
+__device__ __forceinline__ uint64_t level_insert_one(const LevelArgs& a, uint32_t p, const uint32_t* w, uint32_t len,
+                                                     uint64_t cap, uint32_t* err) {
+    const uint64_t hm = cap - 1;
+    uint64_t hv = len;
+    for (uint32_t q = 0; q < len; q++) hv = mix64(hv ^ w[q]);
+    uint64_t h = hv & hm;
+    uint64_t probes = 0;
+    for (;;) {
+        uint32_t cur = load_agent(&a.table[h]);
+        if (cur == 0) {
+            cur = atomicCAS(&a.table[h], 0u, p + 1);
+            if (cur == 0) break;
+        }
+        const uint32_t other = cur - 1;
+        if (a.node_len[other] == len) {
+            const uint32_t* ow = a.node_tmp + (uint64_t)other * kNodeStride;
+            bool same = true;
+            for (uint32_t q = 0; q < len; q++) same &= ow[q] == w[q];
+            if (same) break;
+        }
+        h = (h + 1) & hm;
+        if (++probes > cap) {
+            atomicOr(err, 8u);
+            break;
+        }
+    }
+    owner_min(a.owner, h, p);
+    return h;
+}
+
 // node_heads/node_build/node_insert/node_owner (gbuild.zig:475-550) in one
 // pass: parents are numbered by a tile scan of the head flags, each head
 // gathers its <= 8 children, and the tuple is inserted into the level's hash
@@ -1416,16 +1501,7 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
             f[i] = 0;
             if (idx < n) {
                 k[i] = a.ch.key[idx];
-                bool finished = false;
-                if (a.chunk_log2) {
-                    // a chunk whose edge is 2^lv ends here: this element is its root
-                    uint32_t c = (uint32_t)(k[i] >> a.top_shift);
-                    if (c < a.chunk_count && a.chunk_log2[c] == a.lv) {
-                        finished = true;
-                        a.roots[c] = a.ch.pos[a.ch.rep[idx]];
-                    }
-                }
-                f[i] = !finished && (idx == 0 || (a.ch.key[idx - 1] >> 3) != (k[i] >> 3));
+                f[i] = level_head(a, idx, k[i]);
             }
         }
         uint32_t total = tile_scan<kScanItems>(f, e, s_wave);
@@ -1438,40 +1514,7 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
                 atomicOr(&sc.counters[C_ERR], 16u);
                 continue;
             }
-            const uint32_t p = (uint32_t)(pre + e[i]);
-            const uint64_t pk = k[i] >> 3;
-            uint32_t w[kNodeStride];
-            uint32_t mask = 0, count = 0, len = 2;
-            // the (at most 8) children follow the head: their keys, then reps
-            // and counts, then positions, each round's loads issued together
-            uint64_t ck[8];
-            uint32_t nkids = 0;
-#pragma unroll
-            for (int q = 0; q < 8; q++) ck[q] = idx + q < n ? a.ch.key[idx + q] : ~0ull;
-#pragma unroll
-            for (int q = 0; q < 8; q++)
-                if (nkids == (uint32_t)q && (ck[q] >> 3) == pk) nkids++;
-            uint32_t rep[8], cnt[8];
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                rep[q] = (uint32_t)q < nkids ? a.ch.rep[idx + q] : 0u;
-                cnt[q] = (uint32_t)q < nkids ? a.ch.count[idx + q] : 0u;
-            }
-#pragma unroll
-            for (int q = 0; q < 8; q++) {
-                if ((uint32_t)q < nkids) {
-                    mask |= 1u << (uint32_t)(ck[q] & 7);
-                    w[len++] = a.ch.pos[rep[q]];
-                    count = sat_add(count, cnt[q]);
-                }
-            }
-            w[0] = mask;
-            w[1] = count;
-            uint32_t* dst = a.node_tmp + (uint64_t)p * kNodeStride;
-            for (uint32_t q = 0; q < len; q++) dst[q] = w[q];
-            a.node_len[p] = len;
-            a.par.key[p] = pk;
-            a.par.count[p] = count;
+            level_tuple(a, idx, n, k[i], (uint32_t)(pre + e[i]));
         }
         if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_NCH + a.lv + 1] = (uint32_t)(pre + total);
     }
@@ -1485,7 +1528,7 @@ __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, Lev
     const uint32_t np = min(counters[C_NCH + a.lv + 1], a.par_cap);
     const uint32_t nch = min(counters[C_NCH + a.lv], a.ch_cap);
     const uint64_t want = hash_cap_for(nch, a.load_pct);
-    const uint64_t cap = want < a.table_cap ? want : a.table_cap, hm = cap - 1;
+    const uint64_t cap = want < a.table_cap ? want : a.table_cap;
     const uint32_t stride = gridDim.x * kBlock;
     for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) {
         const uint32_t len = a.node_len[p];
@@ -1500,32 +1543,7 @@ __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, Lev
         for (int q = 0; q < kNodeStride; q++) same0 &= first_lane32(w[q]) == w[q];
         const bool follower = same0 && p != p0;
         uint64_t h = 0;
-        if (!follower) {
-            uint64_t hv = len;
-            for (uint32_t q = 0; q < len; q++) hv = mix64(hv ^ w[q]);
-            h = hv & hm;
-            uint64_t probes = 0;
-            for (;;) {
-                uint32_t cur = load_agent(&a.table[h]);
-                if (cur == 0) {
-                    cur = atomicCAS(&a.table[h], 0u, p + 1);
-                    if (cur == 0) break;
-                }
-                const uint32_t other = cur - 1;
-                if (a.node_len[other] == len) {
-                    const uint32_t* ow = a.node_tmp + (uint64_t)other * kNodeStride;
-                    bool same = true;
-                    for (uint32_t q = 0; q < len; q++) same &= ow[q] == w[q];
-                    if (same) break;
-                }
-                h = (h + 1) & hm;
-                if (++probes > cap) {
-                    atomicOr(&counters[C_ERR], 8u);
-                    break;
-                }
-            }
-            owner_min(a.owner, h, p);
-        }
+        if (!follower) h = level_insert_one(a, p, w, len, cap, &counters[C_ERR]);
         const uint64_t h0 = first_lane64(h);  // the slot of the wave's first active lane
         if (follower) h = h0;
         a.slot[p] = (uint32_t)h;
@@ -1573,6 +1591,122 @@ __global__ void __launch_bounds__(kBlock) k_level_compact(ScanCtx sc, LevelArgs 
             }
         }
         if (tile == ntiles - 1 && threadIdx.x == 0) sc.counters[C_CUR + a.lv + 1] = cur + (uint32_t)(pre + total);
+    }
+}
+
+// Every node level of a small build in one workgroup (1024 threads) and one
+// launch: per level the hash is cleared, heads are numbered by a block scan
+// carried from round to round, tuples are gathered and inserted, and the
+// owners' words are appended at the level cursor, the same arrays and
+// counters as k_hash_clear, k_level_build, k_level_insert and
+// k_level_compact write, with barriers instead of launches and look-backs.
+// `a` describes the first level (children of edge 4 in set ch, parents in
+// par); the sets swap from level to level like the launches' arguments.
+constexpr uint32_t kLevelsBlock = 1024;
+constexpr uint64_t kLevelsSmallBricks = 32768;
+
+__device__ __forceinline__ uint32_t levels_scan(uint32_t v, uint32_t* s_wave, uint32_t* total) {
+    constexpr uint32_t W = kLevelsBlock / kWave;
+    const uint32_t lane = lane_id(), wave = threadIdx.x / kWave;
+    const uint32_t inc = wave_inclusive_scan(v);
+    if (lane == kWave - 1) s_wave[wave] = inc;
+    __syncthreads();
+    uint32_t before = 0, t = 0;
+#pragma unroll
+    for (uint32_t q = 0; q < W; q++) {
+        const uint32_t x = s_wave[q];
+        before += q < wave ? x : 0u;
+        t += x;
+    }
+    __syncthreads();  // s_wave is written again by the next round
+    *total = t;
+    return before + inc - v;
+}
+
+__global__ void __launch_bounds__(kLevelsBlock) k_levels_small(uint32_t* counters, LevelArgs a, uint32_t log2_size,
+                                                               uint32_t cap0, uint32_t cap1, uint32_t par_max,
+                                                               uint32_t* nodes_out) {
+    __shared__ uint32_t s_wave[kLevelsBlock / kWave];
+    const uint32_t tid = threadIdx.x;
+    uint32_t caps[2] = {cap0, cap1};
+    int ch = 0;
+    for (uint32_t lv = 2; lv < log2_size; lv++) {
+        a.lv = lv;
+        a.top_shift = 3 * (log2_size - lv);
+        a.ch_cap = caps[ch];
+        a.par_cap = min(caps[ch ^ 1], par_max);
+        const uint32_t nch = min(counters[C_NCH + lv], a.ch_cap);
+        // hash of this level (k_hash_clear)
+        const uint64_t want = hash_cap_for(nch, a.load_pct);
+        const uint64_t cap = want < a.table_cap ? want : a.table_cap;
+        for (uint64_t i = tid; i < cap; i += kLevelsBlock) {
+            a.table[i] = 0;
+            a.owner[i] = 0xFFFFFFFFu;
+        }
+        // heads and tuples (k_level_build)
+        uint32_t np = 0;
+        for (uint32_t base = 0; base < nch; base += kLevelsBlock) {
+            const uint32_t idx = base + tid;
+            uint64_t k = 0;
+            uint32_t f = 0;
+            if (idx < nch) {
+                k = a.ch.key[idx];
+                f = level_head(a, idx, k);
+            }
+            uint32_t total;
+            const uint32_t e = levels_scan(f, s_wave, &total);
+            if (f) {
+                if (np + e >= a.par_cap) atomicOr(&counters[C_ERR], 16u);
+                else level_tuple(a, idx, nch, k, np + e);
+            }
+            np += total;
+        }
+        __syncthreads();  // tuples and the cleared hash are complete
+        if (tid == 0) counters[C_NCH + lv + 1] = np;
+        np = min(np, a.par_cap);
+        // insert (k_level_insert)
+        for (uint32_t p = tid; p < np; p += kLevelsBlock) {
+            const uint32_t len = a.node_len[p];
+            const uint32_t* src = a.node_tmp + (uint64_t)p * kNodeStride;
+            uint32_t w[kNodeStride];
+#pragma unroll
+            for (int q = 0; q < kNodeStride; q++) w[q] = (uint32_t)q < len ? src[q] : 0u;
+            a.slot[p] = (uint32_t)level_insert_one(a, p, w, len, cap, &counters[C_ERR]);
+        }
+        __syncthreads();  // every owner is final
+        // owners' words at the level cursor (k_level_compact)
+        const uint32_t cur = counters[C_CUR + lv];
+        uint32_t at0 = 0;
+        for (uint32_t base = 0; base < np; base += kLevelsBlock) {
+            const uint32_t idx = base + tid;
+            uint32_t f = 0, o = 0;
+            if (idx < np) {
+                o = a.owner[a.slot[idx]];
+                f = o == idx ? a.node_len[idx] : 0u;
+            }
+            uint32_t total;
+            const uint32_t e = levels_scan(f, s_wave, &total);
+            if (idx < np) {
+                a.par.rep[idx] = o;
+                if (f) {
+                    const uint32_t at = cur + at0 + e;
+                    if ((uint64_t)at + f > a.word_cap) {
+                        atomicOr(&counters[C_ERR], 16u);
+                    } else {
+                        const uint32_t* src = a.node_tmp + (uint64_t)idx * kNodeStride;
+                        for (uint32_t q = 0; q < f; q++) nodes_out[at + q] = src[q];
+                        a.par.pos[idx] = at;
+                    }
+                }
+            }
+            at0 += total;
+        }
+        if (tid == 0) counters[C_CUR + lv + 1] = cur + at0;
+        __syncthreads();  // the next level reads this one's counters and parents
+        const LevelArrays t = a.ch;
+        a.ch = a.par;
+        a.par = t;
+        ch ^= 1;
     }
 }
 
@@ -1873,6 +2007,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (tun.status_words < 1 || tun.status_words > 2)
         return fail(PYR_ERROR_INVALID_ARGUMENT, "status_words must be 1 or 2");
     if (tun.counted_pass > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "counted_pass must be 1 or 2");
+    if (tun.levels > 2) return fail(PYR_ERROR_INVALID_ARGUMENT, "levels must be 0, 1 or 2");
     const uint32_t lbits = local_bits(log2_size);
     // records sort by chunk-local key inside their chunk's segment: the key
     // is the local bits plus the invalid bit; the chunk index comes back into
@@ -2185,10 +2320,31 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                            leaves_out, set[0].rep, set[0].pos, (uint32_t)lz.nb);
     }
     tm.mark();
-    // 5. node levels
+    // 5. node levels: one workgroup and one launch for small builds, else
+    // four launches per level
     int ch = 0;
     const uint64_t set_cap[2] = {lz.nb, lz.np2};
-    for (uint32_t lv = 2; lv < log2_size; lv++) {
+    const bool small_levels = tun.levels == 2 || (tun.levels == 0 && lz.nb <= kLevelsSmallBricks);
+    if (small_levels && log2_size > 2) {
+        LevelArgs la{};
+        la.chunk_log2 = enc.chunk_log2;
+        la.roots = roots_dev;
+        la.ch = set[0];
+        la.par = set[1];
+        la.node_tmp = node_tmp;
+        la.node_len = node_len;
+        la.table = (uint32_t*)table;
+        la.owner = owner;
+        la.slot = slot;
+        la.table_cap = lz.cap;
+        la.word_cap = lz.words;
+        la.chunk_count = enc.chunk_count;
+        la.load_pct = load_pct;
+        hipLaunchKernelGGL(k_levels_small, dim3(1), dim3(kLevelsBlock), 0, s, counters, la, log2_size,
+                           (uint32_t)set_cap[0], (uint32_t)set_cap[1], (uint32_t)lz.np2, nodes_out);
+        ch = (log2_size - 2) & 1;
+    }
+    for (uint32_t lv = 2; lv < log2_size && !small_levels; lv++) {
         LevelArgs la{};
         la.lv = lv;
         la.top_shift = 3 * (log2_size - lv);

@@ -23,7 +23,7 @@
 namespace pyr {
 namespace {
 
-constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2, 0, 1};
+constexpr PyrhipBuildTuning kDefaults = {8, 20, 2, 12, 80, 2, 0, 1, 0};
 
 struct Table {
     std::mutex m;
@@ -57,12 +57,13 @@ void overlay(PyrhipBuildTuning& t, const PyrhipBuildTuning& o) {
     if (o.status_words) t.status_words = o.status_words;
     if (o.finish) t.finish = o.finish;
     if (o.counted_pass) t.counted_pass = o.counted_pass;
+    if (o.levels) t.levels = o.levels;
 }
 
 bool same_tuning(const PyrhipBuildTuning& a, const PyrhipBuildTuning& b) {
     return a.radix_bits == b.radix_bits && a.sort_items == b.sort_items && a.pair_layout == b.pair_layout &&
            a.scan_items == b.scan_items && a.hash_load_pct == b.hash_load_pct && a.status_words == b.status_words &&
-           a.finish == b.finish && a.counted_pass == b.counted_pass;
+           a.finish == b.finish && a.counted_pass == b.counted_pass && a.levels == b.levels;
 }
 
 // what a build produced, enough to tell two settings' outputs apart

@@ -59,7 +59,7 @@ class Built:
 
 def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
             hash_load_pct: int = 0, status_words: int = 0, finish: int = 0,
-            counted_pass: int = 0) -> abi.PyrhipBuildTuning:
+            counted_pass: int = 0, levels: int = 0) -> abi.PyrhipBuildTuning:
     """0 = the setting installed for the workload class, else the default;
     pair_layout 1 = separate key / attribute arrays, 2 = packed when keys fit"""
     t = abi.PyrhipBuildTuning()
@@ -68,6 +68,7 @@ def _tuning(radix_bits: int, sort_items: int = 0, pair_layout: int = 0, scan_ite
     t.status_words = status_words
     t.finish = finish
     t.counted_pass = counted_pass
+    t.levels = levels
     return t
 
 
@@ -81,7 +82,8 @@ def _ptr(x) -> int:
 
 def build(log2_size: int, voxels, count: int | None = None, old: Built | None = None, radix_bits: int = 0,
           stream: int | None = None, sort_items: int = 0, pair_layout: int = 0, scan_items: int = 0,
-          hash_load_pct: int = 0, status_words: int = 0, finish: int = 0, counted_pass: int = 0) -> Built:
+          hash_load_pct: int = 0, status_words: int = 0, finish: int = 0, counted_pass: int = 0,
+          levels: int = 0) -> Built:
     """Build a DAG from PyrVoxel records (numpy VOXEL array or device Buffer).
     `old` supplies the sorted voxels of an earlier editable build (edits)."""
     keep = None
@@ -94,7 +96,8 @@ def build(log2_size: int, voxels, count: int | None = None, old: Built | None = 
         vptr = _ptr(voxels)
         assert count is not None
     out = abi.PyrhipBuilt()
-    t = _tuning(radix_bits, sort_items, pair_layout, scan_items, hash_load_pct, status_words, finish, counted_pass)
+    t = _tuning(radix_bits, sort_items, pair_layout, scan_items, hash_load_pct, status_words, finish, counted_pass,
+                levels)
     L = lib()
     code = L.pyrhip_build(log2_size, old.raw.keys if old else 0, old.raw.attrs if old else 0,
                           old.voxel_count if old else 0, vptr, count, C.byref(t), stream, C.byref(out))
@@ -129,7 +132,7 @@ def _batch(voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None, capacity:
 
 def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.ndarray | None = None,
                  capacity: int = 0, radix_bits: int = 0, stream: int | None = None, hash_load_pct: int = 0,
-                 status_words: int = 0, finish: int = 0, counted_pass: int = 0):
+                 status_words: int = 0, finish: int = 0, counted_pass: int = 0, levels: int = 0):
     """Chunk batch (src/gpu_build.zig:98-122). voxels: VOXEL array or device
     Buffer; offsets: u32[count+1] host array. Returns (Built, roots, first)."""
     batch, keep = _batch(voxels, offsets, chunk_log2, capacity)
@@ -138,7 +141,7 @@ def build_chunks(log2_size: int, voxels, offsets: np.ndarray, chunk_log2: np.nda
     first = np.zeros(count, np.uint32)
     out = abi.PyrhipBuilt()
     t = _tuning(radix_bits, hash_load_pct=hash_load_pct, status_words=status_words, finish=finish,
-                counted_pass=counted_pass)
+                counted_pass=counted_pass, levels=levels)
     L = lib()
     code = L.pyrhip_build_chunks(log2_size, C.byref(batch), C.byref(t), stream, C.byref(out),
                                  roots.ctypes.data_as(C.POINTER(C.c_uint32)),

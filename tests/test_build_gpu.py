@@ -166,6 +166,39 @@ def test_counted_and_lookback_first_pass_identical(hip_device, counted_pass):
     assert np.array_equal(roots, rr) and np.array_equal(first, rf)
 
 
+@pytest.mark.parametrize("levels", [1, 2])
+def test_level_paths_identical(hip_device, levels):
+    """Node levels built by four launches per level over the GPU (1) or by one
+    workgroup in one launch (2): single builds from 2^3 to 2^12 with
+    duplicates, deletes and out-of-range records, an edit, an empty build,
+    a batch whose chunks end at different levels (roots recorded on the way)
+    and a dense solid grid (hot tuples), against the oracle."""
+    from pyrit_amd import workloads
+
+    g = _gpu()
+    print(f"case: levels {levels}, single builds", flush=True)
+    for log2, n, seed in [(3, 40, 1), (5, 3000, 2), (7, 40000, 3), (9, 120000, 4), (12, 60000, 5)]:
+        pts = random_points(n, log2, seed, dup_frac=0.2, delete_frac=0.05, out_frac=0.02)
+        ref = O.gb_build(log2, pts)
+        b = g.build(log2, pts, levels=levels)
+        assert_same_build(b, ref, log2)
+    ed = random_points(5000, 12, 6, dup_frac=0.2, delete_frac=0.3)
+    assert_same_build(g.build(12, ed, old=b, levels=levels), O.gb_build(12, ed, ref["keys"], ref["attrs"]), 12)
+    empty = np.zeros(0, O.VOXEL)
+    assert_same_build(g.build(6, empty, levels=levels), O.gb_build(6, empty), 6)
+    solid = grid_to_points(np.ones((32, 32, 32), np.uint32) * 3)
+    assert_same_build(g.build(5, solid, levels=levels), O.gb_build(5, solid), 5)
+    print(f"case: levels {levels}, batch of mixed sizes", flush=True)
+    w = workloads.uneven_chunks(seed=97, count=40, edges=(8, 16, 32, 64))
+    vox, offs64 = w.generate_host()
+    offs = offs64.astype(np.uint32)
+    sizes = np.array([c.log2_size for c in w.chunks], np.uint8)
+    rb_, rr, rf = O.gb_build_chunks(6, vox, offs, chunk_log2=sizes)
+    bb, roots, first = g.build_chunks(6, vox, offs, chunk_log2=sizes, levels=levels)
+    assert_same_build(bb, rb_, 6)
+    assert np.array_equal(roots, rr) and np.array_equal(first, rf)
+
+
 def test_build_memory_per_voxel(hip_device):
     """The reference's bound on the build's device memory (tests/cpu_test.zig:
     1559-1629): every buffer the build requests, scratch and outputs, held
@@ -323,7 +356,7 @@ def test_autotune_installs_an_identical_setting(hip_device):
     print("case: autotune single build and batch", flush=True)
     g = _gpu()
     default = {"radix_bits": 8, "sort_items": 20, "pair_layout": 2, "scan_items": 12, "hash_load_pct": 80,
-               "status_words": 2, "finish": 0, "counted_pass": 1}
+               "status_words": 2, "finish": 0, "counted_pass": 1, "levels": 0}
     g.tuning_clear()
     try:
         pts = random_points(300000, 10, 41, dup_frac=0.1, delete_frac=0.02)
