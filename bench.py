@@ -257,7 +257,8 @@ def trace_section(args, out):
         out["traversal"]["roofline"] = {
             "kernel": "k_trace_camera", "bound": "hbm", "bytes_per_ray": per_ray, "achieved": achieved,
             "peak": HBM_PEAK_GBPS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBPS,
-            "note": "distinct DAG bytes per ray; most of them are served by L2 / MALL, not HBM"}
+            "note": "distinct DAG bytes per ray; almost all of them hit the per-CU L1, and the kernel is bound "
+                    "by vector instruction issue (VALU busy most of its cycles), not by these bytes"}
         log(f"traversal: {per_ray:.0f} algorithmic bytes per primary ray, {achieved:.0f} GB/s")
     L.pyrhip_built_free(C.byref(built), None)
     for b in (hits, rays, occ):
