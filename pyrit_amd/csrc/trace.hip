@@ -144,6 +144,11 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
     const float tend = fminf(tf, tmax);
     if (!(t < tend)) return false;
     const bool starts_inside = tn < tmin;
+    // brick bit of a mirrored cell: its low two bits per axis, flipped on the
+    // mirrored axes (3 - x for a voxel, 2 - x for a 2 x 2 x 2 block; n is a
+    // multiple of 4)
+    const uint32_t flip1 = (oct & 1u ? 3u : 0u) | (oct & 2u ? 12u : 0u) | (oct & 4u ? 48u : 0u);
+    const uint32_t flip2 = flip1 & 0x2Au;
     bool moved = false;
 
     uint32_t scale = g.log2_size - 1;
@@ -175,13 +180,10 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
             exists = (mask & bit) != 0;
             if (exists) child = g.nodes[node + 2 + __popc(mask & (bit - 1))];
         } else {
-            const uint32_t size = 1u << scale;
-            uint32_t l[3];
-#pragma unroll
-            for (int a = 0; a < 3; a++) l[a] = (((oct >> a) & 1u) ? n - size - cell[a] : cell[a]) & 3u;
+            const uint32_t b = brick_bit(cell[0] & 3u, cell[1] & 3u, cell[2] & 3u) ^ (scale ? flip2 : flip1);
             // scale 1: any voxel of the 2 x 2 x 2 block
             const uint64_t sel = scale ? 0x00330033ull : 1ull;
-            exists = ((brick >> brick_bit(l[0], l[1], l[2])) & sel) != 0;
+            exists = ((brick >> b) & sel) != 0;
         }
         if (exists && scale == 0) {
             if constexpr (SKIPPING) {
@@ -222,18 +224,19 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
         }
         // leaving the cell: the nearest of its three far faces
         const uint32_t size = 1u << scale;
-        float tnext = ((float)(cell[0] + size) - od[0]) * inv[0];
-        uint32_t ax = 0;
-#pragma unroll
-        for (int a = 1; a < 3; a++) {
-            const float ta = ((float)(cell[a] + size) - od[a]) * inv[a];
-            if (ta < tnext) {
-                tnext = ta;
-                ax = (uint32_t)a;
-            }
-        }
-        const uint32_t old = ax == 0 ? cell[0] : ax == 1 ? cell[1] : cell[2];
-        const uint32_t next = old + size;
+        const uint32_t n0 = cell[0] + size, n1 = cell[1] + size, n2 = cell[2] + size;
+        const float t0 = ((float)n0 - od[0]) * inv[0];
+        const float t1 = ((float)n1 - od[1]) * inv[1];
+        const float t2 = ((float)n2 - od[2]) * inv[2];
+        // the first axis with the smallest exit (ties to the lower axis)
+        const bool a1 = t1 < t0;
+        const float t01 = a1 ? t1 : t0;
+        const bool s2 = t2 < t01;
+        const bool s1 = a1 && !s2, s0 = !a1 && !s2;
+        const float tnext = s2 ? t2 : t01;
+        const uint32_t ax = s2 ? 2u : (a1 ? 1u : 0u);
+        const uint32_t old = s2 ? cell[2] : (a1 ? cell[1] : cell[0]);
+        const uint32_t next = s2 ? n2 : (a1 ? n1 : n0);
         if (!exists && (!(tnext < tend) || next >= n)) break;
         if (exists) {
             // entering it: the child cell that holds the point at t
@@ -249,9 +252,9 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
             t = fmaxf(t, tnext);
             moved = true;
             axis = ax;
-            if (ax == 0) cell[0] = next;
-            else if (ax == 1) cell[1] = next;
-            else cell[2] = next;
+            cell[0] = s0 ? n0 : cell[0];
+            cell[1] = s1 ? n1 : cell[1];
+            cell[2] = s2 ? n2 : cell[2];
             const uint32_t h = 31 - (uint32_t)__clz(old ^ next);
             if (h > scale) {
                 scale = h;
