@@ -174,8 +174,8 @@ __device__ bool walk(const DagView& g, const float o[3], const float d[3], float
         if (scale >= 2) {
             const uint32_t node = stk[scale * kTraceBlock];
             const uint32_t mask = g.nodes[node] & 0xFFu;
-            const uint32_t idx =
-                (bit_of(cell[0], scale) | (bit_of(cell[1], scale) << 1) | (bit_of(cell[2], scale) << 2)) ^ oct;
+            const uint32_t idx = (__builtin_amdgcn_ubfe(cell[0], scale, 1) | (__builtin_amdgcn_ubfe(cell[1], scale, 1) << 1) |
+                                  (__builtin_amdgcn_ubfe(cell[2], scale, 1) << 2)) ^ oct;
             const uint32_t bit = 1u << idx;
             exists = (mask & bit) != 0;
             if (exists) child = g.nodes[node + 2 + __popc(mask & (bit - 1))];
