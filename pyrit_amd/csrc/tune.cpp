@@ -227,7 +227,7 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
         // (11-bit digits are left out: 2048 look-back words per tile made a
         // pass ten times slower than 8-bit digits on every workload measured)
         const uint32_t layouts[][2] = {{8, 2}, {8, 1}, {10, 2}, {9, 2}};
-        for (int dim = 0; dim < 7; dim++) {
+        for (int dim = 0; dim < 8; dim++) {
             for (int v = 0; v < (dim >= 4 ? 2 : 4); v++) {
                 PyrhipBuildTuning c = best;
                 if (dim == 0) c.sort_items = sorts[v];
@@ -237,7 +237,13 @@ PYR_API PyrResult pyrhip_autotune(uint32_t log2_size, uint64_t voxels, uint32_t 
                 if (dim == 4) c.status_words = 2 - v;
                 if (dim == 5) c.finish = 2 - v;
                 if (dim == 6) c.counted_pass = 1 + v;
+                if (dim == 7) c.levels = 1 + v;
                 if (same_tuning(c, best)) continue;
+                // the level path only matters to small builds: one workgroup
+                // for every node level pays below a few tens of thousands of
+                // bricks (at most the records), far above it a build would
+                // take seconds, and there the default is the launch path anyway
+                if (dim == 7 && count_of(w) > (1u << 17)) continue;
                 // the scan tile only matters to the unique scan, not to the bucket finish
                 if (dim == 1 && bucket_finish(best, log2_size, count_of(w), chunks_of(w))) continue;
                 float ms = 0;
