@@ -194,7 +194,7 @@ Instance* inst_of(PyrContext* ctx, PyrInstance in) {
 
 PyrResult hip_check(hipError_t e, const char* what) {
     if (e == hipSuccess) return PYR_OK;
-    hipGetLastError();
+    (void)hipGetLastError();
     return fail(PYR_ERROR_CUDA, "%s: %s", what, hipGetErrorString(e));
 }
 
@@ -620,7 +620,7 @@ PYR_API PyrResult pyr_create(const PyrCreateInfo* info, PyrContext** out) {
     if ((info->version >> 16) != PYR_VERSION_MAJOR) return fail(PYR_ERROR_VERSION, "incompatible PYR_VERSION");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(PYR_ERROR_NOT_FOUND, "no HIP device");
     }
     if (info->device < 0 || info->device >= n) return fail(PYR_ERROR_INVALID_ARGUMENT, "device %d", info->device);

@@ -67,8 +67,6 @@ enum : uint32_t {
 };
 constexpr uint32_t kMaxLaunches = C_COUNT - C_TILE;
 
-constexpr int kSortItems = 16;                    // keys per thread in a sort tile
-constexpr int kSortTile = kBlock * kSortItems;    // 4096
 constexpr int kScanItems = 8;                     // elements per thread in a scan tile
 constexpr int kScanTile = kBlock * kScanItems;    // 2048
 constexpr int kNodeStride = 10;                   // words per parent in node_tmp (gbuild.zig:66)
@@ -1870,17 +1868,24 @@ struct Timer {
     bool on = g_timing;
     void mark() {
         if (!on) return;
-        hipEventCreate(&ev[n]);
-        hipEventRecord(ev[n], s);
+        if (hipEventCreate(&ev[n]) != hipSuccess) {  // timing is best effort: stop marking
+            (void)hipGetLastError();
+            on = false;
+            return;
+        }
+        (void)hipEventRecord(ev[n], s);
         n++;
     }
     float between(int a, int b) {
         float ms = 0;
-        if (on && b < n) hipEventElapsedTime(&ms, ev[a], ev[b]);
+        if (on && b < n && hipEventElapsedTime(&ms, ev[a], ev[b]) != hipSuccess) {
+            (void)hipGetLastError();
+            ms = 0;
+        }
         return ms;
     }
     ~Timer() {
-        for (int i = 0; i < n; i++) hipEventDestroy(ev[i]);
+        for (int i = 0; i < n; i++) (void)hipEventDestroy(ev[i]);
     }
 };
 
@@ -2187,7 +2192,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     if (A.failed || !out_ok) {
         A.release();
         free_outputs();
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(PYR_ERROR_OUT_OF_MEMORY, "device memory for a build of %u voxels", n);
     }
 

@@ -226,7 +226,7 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         *input = nullptr;
         *input_cap = 0;
         if (pool_alloc(input, need, s) != hipSuccess) {
-            hipGetLastError();
+            (void)hipGetLastError();
             return fail(PYR_ERROR_OUT_OF_MEMORY, "server batch input of %llu records", (unsigned long long)total);
         }
         *input_cap = need;
@@ -330,7 +330,7 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         if (!graph) pyrhip_built_free(&b, s);  // zeroes b
         if (ok && hipStreamSynchronize(s) != hipSuccess) ok = false;
         if (!ok) {
-            hipGetLastError();
+            (void)hipGetLastError();
             pool_free(g->nodes, s);
             pool_free(g->leaves, s);
             pool_free(g->keys, s);
@@ -440,7 +440,7 @@ PYR_API PyrResult pyrhip_server_create(const PyrhipServerInfo* info, PyrhipServe
     if (!out) return fail(PYR_ERROR_INVALID_ARGUMENT, "null argument");
     int n = 0;
     if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
-        hipGetLastError();
+        (void)hipGetLastError();
         return fail(PYR_ERROR_NOT_FOUND, "no HIP device");
     }
     PyrhipServer* sv = new PyrhipServer();
