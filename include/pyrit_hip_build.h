@@ -400,6 +400,10 @@ PYR_API PyrResult pyrhip_server_chunk(PyrhipServer* server, uint64_t chunk, Pyrh
                                       uint32_t* node_words, uint32_t* leaf_count);
 PYR_API double pyrhip_server_clock(void); /* the steady clock of the completion time stamps */
 PYR_API void pyrhip_server_stats(PyrhipServer* server, PyrhipServerStats* out);
+/* message of the last failed batch or rejected request ("" if none) into out
+ * (size bytes, truncated); returns the message's length. A worker's batch
+ * fails on its own thread, so pyr_error_message of the caller does not hold it. */
+PYR_API uint32_t pyrhip_server_error(PyrhipServer* server, char* out, uint32_t size);
 PYR_API void pyrhip_server_destroy(PyrhipServer* server);
 
 #ifdef __cplusplus

@@ -303,6 +303,7 @@ FUNCTIONS = {
     "pyrhip_server_chunk": (i32, [vp, u64, P(PyrhipDag), P(u32), P(u32), P(u32)]),
     "pyrhip_server_clock": (f64, []),
     "pyrhip_server_stats": (None, [vp, P(PyrhipServerStats)]),
+    "pyrhip_server_error": (u32, [vp, C.c_char_p, u32]),
     "pyrhip_server_destroy": (None, [vp]),
     "pyr_world_create": (i32, [vp, P(PyrWorldInfo), P(vp)]),
     "pyr_world_destroy": (i32, [vp, vp]),

@@ -23,10 +23,16 @@ struct PyrDag {
 namespace pyr {
 
 // Per-thread error message (the reference's diag.zig:17-35): cleared at every
-// API entry, set by fail(), read by pyr_error_message().
+// API entry, set by fail(), read by pyr_error_message(). The entry also clears
+// the thread's HIP error state: launches are checked with hipGetLastError,
+// which would otherwise report an earlier failed call of the thread (one
+// whose result was already handled) as the launch's own error.
 extern thread_local char g_diag[512];
 
-inline void diag_clear() { g_diag[0] = 0; }
+inline void diag_clear() {
+    g_diag[0] = 0;
+    (void)hipGetLastError();
+}
 
 inline PyrResult fail(PyrResult code, const char* fmt, ...) {
     va_list ap;

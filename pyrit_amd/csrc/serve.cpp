@@ -88,7 +88,7 @@ struct PyrhipServer {
     PyrhipServerInfo info{};
     int device = 0;
     std::mutex m;
-    // graph captures exclusive of every worker's graph replays
+    // graph captures (held exclusively) against every batch's device work (held shared)
     std::shared_mutex graph_mx;
     std::condition_variable cv_work, cv_done;
     std::deque<Req> pending;
@@ -99,6 +99,7 @@ struct PyrhipServer {
     uint64_t resident_bytes = 0, reserved_bytes = 0;
     uint32_t in_flight = 0;
     double busy_s = 0;
+    char error[512] = {};  // message of the last failed batch or rejected request
     bool stop = false;
     std::vector<std::thread> workers;
 
@@ -155,6 +156,8 @@ struct PyrhipServer {
                 // when that build was rejected the edit has no edge either
                 auto c = chunks.find(r.chunk);
                 if (c == chunks.end() || !c->second.log2) {
+                    snprintf(error, sizeof(error), "edit of chunk %llu, whose build failed, has no log2_size",
+                             (unsigned long long)r.chunk);
                     complete_locked(r, PYR_ERROR_INVALID_ARGUMENT, 0, steady_s(), 0);
                     it = pending.erase(it);
                     cv_done.notify_all();
@@ -166,6 +169,8 @@ struct PyrhipServer {
             const bool alone_fits = rec <= info.max_batch_voxels && resident_bytes + e <= budget;
             if (!alone_fits && batch.empty() && in_flight == 0) {
                 // can never run: reject (the reference reports pool exhaustion as OUT_OF_MEMORY)
+                snprintf(error, sizeof(error), "request of %llu records on chunk %llu exceeds the batch limit or budget",
+                         (unsigned long long)rec, (unsigned long long)r.chunk);
                 complete_locked(r, PYR_ERROR_OUT_OF_MEMORY, 0, steady_s(), 0);
                 it = pending.erase(it);
                 cv_done.notify_all();
@@ -220,6 +225,23 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         offs[i + 1] = offs[i] + (uint32_t)r.records;
     }
     const uint64_t total = offs[k];
+    // a small batch replays the graph of its largest edge (outputs stay in the
+    // graph's buffers: copied below like the eager build's, never freed). A
+    // graph is captured while no other worker has device work in flight: the
+    // rest of every batch holds graph_mx shared.
+    PyrhipGraph* graph = nullptr;
+    if (graphs && !(info.flags & PYRHIP_SERVER_NO_GRAPHS) && k <= kGraphChunks && total <= kGraphVoxels) {
+        PyrhipGraph*& slot = (*graphs)[top];
+        if (!slot) {
+            std::unique_lock<std::shared_mutex> gl(graph_mx);
+            if (pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
+                slot = nullptr;  // not enough memory for it: build eagerly
+                diag_clear();
+            }
+        }
+        graph = slot;
+    }
+    std::shared_lock<std::shared_mutex> device_work(graph_mx);
     const uint64_t need = std::max<uint64_t>(total, 1) * sizeof(PyrVoxel) + (k + 1) * 4 + k + 256;
     if (*input_cap < need) {
         pool_free(*input, s);
@@ -276,30 +298,12 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
     (void)hipEventCreate(&e1);
     (void)hipEventRecord(e0, s);
     PyrhipBuilt b{};
-    // a small batch replays the graph of its largest edge (outputs stay in the
-    // graph's buffers: copied below like the eager build's, never freed)
-    PyrhipGraph* graph = nullptr;
-    if (graphs && !(info.flags & PYRHIP_SERVER_NO_GRAPHS) && k <= kGraphChunks && total <= kGraphVoxels) {
-        PyrhipGraph*& slot = (*graphs)[top];
-        if (!slot) {
-            std::unique_lock<std::shared_mutex> gl(graph_mx);
-            if (pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
-                slot = nullptr;  // not enough memory for it: build eagerly
-                diag_clear();
-            }
-        }
-        graph = slot;
-    }
     if (graph) {
         std::vector<uint32_t> po(kGraphChunks + 1, (uint32_t)total), gr(kGraphChunks), gf(kGraphChunks);
         std::vector<uint8_t> pl(kGraphChunks, 3);
         std::copy(offs.begin(), offs.end(), po.begin());
         std::copy(l2.begin(), l2.end(), pl.begin());
-        {
-            std::shared_lock<std::shared_mutex> gl(graph_mx);
-            r = pyrhip_graph_launch(graph, (uint64_t)(uintptr_t)vox, po.data(), pl.data(), s, &b, gr.data(),
-                                    gf.data());
-        }
+        r = pyrhip_graph_launch(graph, (uint64_t)(uintptr_t)vox, po.data(), pl.data(), s, &b, gr.data(), gf.data());
         std::copy(gr.begin(), gr.begin() + k, roots.begin());
         std::copy(gf.begin(), gf.begin() + k, first.begin());
     } else {
@@ -350,7 +354,12 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         }
     }
     float ms = 0;
-    if (r == PYR_OK && hipEventElapsedTime(&ms, e0, e1) == hipSuccess) *dev_ms = ms;
+    if (r == PYR_OK) {
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+            *dev_ms = ms;
+        else
+            (void)hipGetLastError();  // timing is best effort
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     *gen_out = g;
@@ -415,6 +424,7 @@ void PyrhipServer::worker_main() {
                 release_generation_locked(tmp, s);
             }
         } else {
+            snprintf(error, sizeof(error), "batch of %u requests: %s", (uint32_t)batch.size(), g_diag);
             for (const Req& q : batch) complete_locked(q, r, 0, t_start, (uint32_t)batch.size());
         }
         for (const Req& q : batch) busy.erase(q.chunk);
@@ -546,6 +556,14 @@ PYR_API void pyrhip_server_stats(PyrhipServer* sv, PyrhipServerStats* out) {
     out->pending = sv->pending.size();
     out->batched_voxels = sv->batched_voxels;
     out->busy_s = sv->busy_s;
+}
+
+PYR_API uint32_t pyrhip_server_error(PyrhipServer* sv, char* out, uint32_t size) {
+    if (!sv) return 0;
+    std::lock_guard<std::mutex> lk(sv->m);
+    const uint32_t n = (uint32_t)strlen(sv->error);
+    if (out && size) snprintf(out, size, "%s", sv->error);
+    return n;
 }
 
 PYR_API void pyrhip_server_destroy(PyrhipServer* sv) {

@@ -77,6 +77,12 @@ class Server:
         self.L.pyrhip_server_stats(self.h, C.byref(s))
         return {k: getattr(s, k) for k, _ in abi.PyrhipServerStats._fields_}
 
+    def error(self) -> str:
+        """message of the last failed batch or rejected request ("" if none)"""
+        buf = C.create_string_buffer(512)
+        self.L.pyrhip_server_error(self.h, buf, len(buf))
+        return buf.value.decode(errors="replace")
+
     def close(self) -> None:
         if self.h:
             self.L.pyrhip_server_destroy(self.h)

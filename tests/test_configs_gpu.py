@@ -308,11 +308,13 @@ def test_poisson_serving_80_20_same_alone_and_mixed(hip_device):
         alone.drain()
     ra = alone.poll()
     got_alone = arrays(alone)
+    err_alone = alone.error()
     alone.close()
 
     live = Server(workers=2)
     rl, _, nreq = poisson_run(live, dev, rate, duration, seed)
     got_live = arrays(live)
+    err_live = live.error()
     live.close()
 
     burst = Server(workers=2)
@@ -325,10 +327,13 @@ def test_poisson_serving_80_20_same_alone_and_mixed(hip_device):
     burst.drain()
     rb = burst.poll()
     got_burst = arrays(burst)
+    err_burst = burst.error()
     burst.close()
 
-    for res in (ra, rl, rb):
-        assert len(res) == len(sched) == nreq and all(c.result == 0 for c in res)
+    for name, res, err in (("alone", ra, err_alone), ("live", rl, err_live), ("burst", rb, err_burst)):
+        bad = [(c.chunk, c.result, c.batch_chunks) for c in res if c.result != 0]
+        assert len(res) == len(sched) == nreq and not bad, (name, bad[:8], err)
+        assert err == "", (name, err)
     assert max(c.batch_chunks for c in rb) > 1, "the burst was never batched"
     for c, pts in expect.items():
         cpu = O.build_points(inputs[c][0], pts)

@@ -98,6 +98,7 @@ def test_same_alone_and_in_the_mix(hip_device, graphs):
         assert np.array_equal(aa, ref.attrs) and np.array_equal(am, ref.attrs), c
     st = mixed.stats()
     assert st["completed"] == len(script) and st["rejected"] == 0
+    assert alone.error() == "" and mixed.error() == ""
     alone.close()
     mixed.close()
 
@@ -148,4 +149,6 @@ def test_admission_rejects_what_cannot_fit(hip_device):
     res = {c.ticket: c for c in sv.poll()}
     assert res[t_big].result == -3  # PYR_ERROR_OUT_OF_MEMORY
     assert res[t_small].result == 0 and res[t_small].voxel_count == len(final_points(small, 5))
+    # the rejection's message stays readable on the server (the worker thread failed it)
+    assert sv.error() == f"request of {len(pts)} records on chunk 1 exceeds the batch limit or budget"
     sv.close()
