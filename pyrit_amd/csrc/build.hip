@@ -1290,6 +1290,11 @@ __device__ __forceinline__ void owner_min(uint32_t* owner, uint64_t h, uint32_t 
 // terrain makes one mask (the full brick) dominate; lanes of a wave holding
 // the same mask as the wave's first lane reuse its slot, and slots are read
 // before any CAS, so a hot mask costs loads instead of serialised atomics.
+// The wave also keeps the mask and slot of its previous row: a lane holding
+// that mask takes the slot without touching the table. Its brick index is
+// larger than the one of the earlier lane that competed for the owner, so
+// skipping the owner update loses nothing. (The device-coherent reads of one
+// hot slot by every wave of the grid serialise at one memory channel.)
 __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, const uint64_t* masks, uint64_t* table,
                                                          uint32_t* owner, uint32_t* slot, uint32_t* ch_count,
                                                          uint64_t max_cap, uint32_t brick_cap, uint32_t load_pct) {
@@ -1297,13 +1302,15 @@ __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, cons
     const uint64_t want = hash_cap_for(n, load_pct);
     const uint64_t cap = want < max_cap ? want : max_cap, m = cap - 1;
     const uint32_t stride = gridDim.x * kBlock;
+    uint64_t c_mask = 0, c_slot = 0;  // previous row's first mask and slot (masks are non-zero)
     for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n; b += stride) {
         const uint64_t mask = masks[b];
         const uint64_t m0 = first_lane64(mask);
         const uint32_t b0 = first_lane32(b);
         const bool follower = mask == m0 && b != b0;
-        uint64_t h = mix64(mask) & m;
-        if (!follower) {
+        const bool cached = mask == c_mask;
+        uint64_t h = cached ? c_slot : mix64(mask) & m;
+        if (!follower && !cached) {
             uint64_t probes = 0;
             for (;;) {
                 uint64_t cur = load_agent(&table[h]);
@@ -1321,6 +1328,8 @@ __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, cons
         if (follower) h = h0;  // b0 < b already competes for the owner
         slot[b] = (uint32_t)h;
         ch_count[b] = (uint32_t)__popcll(mask);
+        c_mask = m0;
+        c_slot = h0;
     }
 }
 
@@ -1521,13 +1530,16 @@ __global__ void __launch_bounds__(kBlock) k_level_build(ScanCtx sc, LevelArgs a)
 // node_insert/node_owner (gbuild.zig:527-554): the tuples written by
 // k_level_build (complete at this launch) are inserted into the level's hash
 // as index + 1; a collision compares the stored tuple word by word. Hot tuples
-// (identical full nodes of solid regions) are handled like hot leaf masks.
+// (identical full nodes of solid regions) are handled like hot leaf masks,
+// the previous row's first tuple and slot included.
 __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, LevelArgs a) {
     const uint32_t np = min(counters[C_NCH + a.lv + 1], a.par_cap);
     const uint32_t nch = min(counters[C_NCH + a.lv], a.ch_cap);
     const uint64_t want = hash_cap_for(nch, a.load_pct);
     const uint64_t cap = want < a.table_cap ? want : a.table_cap;
     const uint32_t stride = gridDim.x * kBlock;
+    uint32_t c_len = ~0u, c_w[kNodeStride] = {};  // previous row's first tuple (lengths are <= 8)
+    uint64_t c_slot = 0;
     for (uint32_t p = blockIdx.x * kBlock + threadIdx.x; p < np; p += stride) {
         const uint32_t len = a.node_len[p];
         const uint32_t* src = a.node_tmp + (uint64_t)p * kNodeStride;
@@ -1540,11 +1552,18 @@ __global__ void __launch_bounds__(kBlock) k_level_insert(uint32_t* counters, Lev
 #pragma unroll
         for (int q = 0; q < kNodeStride; q++) same0 &= first_lane32(w[q]) == w[q];
         const bool follower = same0 && p != p0;
-        uint64_t h = 0;
-        if (!follower) h = level_insert_one(a, p, w, len, cap, &counters[C_ERR]);
+        bool cached = len == c_len;
+#pragma unroll
+        for (int q = 0; q < kNodeStride; q++) cached &= c_w[q] == w[q];
+        uint64_t h = c_slot;
+        if (!follower && !cached) h = level_insert_one(a, p, w, len, cap, &counters[C_ERR]);
         const uint64_t h0 = first_lane64(h);  // the slot of the wave's first active lane
         if (follower) h = h0;
         a.slot[p] = (uint32_t)h;
+        c_len = first_lane32(len);
+#pragma unroll
+        for (int q = 0; q < kNodeStride; q++) c_w[q] = first_lane32(w[q]);
+        c_slot = h0;
     }
 }
 
