@@ -1295,6 +1295,14 @@ __device__ __forceinline__ void owner_min(uint32_t* owner, uint64_t h, uint32_t 
 // larger than the one of the earlier lane that competed for the owner, so
 // skipping the owner update loses nothing. (The device-coherent reads of one
 // hot slot by every wave of the grid serialise at one memory channel.)
+// Workgroups of the hash inserts. Every wave's first row makes the hot key's
+// device-scope CAS and owner atomic (later rows reuse the slot), and atomics
+// on one address are serialised, so fewer waves, each with more rows: leaf
+// insert of the 1024^3 grid 0.32 ms at 4096 workgroups, 0.31 at 2048, 0.22 at
+// 1024, 0.17 at 512 and 0.16 at 256; of the 1024-chunk batch 0.43 / 0.30 /
+// 0.25 / 0.28 ms at 2048 / 1024 / 512 / 256.
+constexpr uint32_t kInsertBlocks = 512;
+
 __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, const uint64_t* masks, uint64_t* table,
                                                          uint32_t* owner, uint32_t* slot, uint32_t* ch_count,
                                                          uint64_t max_cap, uint32_t brick_cap, uint32_t load_pct) {
@@ -1303,8 +1311,11 @@ __global__ void __launch_bounds__(kBlock) k_leaf_insert(uint32_t* counters, cons
     const uint64_t cap = want < max_cap ? want : max_cap, m = cap - 1;
     const uint32_t stride = gridDim.x * kBlock;
     uint64_t c_mask = 0, c_slot = 0;  // previous row's first mask and slot (masks are non-zero)
-    for (uint32_t b = blockIdx.x * kBlock + threadIdx.x; b < n; b += stride) {
-        const uint64_t mask = masks[b];
+    uint32_t b = blockIdx.x * kBlock + threadIdx.x;
+    uint64_t next = b < n ? masks[b] : 0;
+    for (; b < n; b += stride) {
+        const uint64_t mask = next;
+        if (b + stride < n) next = masks[b + stride];  // the next row's load overlaps this row's probes
         const uint64_t m0 = first_lane64(mask);
         const uint32_t b0 = first_lane32(b);
         const bool follower = mask == m0 && b != b0;
@@ -2336,7 +2347,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
     const uint32_t brick_grid = grid_for(lz.nb, kScanTile, 2048);
     hipLaunchKernelGGL(k_hash_clear, dim3(grid_for(lz.cap, 1024, 2048)), dim3(kBlock), 0, s, counters, (uint32_t)C_B,
                        (void*)table, 8u, owner, lz.cap, load_pct);
-    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(lz.nb, kBlock * 4, 4096)), dim3(kBlock), 0, s, counters, masks,
+    hipLaunchKernelGGL(k_leaf_insert, dim3(grid_for(lz.nb, kBlock * 4, kInsertBlocks)), dim3(kBlock), 0, s, counters, masks,
                        table, owner, slot, set[0].count, lz.cap, (uint32_t)lz.nb, load_pct);
     {
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
@@ -2393,7 +2404,7 @@ PyrResult run_build(const EncodeArgs& enc_in, uint32_t log2_size, const PyrhipBu
                            counters, (uint32_t)(C_NCH + lv), (void*)table, 4u, owner, lz.cap, load_pct);
         ScanCtx sc{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_build, dim3(g), dim3(kBlock), 0, s, sc, la);
-        hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(lz.nb, cells(lv + 1)), kBlock * 4, 4096)),
+        hipLaunchKernelGGL(k_level_insert, dim3(grid_for(std::min(lz.nb, cells(lv + 1)), kBlock * 4, kInsertBlocks)),
                            dim3(kBlock), 0, s, counters, la);
         ScanCtx sc2{status, ++epoch, tile_ctr(), counters};
         hipLaunchKernelGGL(k_level_compact, dim3(grid_for(std::min(lz.nb, cells(lv + 1)), kScanTile, 2048)),
