@@ -352,9 +352,11 @@ typedef struct PyrhipServerInfo {
 } PyrhipServerInfo;
 
 /* Build every batch with pyrhip_build_chunks. Without it, a batch of at most
- * 64 chunks and 2^20 records replays a HIP graph (pyrhip_graph_*, one per
- * worker and largest chunk edge, up to about 100 MB each, made on first use and kept
- * until pyrhip_server_destroy; not counted in memory_budget). */
+ * 64 chunks, 2^20 records and edges up to 2^8 replays a HIP graph
+ * (pyrhip_graph_*, one per worker and largest edge 2^3..2^8, up to about 100 MB each, captured by
+ * pyrhip_server_create and kept until pyrhip_server_destroy; not counted in
+ * memory_budget). No capture runs while the server serves: HIP fails
+ * legacy-stream calls of every thread of the process during a capture. */
 #define PYRHIP_SERVER_NO_GRAPHS 1u
 
 #define PYRHIP_REQUEST_BUILD 0u /* (re)build a chunk from voxel records (device memory) */

@@ -12,8 +12,9 @@
 // (mixed chunk edges); copy the outputs into exact-size buffers (a
 // "generation" shared by the batch's chunks, freed when its last chunk is
 // replaced); complete the requests. A small batch (at most kGraphChunks
-// chunks and kGraphVoxels records) replays a HIP graph of the batch build
-// instead (one per worker and largest edge), padded with empty chunks.
+// chunks and kGraphVoxels records, edges up to 2^kGraphEdge) replays a HIP
+// graph of the batch build for its largest edge instead (one per worker and
+// edge, captured when the server is created), padded with empty chunks.
 #include <hip/hip_runtime.h>
 
 #include <stdint.h>
@@ -25,7 +26,6 @@
 #include <deque>
 #include <memory>
 #include <mutex>
-#include <shared_mutex>
 #include <thread>
 #include <unordered_map>
 #include <unordered_set>
@@ -39,7 +39,15 @@ namespace {
 constexpr uint32_t kGraphChunks = 64;
 constexpr uint32_t kGraphVoxels = 1u << 20;
 
-using GraphCache = std::unordered_map<uint32_t, PyrhipGraph*>;  // largest chunk edge -> graph
+// Graphs are captured by pyrhip_server_create, before the workers start, and
+// never while serving: during a capture HIP fails the legacy-stream calls of
+// every other thread of the process ("operation would make the legacy stream
+// depend on a capturing blocking stream"), the caller's uploads included.
+// A replay costs more the larger the graph's edge (one graph of edge 2^8 for
+// every small batch: p50 0.67 ms against 0.36 ms with one graph per largest
+// edge on the 2000 requests/s workload), so each edge up to 2^kGraphEdge has
+// its own.
+constexpr uint32_t kGraphEdge = 8;
 
 double steady_s() {
     using namespace std::chrono;
@@ -88,8 +96,7 @@ struct PyrhipServer {
     PyrhipServerInfo info{};
     int device = 0;
     std::mutex m;
-    // graph captures (held exclusively) against every batch's device work (held shared)
-    std::shared_mutex graph_mx;
+    std::vector<PyrhipGraph*> graphs;  // [worker * (kGraphEdge + 1) + edge] (null: none)
     std::condition_variable cv_work, cv_done;
     std::deque<Req> pending;
     std::unordered_map<uint64_t, ChunkState> chunks;
@@ -196,15 +203,15 @@ struct PyrhipServer {
         return batch;
     }
 
-    void worker_main();
+    void worker_main(uint32_t w);
     PyrResult run_batch(std::vector<Req>& batch, hipStream_t s, void** input, uint64_t* input_cap, double* dev_ms,
                         std::shared_ptr<Generation>* gen, std::vector<uint32_t>& roots, std::vector<uint32_t>& first,
-                        std::vector<uint32_t>& counts, GraphCache* graphs);
+                        std::vector<uint32_t>& counts, PyrhipGraph* const* graphs);
 };
 
 PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void** input, uint64_t* input_cap,
                                   double* dev_ms, std::shared_ptr<Generation>* gen_out, std::vector<uint32_t>& roots,
-                                  std::vector<uint32_t>& first, std::vector<uint32_t>& counts, GraphCache* graphs) {
+                                  std::vector<uint32_t>& first, std::vector<uint32_t>& counts, PyrhipGraph* const* graphs) {
     const uint32_t k = (uint32_t)batch.size();
     std::vector<uint32_t> offs(k + 1, 0);
     std::vector<uint8_t> l2(k, 0);
@@ -225,23 +232,9 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
         offs[i + 1] = offs[i] + (uint32_t)r.records;
     }
     const uint64_t total = offs[k];
-    // a small batch replays the graph of its largest edge (outputs stay in the
-    // graph's buffers: copied below like the eager build's, never freed). A
-    // graph is captured while no other worker has device work in flight: the
-    // rest of every batch holds graph_mx shared.
-    PyrhipGraph* graph = nullptr;
-    if (graphs && !(info.flags & PYRHIP_SERVER_NO_GRAPHS) && k <= kGraphChunks && total <= kGraphVoxels) {
-        PyrhipGraph*& slot = (*graphs)[top];
-        if (!slot) {
-            std::unique_lock<std::shared_mutex> gl(graph_mx);
-            if (pyrhip_graph_create(top, kGraphChunks, 0, kGraphVoxels, 1, &info.tuning, &slot) != PYR_OK) {
-                slot = nullptr;  // not enough memory for it: build eagerly
-                diag_clear();
-            }
-        }
-        graph = slot;
-    }
-    std::shared_lock<std::shared_mutex> device_work(graph_mx);
+    // a small batch replays the worker's graph of its largest edge (outputs
+    // stay in the graph's buffers: copied below like the eager build's, never freed)
+    PyrhipGraph* graph = k <= kGraphChunks && total <= kGraphVoxels && top <= kGraphEdge ? graphs[top] : nullptr;
     const uint64_t need = std::max<uint64_t>(total, 1) * sizeof(PyrVoxel) + (k + 1) * 4 + k + 256;
     if (*input_cap < need) {
         pool_free(*input, s);
@@ -314,7 +307,7 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
     // upper bounds, and the generation stays resident
     auto g = std::make_shared<Generation>();
     if (r == PYR_OK) {
-        g->log2_size = top;
+        g->log2_size = b.log2_size;  // the graph's edge when replayed: its keys carry that layout
         g->node_words = b.node_words;
         g->leaf_count = b.leaf_count;
         const uint64_t nb = (uint64_t)b.node_words * 4, lb = (uint64_t)b.leaf_count * 8;
@@ -366,13 +359,12 @@ PyrResult PyrhipServer::run_batch(std::vector<Req>& batch, hipStream_t s, void**
     return r;
 }
 
-void PyrhipServer::worker_main() {
+void PyrhipServer::worker_main(uint32_t w) {
     (void)hipSetDevice(device);
     hipStream_t s = nullptr;
     (void)hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
     void* input = nullptr;
     uint64_t input_cap = 0;
-    GraphCache graphs;
     std::unique_lock<std::mutex> lk(m);
     for (;;) {
         std::vector<Req> batch;
@@ -388,7 +380,8 @@ void PyrhipServer::worker_main() {
         std::shared_ptr<Generation> g;
         std::vector<uint32_t> roots, first, counts;
         double dev_ms = 0;
-        PyrResult r = run_batch(batch, s, &input, &input_cap, &dev_ms, &g, roots, first, counts, &graphs);
+        PyrResult r = run_batch(batch, s, &input, &input_cap, &dev_ms, &g, roots, first, counts,
+                                graphs.data() + (size_t)w * (kGraphEdge + 1));
         lk.lock();
         uint64_t est = 0;
         for (const Req& q : batch) est += q.est;
@@ -434,10 +427,6 @@ void PyrhipServer::worker_main() {
     lk.unlock();
     pool_free(input, s);
     (void)hipStreamSynchronize(s);
-    {
-        std::unique_lock<std::shared_mutex> gl(graph_mx);
-        for (auto& kv : graphs) pyrhip_graph_destroy(kv.second);
-    }
     (void)hipStreamDestroy(s);
 }
 
@@ -460,7 +449,17 @@ PYR_API PyrResult pyrhip_server_create(const PyrhipServerInfo* info, PyrhipServe
     if (!sv->info.max_batch_voxels) sv->info.max_batch_voxels = 1u << 28;
     if (!sv->info.workers) sv->info.workers = 2;
     (void)hipGetDevice(&sv->device);
-    for (uint32_t w = 0; w < sv->info.workers; w++) sv->workers.emplace_back([sv] { sv->worker_main(); });
+    sv->graphs.assign((size_t)sv->info.workers * (kGraphEdge + 1), nullptr);
+    if (!(sv->info.flags & PYRHIP_SERVER_NO_GRAPHS))
+        for (uint32_t w = 0; w < sv->info.workers; w++)
+            for (uint32_t e = 3; e <= kGraphEdge; e++) {
+                PyrhipGraph*& g = sv->graphs[(size_t)w * (kGraphEdge + 1) + e];
+                if (pyrhip_graph_create(e, kGraphChunks, 0, kGraphVoxels, 1, &sv->info.tuning, &g) != PYR_OK) {
+                    g = nullptr;  // not enough memory for it: batches of that edge are built eagerly
+                    diag_clear();
+                }
+            }
+    for (uint32_t w = 0; w < sv->info.workers; w++) sv->workers.emplace_back([sv, w] { sv->worker_main(w); });
     *out = sv;
     return PYR_OK;
 }
@@ -574,6 +573,7 @@ PYR_API void pyrhip_server_destroy(PyrhipServer* sv) {
         sv->cv_work.notify_all();
     }
     for (std::thread& t : sv->workers) t.join();
+    for (PyrhipGraph* g : sv->graphs) pyrhip_graph_destroy(g);
     for (auto& kv : sv->chunks) sv->release_generation_locked(kv.second.gen, nullptr);
     (void)hipDeviceSynchronize();
     delete sv;
